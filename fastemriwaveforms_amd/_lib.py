@@ -1,0 +1,76 @@
+"""ctypes binding of libfew_rocm.so (the HIP kernels behind the C ABI in include/few_rocm.h).
+
+There is no CPU fallback: if the library is missing or cannot be loaded, every
+entry point raises :class:`BackendNotInstalled`.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .exceptions import BackendNotInstalled, FewRocmError
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libfew_rocm.so")
+
+FEW_PRECISION_FP64 = 0
+FEW_PRECISION_FP64_FACT = 1
+FEW_PRECISION_MIXED = 2
+
+_lib = None
+
+_d = ctypes.c_void_p  # every pointer travels as an untyped address
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    c_int, c_long, c_double, c_size = ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_size_t
+    sig = {
+        "few_last_error": ([], ctypes.c_char_p),
+        "few_abi_version": ([], c_int),
+        "few_interpolate_arrays": ([_d, _d, c_int, c_int, _d, _d, _d, _d, _d], c_int),
+        "few_get_waveform": ([_d, _d, _d, _d, _d, _d, _d, c_int, c_int, c_int, _d, c_double, _d,
+                              c_int, _d], c_int),
+        "few_get_waveform_async": ([_d, _d, _d, _d, _d, _d, _d, c_int, c_int, c_int, _d, c_double,
+                                    _d, c_int, _d, c_size, _d], c_int),
+        "few_waveform_workspace_bytes": ([c_int, c_int, c_int, c_int], c_size),
+        "few_spline_build_async": ([_d, c_long, c_long, _d, c_long, c_long, _d, c_long, c_long,
+                                    c_long, c_int, c_int, c_int, _d, _d], c_int),
+        "few_prepare_modes_async": ([_d, _d, _d, c_int, c_int, _d, _d, _d, _d], c_int),
+        "few_release_workspace": ([], c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+
+
+def lib() -> ctypes.CDLL:
+    """Load libfew_rocm.so once; raise BackendNotInstalled when it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BackendNotInstalled(
+                f"{LIB_PATH} is not built; run `python -m fastemriwaveforms_amd._build` "
+                "(hipcc, gfx950) first")
+        try:
+            handle = ctypes.CDLL(LIB_PATH)
+        except OSError as e:
+            raise BackendNotInstalled(f"cannot load {LIB_PATH}: {e}") from e
+        _declare(handle)
+        _lib = handle
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().few_last_error().decode(errors="replace")
+        raise FewRocmError(f"{what} failed (code {rc}): {msg}")
+
+
+def exported_symbols() -> list[str]:
+    return [
+        "few_last_error", "few_abi_version", "few_interpolate_arrays", "few_get_waveform",
+        "few_get_waveform_async", "few_waveform_workspace_bytes", "few_spline_build_async",
+        "few_prepare_modes_async", "few_release_workspace",
+    ]

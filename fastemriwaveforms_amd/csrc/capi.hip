@@ -1,0 +1,332 @@
+// C ABI of the MI355X interpolated mode-summation library.
+//
+// The entry points mirror the reference's native interface
+// (src/few/cutils/interpolate.hh:27-38, bound at src/few/cutils/pyinterp.pyx:10-16)
+// with plain pointers and sizes, an explicit HIP stream, and an int status
+// instead of exit() on error (the reference aborts the process through
+// gpuErrchk / CUSPARSE_CALL, src/few/cutils/global.h:49-57,
+// src/few/cutils/interpolate.hh:12-23).  The message of the last failing
+// call on the calling thread is available from few_last_error().
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "few_common.hpp"
+#include "few_rocm.h"
+#include "modesum_direct.hpp"
+
+namespace few {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int check_hip(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return FEW_OK;
+    set_error("%s failed: %s", what, hipGetErrorString(e));
+    return FEW_ERR_HIP;
+}
+
+// Library-owned device workspace for the synchronous drop-in entry points
+// (one per device, grown on demand, kept until few_release_workspace()).
+struct DeviceWorkspace {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+static std::mutex g_ws_mu;
+static DeviceWorkspace g_ws[64];
+
+static int library_workspace(int dev, size_t bytes, void **out)
+{
+    if (dev < 0 || dev >= 64) {
+        set_error("device index %d out of range", dev);
+        return FEW_ERR_ARG;
+    }
+    DeviceWorkspace &w = g_ws[dev];
+    if (w.bytes < bytes) {
+        if (w.ptr) {
+            FEW_HIP(hipDeviceSynchronize());
+            FEW_HIP(hipFree(w.ptr));
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+        size_t nb = bytes < (1u << 20) ? (1u << 20) : bytes;
+        FEW_HIP(hipMalloc(&w.ptr, nb));
+        w.bytes = nb;
+    }
+    *out = w.ptr;
+    return FEW_OK;
+}
+
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Layout of the mode-sum planning workspace.
+struct PlanLayout {
+    size_t h_t, pt, segs, prefix, total;
+};
+static PlanLayout plan_layout(int L)
+{
+    PlanLayout p;
+    size_t off = 0;
+    p.h_t = off; off = align256(off + sizeof(double) * (size_t)L);
+    p.pt = off; off = align256(off + sizeof(double) * (size_t)L);
+    p.segs = off; off = align256(off + sizeof(Segment) * (size_t)(L > 1 ? L - 1 : 1));
+    p.prefix = off; off = align256(off + sizeof(int) * (size_t)L);
+    p.total = off;
+    return p;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    int ok = FEW_OK;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (dev >= 0 && dev != prev) ok = check_hip(hipSetDevice(dev), "hipSetDevice");
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+static int check_mode_args(const void *wave, const double *interp, const double *pcoef, const int *m,
+                           const int *k, const int *n, int init_len, int out_len, int K,
+                           const void *ylm, double dt)
+{
+    if (!wave || !interp || !pcoef || !m || !k || !n || !ylm) {
+        set_error("get_waveform: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (init_len < 2 || out_len < 0 || K < 0 || !(dt > 0.0)) {
+        set_error("get_waveform: invalid sizes (init_len=%d out_len=%d modes=%d dt=%g)", init_len,
+                  out_len, K, dt);
+        return FEW_ERR_ARG;
+    }
+    return FEW_OK;
+}
+
+// Plans (device) and runs one waveform's mode sum on `stream`; h_t and pt
+// are device pointers.
+static int run_waveform(double2 *wave, const double *interp, const double *pt, const double *pcoef,
+                        const int *m, const int *k, const int *n, int L, int out_len, int K,
+                        const double2 *ylm, double dt, const double *h_t, int precision, char *ws,
+                        size_t ws_bytes, hipStream_t stream)
+{
+    if (out_len == 0 || K == 0) return FEW_OK;
+    const PlanLayout pl = plan_layout(L);
+    if (ws_bytes < pl.total) {
+        set_error("get_waveform: workspace of %zu bytes, need %zu", ws_bytes, pl.total);
+        return FEW_ERR_ARG;
+    }
+    Segment *segs = reinterpret_cast<Segment *>(ws + pl.segs);
+    int *prefix = reinterpret_cast<int *>(ws + pl.prefix);
+    if (precision != FEW_PRECISION_FP64) {
+        set_error("get_waveform: precision %d is not available in this build", precision);
+        return FEW_ERR_UNSUPPORTED;
+    }
+    const int tile = direct_tile(precision);
+    PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
+    int rc = launch_plan(p, stream);
+    if (rc) return rc;
+    DirectArgs a;
+    a.wave = wave;
+    a.interp = interp;
+    a.pcoef = pcoef;
+    a.m = m;
+    a.k = k;
+    a.n = n;
+    a.ylm = ylm;
+    a.segs = segs;
+    a.prefix = prefix;
+    a.nseg = L - 1;
+    a.L = L;
+    a.K = K;
+    a.dt = dt;
+    const long max_blocks = ((long)out_len + tile - 1) / tile + (L - 1);
+    if (max_blocks > 0x7fffffffL) {
+        set_error("get_waveform: output too long (%d samples)", out_len);
+        return FEW_ERR_LIMIT;
+    }
+    return launch_direct_fp64(a, (int)max_blocks, stream);
+}
+
+}  // namespace few
+
+using namespace few;
+
+extern "C" {
+
+const char *few_last_error(void) { return g_err; }
+
+int few_abi_version(void) { return FEW_ROCM_ABI_VERSION; }
+
+int few_interpolate_arrays(double *t_arr, double *interp_array, int ninterps, int length, double *B,
+                           double *upper_diag, double *diag, double *lower_diag, void *stream)
+{
+    if (!t_arr || !interp_array || !B || !upper_diag || !diag || !lower_diag) {
+        set_error("interpolate_arrays: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (ninterps <= 0 || length < 3) {
+        set_error("interpolate_arrays: need ninterps > 0 and length >= 3 (got %d, %d)", ninterps,
+                  length);
+        return FEW_ERR_ARG;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    SplineGroup g;
+    g.y = interp_array;
+    g.y_sys = length;
+    g.y_knot = 1;
+    g.t = t_arr;
+    g.t_sys = length;
+    g.t_knot = 1;
+    g.out = interp_array;
+    g.o_coef = (long)ninterps * length;
+    g.o_sys = length;
+    g.o_knot = 1;
+    g.nsys = ninterps;
+    g.dense = 0;
+    double *scratch[4] = {diag, upper_diag, lower_diag, B};
+    int rc = launch_spline_build(&g, 1, length, scratch, nullptr, s);
+    if (rc) return rc;
+    FEW_HIP(hipStreamSynchronize(s));
+    return FEW_OK;
+}
+
+int few_spline_build_async(const double *y, long y_sys, long y_knot, const double *t, long t_sys,
+                           long t_knot, double *out, long o_coef, long o_sys, long o_knot, int nsys,
+                           int length, int dense, double *scratch, void *stream)
+{
+    if (!y || !t || !out || !scratch) {
+        set_error("spline_build: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (nsys <= 0 || length < 3) {
+        set_error("spline_build: need nsys > 0 and length >= 3 (got %d, %d)", nsys, length);
+        return FEW_ERR_ARG;
+    }
+    SplineGroup g{y, y_sys, y_knot, t, t_sys, t_knot, out, o_coef, o_sys, o_knot, nsys, dense ? 1 : 0};
+    const long f = (long)length * nsys;
+    double *sc[4] = {scratch, scratch + f, scratch + 2 * f, scratch + 3 * f};
+    return launch_spline_build(&g, 1, length, sc, nullptr, reinterpret_cast<hipStream_t>(stream));
+}
+
+int few_prepare_modes_async(const double *teuk, const double *phase_knots, const double *t,
+                            int length, int num_modes, double *interp_out, double *pcoef_out,
+                            double *scratch, void *stream)
+{
+    if (!teuk || !t || !interp_out || !scratch || (phase_knots && !pcoef_out)) {
+        set_error("prepare_modes: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (num_modes <= 0 || length < 3) {
+        set_error("prepare_modes: need modes > 0 and length >= 3 (got %d, %d)", num_modes, length);
+        return FEW_ERR_ARG;
+    }
+    const long K = num_modes, L = length;
+    SplineGroup g[3];
+    // Re parts of teuk (L, K) complex -> interp arrays [0, K)
+    g[0] = SplineGroup{teuk, 2, 2 * K, t, 0, 1, interp_out, L * 2 * K, 1, 2 * K, num_modes, 0};
+    // Im parts -> interp arrays [K, 2K)
+    g[1] = SplineGroup{teuk + 1, 2, 2 * K, t, 0, 1, interp_out + K, L * 2 * K, 1, 2 * K, num_modes, 0};
+    // phase knots (3, L) -> dense-output coefficients (8, L-1, 3)
+    g[2] = SplineGroup{phase_knots, L, 1, t, 0, 1, pcoef_out, 0, 0, 0, 3, 1};
+    const int ng = phase_knots ? 3 : 2;
+    const long f = L * (2 * K + (phase_knots ? 3 : 0));
+    double *sc[4] = {scratch, scratch + f, scratch + 2 * f, scratch + 3 * f};
+    return launch_spline_build(g, ng, length, sc, nullptr, reinterpret_cast<hipStream_t>(stream));
+}
+
+size_t few_waveform_workspace_bytes(int init_len, int out_len, int num_modes, int precision)
+{
+    (void)out_len;
+    (void)num_modes;
+    (void)precision;
+    return plan_layout(init_len < 2 ? 2 : init_len).total;
+}
+
+int few_get_waveform(void *d_waveform, double *interp_array, double *phase_interp_t,
+                     double *phase_interp_coeffs, int *d_m, int *d_k, int *d_n, int init_len,
+                     int out_len, int num_teuk_modes, void *d_Ylms, double delta_t, double *h_t,
+                     int dev, void *stream)
+{
+    int rc = check_mode_args(d_waveform, interp_array, phase_interp_coeffs, d_m, d_k, d_n, init_len,
+                             out_len, num_teuk_modes, d_Ylms, delta_t);
+    if (rc) return rc;
+    if (!phase_interp_t || !h_t) {
+        set_error("get_waveform: null host time array");
+        return FEW_ERR_ARG;
+    }
+    DeviceGuard guard(dev);
+    if (guard.ok) return guard.ok;
+    int cur = 0;
+    FEW_HIP(hipGetDevice(&cur));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    const PlanLayout pl = plan_layout(init_len);
+    void *ws = nullptr;
+    rc = library_workspace(cur, pl.total, &ws);
+    if (rc) return rc;
+    char *w = static_cast<char *>(ws);
+    // h_t and phase_interp_t are host arrays in the reference interface
+    // (interpolate.cu:739, :767-768); stage them on the device.
+    FEW_HIP(hipMemcpyAsync(w + pl.h_t, h_t, sizeof(double) * init_len, hipMemcpyHostToDevice, s));
+    FEW_HIP(hipMemcpyAsync(w + pl.pt, phase_interp_t, sizeof(double) * init_len,
+                           hipMemcpyHostToDevice, s));
+    rc = run_waveform(static_cast<double2 *>(d_waveform), interp_array,
+                      reinterpret_cast<double *>(w + pl.pt), phase_interp_coeffs, d_m, d_k, d_n,
+                      init_len, out_len, num_teuk_modes, static_cast<const double2 *>(d_Ylms),
+                      delta_t, reinterpret_cast<double *>(w + pl.h_t), FEW_PRECISION_FP64, w,
+                      pl.total, s);
+    if (rc) return rc;
+    FEW_HIP(hipStreamSynchronize(s));
+    return FEW_OK;
+}
+
+int few_get_waveform_async(void *d_waveform, const double *interp_array, const double *d_phase_interp_t,
+                           const double *phase_interp_coeffs, const int *d_m, const int *d_k,
+                           const int *d_n, int init_len, int out_len, int num_teuk_modes,
+                           const void *d_Ylms, double delta_t, const double *d_h_t, int precision,
+                           void *workspace, size_t workspace_bytes, void *stream)
+{
+    int rc = check_mode_args(d_waveform, interp_array, phase_interp_coeffs, d_m, d_k, d_n, init_len,
+                             out_len, num_teuk_modes, d_Ylms, delta_t);
+    if (rc) return rc;
+    if (!d_phase_interp_t || !d_h_t || !workspace) {
+        set_error("get_waveform_async: null device time array or workspace");
+        return FEW_ERR_ARG;
+    }
+    char *w = static_cast<char *>(workspace);
+    return run_waveform(static_cast<double2 *>(d_waveform), interp_array, d_phase_interp_t,
+                        phase_interp_coeffs, d_m, d_k, d_n, init_len, out_len, num_teuk_modes,
+                        static_cast<const double2 *>(d_Ylms), delta_t, d_h_t, precision, w,
+                        workspace_bytes, reinterpret_cast<hipStream_t>(stream));
+}
+
+int few_release_workspace(void)
+{
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    for (auto &w : g_ws) {
+        if (w.ptr) {
+            FEW_HIP(hipFree(w.ptr));
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+    }
+    return FEW_OK;
+}
+
+}  // extern "C"

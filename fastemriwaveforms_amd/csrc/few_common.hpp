@@ -1,0 +1,56 @@
+// Shared definitions of the MI355X (gfx950) interpolated mode-summation library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FEW_OK 0
+#define FEW_ERR_ARG 1
+#define FEW_ERR_HIP 2
+#define FEW_ERR_SINGULAR 3
+#define FEW_ERR_UNSUPPORTED 4
+#define FEW_ERR_LIMIT 5
+
+namespace few {
+
+// Thread-local error message kept for the most recent failing call.
+void set_error(const char *fmt, ...);
+
+// Records the HIP error (if any) and returns FEW_ERR_HIP, else FEW_OK.
+int check_hip(hipError_t e, const char *what);
+
+#define FEW_HIP(call)                                   \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return ::few::check_hip(e_, #call); \
+    } while (0)
+
+// One group of cubic-spline systems sharing a length; see spline.hip.
+struct SplineGroup {
+    const double *y;     // knot values: y[s*y_sys + i*y_knot]
+    long y_sys, y_knot;
+    const double *t;     // knot times: t[s*t_sys + i*t_knot] (t_sys = 0: shared row)
+    long t_sys, t_knot;
+    double *out;         // cubic: out[c*o_coef + s*o_sys + i*o_knot], c = 0..3
+    long o_coef, o_sys, o_knot;
+    int nsys;
+    int dense;           // 1: write the 8-coefficient dense-output layout
+                         //    out[c*(length-1)*nsys + j*nsys + s] instead
+};
+
+// Launches the fused not-a-knot build for up to two groups of systems of a
+// common length.  scratch holds 4*length*(nsys0+nsys1) doubles.
+int launch_spline_build(const SplineGroup *groups, int ngroups, int length, double *scratch4[4],
+                        int *info_out, hipStream_t stream);
+
+// Per-interval description of the output samples it owns, shared by the
+// mode-sum kernels (filled on the host or by a device planning kernel).
+struct Segment {
+    int start;        // first sample index (inclusive)
+    int end;          // last sample index (exclusive)
+    double h_t;       // amplitude-spline knot time of the interval
+    double pt;        // phase dense-output segment start
+    double pw;        // phase dense-output segment width
+};
+
+}  // namespace few

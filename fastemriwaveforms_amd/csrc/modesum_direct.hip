@@ -1,0 +1,225 @@
+// Direct (per-mode) interpolated mode summation for gfx950.
+//
+// Two kernels:
+//   * plan_segments_kernel -- the interval -> output-sample map of
+//     find_start_inds (src/few/cutils/interpolate.cu:694-723) computed on the
+//     device, plus an exclusive prefix of fixed-size sample tiles per
+//     interval, so one launch covers every interval (the reference launches
+//     once per interval with a device sync after each, interpolate.cu:750-771).
+//   * direct_fp64_kernel -- the plain FP64 mode sum of make_waveform
+//     (interpolate.cu:444-689): per output sample, for every mode, a cubic
+//     amplitude, the phase m*Phi_phi + k*Phi_theta + n*Phi_r, one FP64 sincos
+//     and three complex products.  Modes are staged through LDS in chunks,
+//     each lane owns SPT samples, the waveform is accumulated into without
+//     atomics (every sample belongs to exactly one lane of one block).
+//
+// Precision policy: the phase pipeline (dense-output evaluation, s, the
+// m/k/n combination) is evaluated with contraction off in exactly the
+// oracle's operation order, so the argument of sincos is bit-identical to
+// the CPU restatement.  Amplitude and complex products may use FMA.
+// Mode partial sums are flushed into the output every 5000 modes, matching
+// the reference CPU chunking (interpolate.cu:22, 554-600, 682-685).
+
+#include "few_common.hpp"
+#include "modesum_direct.hpp"
+
+namespace few {
+
+__global__ __launch_bounds__(256) void plan_segments_kernel(PlanArgs a)
+{
+    __shared__ int s_used;
+    __shared__ int s_scan[256];
+    __shared__ int s_carry;
+    const int tid = threadIdx.x;
+    const int L = a.L;
+    const double T = (a.out_len - 1) * a.dt;
+    if (tid == 0) {
+        s_used = L;
+        s_carry = 0;
+    }
+    __syncthreads();
+    for (int i = 1 + tid; i < L; i += 256)
+        if (!(a.h_t[i] < T)) atomicMin(&s_used, i + 1);
+    __syncthreads();
+    const int used = s_used;
+    const int nseg = L - 1;
+    for (int base = 0; base < nseg; base += 256) {
+        const int j = base + tid;
+        int nt = 0;
+        if (j < nseg) {
+            Segment sg;
+            int st = 0, en = 0;
+            if (j < used - 1) {
+                st = plan_start(a.h_t, j, T, a.dt, a.out_len);
+                en = plan_start(a.h_t, j + 1, T, a.dt, a.out_len);
+                if (j + 1 == used - 1 && used < L) en = a.out_len;
+                st = st < 0 ? 0 : (st > a.out_len ? a.out_len : st);
+                en = en < 0 ? 0 : (en > a.out_len ? a.out_len : en);
+                if (en < st) en = st;
+            }
+            sg.start = st;
+            sg.end = en;
+            sg.h_t = a.h_t[j];
+            sg.pt = a.pt[j];
+            sg.pw = a.pt[j + 1] - a.pt[j];
+            a.segs[j] = sg;
+            nt = (en - st + a.tile - 1) / a.tile;
+        }
+        s_scan[tid] = nt;
+        __syncthreads();
+        // Hillis-Steele inclusive scan over 256 entries
+        for (int off = 1; off < 256; off <<= 1) {
+            int v = (tid >= off) ? s_scan[tid - off] : 0;
+            __syncthreads();
+            s_scan[tid] += v;
+            __syncthreads();
+        }
+        const int carry = s_carry;
+        if (j < nseg) a.prefix[j] = carry + s_scan[tid] - nt;
+        __syncthreads();
+        if (tid == 255) s_carry = carry + s_scan[255];
+        __syncthreads();
+    }
+    if (tid == 0) a.prefix[nseg] = s_carry;
+}
+
+// Block -> (interval, tile) by binary search over the tile prefix.
+__device__ __forceinline__ int find_segment(const int *prefix, int nseg, int b)
+{
+    int lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+#define MCH 256          // modes staged per LDS chunk
+#define CPU_CHUNK 5000   // interpolate.cu:22
+
+template <int SPT>
+__global__ __launch_bounds__(256) void direct_fp64_kernel(DirectArgs a)
+{
+    __shared__ double sh[15][MCH];
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x;
+    const int total = a.prefix[a.nseg];
+    if (b >= total) return;
+    const int j = find_segment(a.prefix, a.nseg, b);
+    const Segment sg = a.segs[j];
+    const int q = b - a.prefix[j];
+
+    const long np = (long)a.nseg * 3;
+    double rp[8], rt[8], rr[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        rp[c] = a.pcoef[c * np + (long)j * 3 + 0];
+        rt[c] = a.pcoef[c * np + (long)j * 3 + 1];
+        rr[c] = a.pcoef[c * np + (long)j * 3 + 2];
+    }
+
+    int idx[SPT];
+    bool ok[SPT];
+    double x[SPT], x2[SPT], x3[SPT], Pp[SPT], Pt[SPT], Pr[SPT];
+    double w_re[SPT], w_im[SPT], tr_re[SPT], tr_im[SPT];
+#pragma unroll
+    for (int u = 0; u < SPT; u++) {
+        idx[u] = sg.start + q * (256 * SPT) + u * 256 + tid;
+        ok[u] = idx[u] < sg.end;
+        SampleState st;
+        sample_state(a.dt, idx[u], sg.h_t, sg.pt, sg.pw, rp, rt, rr, st);
+        x[u] = st.x; x2[u] = st.x2; x3[u] = st.x3;
+        Pp[u] = st.Pp; Pt[u] = st.Pt; Pr[u] = st.Pr;
+        w_re[u] = 0.0; w_im[u] = 0.0;
+        if (ok[u]) {
+            const double2 w = a.wave[idx[u]];
+            w_re[u] = w.x;
+            w_im[u] = w.y;
+        }
+        tr_re[u] = 0.0;
+        tr_im[u] = 0.0;
+    }
+
+    const int K = a.K;
+    const long nb = (long)a.L * 2 * K;
+    for (int c0 = 0; c0 < K; c0 += MCH) {
+        const int cnt = (K - c0) < MCH ? (K - c0) : MCH;
+        __syncthreads();
+        if (tid < cnt) {
+            const int mode = c0 + tid;
+            const long br = (long)j * 2 * K + mode;
+            const long bi = br + K;
+            sh[0][tid] = a.interp[br];
+            sh[1][tid] = a.interp[nb + br];
+            sh[2][tid] = a.interp[2 * nb + br];
+            sh[3][tid] = a.interp[3 * nb + br];
+            sh[4][tid] = a.interp[bi];
+            sh[5][tid] = a.interp[nb + bi];
+            sh[6][tid] = a.interp[2 * nb + bi];
+            sh[7][tid] = a.interp[3 * nb + bi];
+            const double2 yp = a.ylm[mode];
+            const double2 ym = a.ylm[K + mode];
+            sh[8][tid] = yp.x;
+            sh[9][tid] = yp.y;
+            sh[10][tid] = ym.x;
+            sh[11][tid] = ym.y;
+            sh[12][tid] = (double)a.m[mode];
+            sh[13][tid] = (double)a.k[mode];
+            sh[14][tid] = (double)a.n[mode];
+        }
+        __syncthreads();
+        for (int jj = 0; jj < cnt; jj++) {
+            const int mode = c0 + jj;
+            if (mode > 0 && (mode % CPU_CHUNK) == 0) {
+#pragma unroll
+                for (int u = 0; u < SPT; u++) {
+                    w_re[u] = w_re[u] + tr_re[u];
+                    w_im[u] = w_im[u] + tr_im[u];
+                    tr_re[u] = 0.0;
+                    tr_im[u] = 0.0;
+                }
+            }
+            ModeCoef mc;
+            mc.ry = sh[0][jj]; mc.rc1 = sh[1][jj]; mc.rc2 = sh[2][jj]; mc.rc3 = sh[3][jj];
+            mc.iy = sh[4][jj]; mc.ic1 = sh[5][jj]; mc.ic2 = sh[6][jj]; mc.ic3 = sh[7][jj];
+            mc.yp_re = sh[8][jj]; mc.yp_im = sh[9][jj];
+            mc.ym_re = sh[10][jj]; mc.ym_im = sh[11][jj];
+            mc.m = sh[12][jj]; mc.k = sh[13][jj]; mc.n = sh[14][jj];
+#pragma unroll
+            for (int u = 0; u < SPT; u++)
+                mode_term_fp64(mc, x[u], x2[u], x3[u], Pp[u], Pt[u], Pr[u], tr_re[u], tr_im[u]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < SPT; u++) {
+        if (ok[u]) {
+            double2 w;
+            w.x = w_re[u] + tr_re[u];
+            w.y = w_im[u] + tr_im[u];
+            a.wave[idx[u]] = w;
+        }
+    }
+}
+
+int launch_plan(const PlanArgs &p, hipStream_t stream)
+{
+    if (p.L < 2) {
+        set_error("plan: need at least 2 knots (got %d)", p.L);
+        return FEW_ERR_ARG;
+    }
+    hipLaunchKernelGGL(plan_segments_kernel, dim3(1), dim3(256), 0, stream, p);
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
+int direct_tile(int /*precision*/) { return 256 * DIRECT_SPT; }
+
+int launch_direct_fp64(const DirectArgs &a, int max_blocks, hipStream_t stream)
+{
+    if (max_blocks <= 0) return FEW_OK;
+    hipLaunchKernelGGL(direct_fp64_kernel<DIRECT_SPT>, dim3(max_blocks), dim3(256), 0, stream, a);
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
+}  // namespace few

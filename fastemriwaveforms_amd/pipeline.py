@@ -1,0 +1,79 @@
+"""Knot values -> waveform on the GPU, device-resident end to end.
+
+:class:`ModeSumProblem` holds one waveform's inputs (knot times, complex knot
+amplitudes, phase knot values, Y_lm, mode indices) in HBM together with every
+buffer the path needs, so a step is exactly:
+
+1. one spline launch: 2K amplitude splines + 3 phase splines (the phase
+   pieces written in the dense-output layout get_waveform reads),
+2. one planning launch (interval -> sample tiles, find_start_inds rule),
+3. one mode-sum launch.
+
+No host synchronisation and no allocation inside :meth:`run`, so a step can
+be captured in a HIP graph.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .synthetic import ModeSumInput
+
+PRECISION_CODES = {"fp64": _lib.FEW_PRECISION_FP64}
+
+
+def _stream() -> int:
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+class ModeSumProblem:
+    def __init__(self, inp: ModeSumInput, device: torch.device | str = "cuda"):
+        dev = torch.device(device)
+        self.inp = inp
+        self.L, self.K, self.N = inp.L, inp.K, inp.num_pts
+        self.dt = float(inp.dt)
+        f64 = torch.float64
+        self.t = torch.from_numpy(np.ascontiguousarray(inp.t, np.float64)).to(dev)
+        self.teuk = torch.from_numpy(np.ascontiguousarray(inp.teuk, np.complex128)).to(dev)
+        self.phase_knots = torch.from_numpy(np.ascontiguousarray(inp.phase_knots, np.float64)).to(dev)
+        self.ylm = torch.from_numpy(np.ascontiguousarray(inp.ylms, np.complex128)).to(dev)
+        self.m = torch.from_numpy(np.ascontiguousarray(inp.m, np.int32)).to(dev)
+        self.k = torch.from_numpy(np.ascontiguousarray(inp.k, np.int32)).to(dev)
+        self.n = torch.from_numpy(np.ascontiguousarray(inp.n, np.int32)).to(dev)
+        L, K = self.L, self.K
+        self.interp = torch.empty((4, L, 2 * K), dtype=f64, device=dev)
+        self.pcoef = torch.empty((8, L - 1, 3), dtype=f64, device=dev)
+        self.scratch = torch.empty(4 * L * (2 * K + 3), dtype=f64, device=dev)
+        self.wave = torch.zeros(self.N, dtype=torch.complex128, device=dev)
+        nbytes = max(int(lib().few_waveform_workspace_bytes(L, self.N, K, code))
+                     for code in PRECISION_CODES.values())
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+
+    @property
+    def mode_samples(self) -> int:
+        return self.K * self.N
+
+    def build_splines(self) -> None:
+        rc = lib().few_prepare_modes_async(
+            torch.view_as_real(self.teuk).data_ptr(), self.phase_knots.data_ptr(),
+            self.t.data_ptr(), self.L, self.K, self.interp.data_ptr(), self.pcoef.data_ptr(),
+            self.scratch.data_ptr(), _stream())
+        check(rc, "prepare_modes")
+
+    def mode_sum(self, precision: str = "fp64") -> None:
+        rc = lib().few_get_waveform_async(
+            self.wave.data_ptr(), self.interp.data_ptr(), self.t.data_ptr(), self.pcoef.data_ptr(),
+            self.m.data_ptr(), self.k.data_ptr(), self.n.data_ptr(), self.L, self.N, self.K,
+            self.ylm.data_ptr(), self.dt, self.t.data_ptr(), PRECISION_CODES[precision],
+            self.ws.data_ptr(), self.ws.numel(), _stream())
+        check(rc, "get_waveform")
+
+    def run(self, precision: str = "fp64") -> torch.Tensor:
+        """One full step: zero the output, build splines, plan, sum."""
+        self.wave.zero_()
+        self.build_splines()
+        self.mode_sum(precision)
+        return self.wave

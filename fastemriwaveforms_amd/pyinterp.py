@@ -1,0 +1,139 @@
+"""Backend methods with the reference's call signatures (src/few/cutils/pyinterp.pyx:25-92).
+
+``interpolate_arrays_wrap`` and ``get_waveform_wrap`` accept exactly the
+positional arguments the reference's Cython wrappers take, so FEW's
+``CubicSplineInterpolant`` / ``InterpolatedModeSum`` call them unchanged.
+Array arguments may be
+
+* torch tensors on the GPU -- passed by device pointer (no copies), or
+* numpy arrays (the ``numpy``-featured drop-in backend) -- staged to the GPU,
+  computed there, and written back in place, mirroring what the reference's
+  CPU backend leaves in them.
+
+Scalars pass through, like ``obj_to_ptr`` (src/few/cutils/wrappers.py:13-25).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+__all__ = ["interpolate_arrays_wrap", "get_waveform_wrap", "get_waveform_generic_fd_wrap"]
+
+
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        from .exceptions import MissingHardware
+        raise MissingHardware("the rocm backend needs a visible AMD GPU (torch.cuda)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream_ptr() -> int:
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+class _Staged:
+    """A device view of one array argument; writes numpy outputs back on finish()."""
+
+    def __init__(self, obj, dtype, writeback: bool):
+        self.obj = obj
+        self.writeback = writeback
+        if isinstance(obj, torch.Tensor):
+            if not obj.is_cuda:
+                self.dev = obj.to(_device()).contiguous()
+            else:
+                if not obj.is_contiguous():
+                    raise ValueError("device tensors passed to the backend must be contiguous")
+                self.dev = obj
+            if self.dev.dtype != dtype:
+                raise TypeError(f"expected {dtype}, got {self.dev.dtype}")
+        elif isinstance(obj, np.ndarray):
+            arr = np.ascontiguousarray(obj)
+            self.dev = torch.from_numpy(arr).to(_device())
+            if self.dev.dtype != dtype:
+                raise TypeError(f"expected {dtype}, got {self.dev.dtype}")
+        else:
+            raise TypeError(f"unsupported array argument of type {type(obj)}")
+
+    @property
+    def ptr(self) -> int:
+        return int(self.dev.data_ptr())
+
+    def finish(self) -> None:
+        if not self.writeback:
+            return
+        if isinstance(self.obj, np.ndarray):
+            self.obj[...] = self.dev.cpu().numpy().reshape(self.obj.shape)
+        elif isinstance(self.obj, torch.Tensor) and self.dev is not self.obj:
+            self.obj.copy_(self.dev.reshape(self.obj.shape))
+
+
+def _host_f64(obj) -> np.ndarray:
+    if isinstance(obj, torch.Tensor):
+        return np.ascontiguousarray(obj.detach().cpu().numpy(), dtype=np.float64)
+    return np.ascontiguousarray(obj, dtype=np.float64)
+
+
+def interpolate_arrays_wrap(*args, **kwargs):
+    """interpolate_arrays(t_arr, interp_array, ninterps, length, B, upper_diag, diag, lower_diag).
+
+    src/few/cutils/pyinterp.pyx:25-38.  interp_array is updated in place.
+    """
+    t_arr, interp_array, ninterps, length, B, upper_diag, diag, lower_diag = args
+    f64 = torch.float64
+    st = [_Staged(t_arr, f64, False), _Staged(interp_array, f64, True), _Staged(B, f64, True),
+          _Staged(upper_diag, f64, True), _Staged(diag, f64, True), _Staged(lower_diag, f64, True)]
+    n = int(ninterps) * int(length)
+    for s, size in zip(st, (n, 4 * n, n, n, n, n)):
+        if s.dev.numel() < size:
+            raise ValueError("interpolate_arrays: array smaller than the sizes imply")
+    rc = lib().few_interpolate_arrays(st[0].ptr, st[1].ptr, int(ninterps), int(length), st[2].ptr,
+                                      st[3].ptr, st[4].ptr, st[5].ptr, _stream_ptr())
+    check(rc, "interpolate_arrays")
+    for s in st:
+        s.finish()
+
+
+def get_waveform_wrap(*args, **kwargs):
+    """get_waveform(d_waveform, interp_array, phase_interp_t, phase_interp_coeffs, d_m, d_k, d_n,
+    init_len, out_len, num_teuk_modes, d_Ylms, delta_t, h_t, dev).
+
+    src/few/cutils/pyinterp.pyx:41-61.  d_waveform is accumulated into in place.
+    phase_interp_t and h_t are host arrays in the reference; device tensors
+    are accepted and copied to the host here, as the reference's Python side
+    does with ``t.get()`` (summation/interpolatedmodesum.py:339).
+    """
+    (d_waveform, interp_array, phase_interp_t, phase_interp_coeffs, d_m, d_k, d_n, init_len,
+     out_len, num_teuk_modes, d_Ylms, delta_t, h_t, dev) = args
+    wave = _Staged(d_waveform, torch.complex128, True)
+    interp = _Staged(interp_array, torch.float64, False)
+    pcoef = _Staged(phase_interp_coeffs, torch.float64, False)
+    m = _Staged(d_m, torch.int32, False)
+    k = _Staged(d_k, torch.int32, False)
+    n = _Staged(d_n, torch.int32, False)
+    ylm = _Staged(d_Ylms, torch.complex128, False)
+    pt = _host_f64(phase_interp_t)
+    ht = _host_f64(h_t)
+    L, N, K = int(init_len), int(out_len), int(num_teuk_modes)
+    if len(pt) < L or len(ht) < L:
+        raise ValueError("get_waveform: time arrays shorter than init_len")
+    if (wave.dev.numel() < N or interp.dev.numel() < 4 * L * 2 * K
+            or pcoef.dev.numel() < 8 * (L - 1) * 3 or min(m.dev.numel(), k.dev.numel(), n.dev.numel()) < K
+            or ylm.dev.numel() < 2 * K):
+        raise ValueError("get_waveform: array smaller than the sizes imply")
+    # The reference selects `dev` with cudaSetDevice (interpolate.cu:753); the
+    # buffers here already live on one device, which is the one used.
+    dev_index = wave.dev.device.index
+    rc = lib().few_get_waveform(wave.ptr, interp.ptr, pt.ctypes.data, pcoef.ptr, m.ptr, k.ptr, n.ptr,
+                                L, N, K, ylm.ptr, float(delta_t), ht.ctypes.data,
+                                int(dev_index), _stream_ptr())
+    check(rc, "get_waveform")
+    wave.finish()
+
+
+def get_waveform_generic_fd_wrap(*args, **kwargs):
+    """The frequency-domain sum (pyinterp.pyx:64-92) is not part of this backend."""
+    raise NotImplementedError("get_waveform_generic_fd is not implemented by the rocm backend")

@@ -1,0 +1,156 @@
+"""Seeded synthetic inputs for the interpolated mode sum (host numpy arrays).
+
+Workloads (sizes follow summation/base.py:83-102 with YRSID_SI from
+cutils/global.h:23):
+
+* ``"cpu0"``   -- T = 0.1 yr, dt = 10 s, 100 knots, 500 of the 3843-mode set.
+* ``"full1"``  -- T = 1 yr, dt = 10 s, 120 knots, all 3843 modes.
+* ``"mixed2"`` -- T = 2 yr, dt = 5 s, 200 knots, 2000 of the 12,987-mode set
+  (|m| <= l, n in [-55, 55]) with Y- = 0 for every mode (explicit +-m).
+* batch    -- :func:`make_batch`: uneven durations / knots / mode counts.
+
+Distributions: knots t0 = 0, t_{L-1} = T, interior sorted uniform draws.
+f(t) = 1e-3 + 2e-3 (3 tau^2 - 2 tau^3) Hz with tau = t/T and
+Phi_phi = 2 pi int f (analytic quartic); Phi_r = kappa Phi_phi with
+kappa ~ U(0.8, 0.95); Phi_theta = 0.  Knot amplitudes
+CN(0,1) 10^(-|n|/10) 10^(-(l-2)/3) (1 + sum_j a_j tau^j), a_j ~ N(0, 0.1^2).
+Y+ and Y- have unit modulus and uniform phase.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+YRSID_SI = 31558149.763545595  # cutils/global.h:23
+
+WORKLOADS = {
+    "cpu0": dict(T=0.1, dt=10.0, L=100, mode_set="m_nonneg_30", keep=500, seed=1000),
+    "full1": dict(T=1.0, dt=10.0, L=120, mode_set="m_nonneg_30", keep=None, seed=1001),
+    "mixed2": dict(T=2.0, dt=5.0, L=200, mode_set="m_all_55", keep=2000, seed=1002),
+}
+
+
+@dataclasses.dataclass
+class ModeSumInput:
+    """One waveform's inputs to InterpolatedModeSum.sum-equivalent code."""
+
+    t: np.ndarray            # (L,) knot times [s]
+    teuk: np.ndarray         # (L, K) complex128 knot amplitudes
+    ylms: np.ndarray         # (2K,) complex128: Y+ then Y- (as the kernel reads them)
+    phase_knots: np.ndarray  # (3, L) Phi_phi, Phi_theta, Phi_r at the knots
+    l: np.ndarray
+    m: np.ndarray
+    k: np.ndarray
+    n: np.ndarray            # (K,) int32
+    dt: float
+    T_years: float
+    num_pts: int
+    name: str = ""
+
+    @property
+    def L(self) -> int:
+        return len(self.t)
+
+    @property
+    def K(self) -> int:
+        return self.teuk.shape[1]
+
+    @property
+    def mode_samples(self) -> int:
+        return self.K * self.num_pts
+
+
+def mode_set(kind: str):
+    """(l, m, n) arrays of a mode set."""
+    ls, ms, ns = [], [], []
+    if kind == "m_nonneg_30":      # SchwarzschildEccentric(lmax=10, nmax=30): 63 * 61 = 3843
+        for l in range(2, 11):
+            for m in range(0, l + 1):
+                for n in range(-30, 31):
+                    ls.append(l); ms.append(m); ns.append(n)
+    elif kind == "m_all_55":       # 117 (l, m) pairs with |m| <= l, times 111 n = 12,987
+        for l in range(2, 11):
+            for m in range(-l, l + 1):
+                for n in range(-55, 56):
+                    ls.append(l); ms.append(m); ns.append(n)
+    else:
+        raise ValueError(kind)
+    return (np.asarray(ls, np.int32), np.asarray(ms, np.int32), np.asarray(ns, np.int32))
+
+
+def num_points(t_first: float, t_last: float, T_years: float, dt: float) -> int:
+    """summation/base.py:83-98 without padding."""
+    n_pts = int(T_years * YRSID_SI / dt)
+    T = n_pts * dt
+    if T < t_last:
+        return int((T - t_first) / dt) + 1
+    return int((t_last - t_first) / dt) + 1
+
+
+def phase_phi(t: np.ndarray, Ts: float) -> np.ndarray:
+    """2 pi int_0^t f, f = 1e-3 + 2e-3 (3 tau^2 - 2 tau^3), tau = t/Ts."""
+    tau = t / Ts
+    return 2.0 * np.pi * (1e-3 * t + 2e-3 * Ts * (tau ** 3 - 0.5 * tau ** 4))
+
+
+def make_input(T_years: float, dt: float, L: int, mode_kind: str, keep, seed: int,
+               minus_m_ylm: bool = True, name: str = "") -> ModeSumInput:
+    rng = np.random.default_rng(seed)
+    Ts = T_years * YRSID_SI
+    interior = np.sort(rng.uniform(0.0, 1.0, L - 2)) * Ts
+    t = np.concatenate([[0.0], interior, [Ts]])
+    l_all, m_all, n_all = mode_set(mode_kind)
+    if keep is not None and keep < len(l_all):
+        sel = np.sort(rng.choice(len(l_all), size=keep, replace=False))
+    else:
+        sel = np.arange(len(l_all))
+    l, m, n = l_all[sel], m_all[sel], n_all[sel]
+    K = len(l)
+    tau = t / Ts
+    base = (rng.normal(size=K) + 1j * rng.normal(size=K)) / np.sqrt(2.0)
+    scale = 10.0 ** (-np.abs(n) / 10.0) * 10.0 ** (-(l - 2) / 3.0)
+    a = rng.normal(0.0, 0.1, size=(3, K))
+    mod = 1.0 + a[0] * tau[:, None] + a[1] * tau[:, None] ** 2 + a[2] * tau[:, None] ** 3
+    teuk = (base * scale)[None, :] * mod
+    yp = np.exp(1j * rng.uniform(0, 2 * np.pi, K))
+    ym = np.exp(1j * rng.uniform(0, 2 * np.pi, K))
+    if not minus_m_ylm:
+        ym = np.zeros(K, dtype=np.complex128)
+    kappa = rng.uniform(0.8, 0.95)
+    Pphi = phase_phi(t, Ts)
+    phase_knots = np.stack([Pphi, np.zeros_like(Pphi), kappa * Pphi])
+    return ModeSumInput(
+        t=t, teuk=np.ascontiguousarray(teuk.astype(np.complex128)),
+        ylms=np.concatenate([yp, ym]).astype(np.complex128),
+        phase_knots=np.ascontiguousarray(phase_knots), l=l, m=m,
+        k=np.zeros(K, np.int32), n=n, dt=float(dt), T_years=float(T_years),
+        num_pts=num_points(t[0], t[-1], T_years, dt), name=name)
+
+
+def make_workload(name: str, seed: int | None = None) -> ModeSumInput:
+    w = WORKLOADS[name]
+    return make_input(w["T"], w["dt"], w["L"], w["mode_set"], w["keep"],
+                      w["seed"] if seed is None else seed,
+                      minus_m_ylm=(w["mode_set"] != "m_all_55"), name=name)
+
+
+def make_batch(count: int = 256, seed: int = 1003, t_range=(0.1, 2.0), L_range=(50, 200),
+               K_range=(50, 3000), dt: float = 10.0) -> list[ModeSumInput]:
+    """Uneven batch: durations U(t_range) yr, knots U{L_range}, modes U{K_range} of the 3843 set."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(count):
+        T = float(rng.uniform(*t_range))
+        L = int(rng.integers(L_range[0], L_range[1] + 1))
+        K = int(rng.integers(K_range[0], K_range[1] + 1))
+        out.append(make_input(T, dt, L, "m_nonneg_30", K, int(rng.integers(1 << 31)),
+                              name=f"batch{b}"))
+    return out
+
+
+def small_input(seed: int = 7, L: int = 12, K: int = 20, T_years: float = 0.002,
+                dt: float = 10.0) -> ModeSumInput:
+    """A tiny case (a few thousand samples) for brute-force checks."""
+    return make_input(T_years, dt, L, "m_nonneg_30", K, seed, name="small")

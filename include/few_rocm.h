@@ -1,0 +1,124 @@
+/*
+ * few_rocm.h -- C ABI of the MI355X (gfx950) interpolated mode-summation
+ * library (libfew_rocm.so), a drop-in for FastEMRIWaveforms' native
+ * interpolation backend.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  "Device" pointers are HIP device
+ *     memory; the drop-in twins keep the reference's host/device split.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *   - Every entry returns 0 (FEW_OK) or a FEW_ERR_* code; the message of the
+ *     calling thread's last failure is few_last_error().  Nothing calls
+ *     exit() (the reference aborts via gpuErrchk, src/few/cutils/global.h:49-57).
+ *   - complex128 buffers are interleaved {double re, double im}, like the
+ *     reference's cmplx (src/few/cutils/global.h:30).
+ *
+ * Citations are to the FastEMRIWaveforms source tree.
+ */
+#ifndef FEW_ROCM_H
+#define FEW_ROCM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEW_ROCM_ABI_VERSION 1
+
+#define FEW_PRECISION_FP64 0   /* plain FP64 direct sum (reference arithmetic) */
+
+const char *few_last_error(void);
+int few_abi_version(void);
+
+/*
+ * Replaces interpolate_arrays (src/few/cutils/interpolate.hh:27,
+ * src/few/cutils/interpolate.cu:284-325; bound at src/few/cutils/pyinterp.pyx:25-38).
+ * Same arguments, all device pointers:
+ *   t_arr        (ninterps, length) knot times, one row per array
+ *   interp_array (4, ninterps, length): slot 0 = y on entry, slots 1..3
+ *                receive c1..c3 (slot 0 is left as y)
+ *   B, upper_diag, diag, lower_diag  (ninterps*length) scratch; contents on
+ *                return are unspecified (the reference leaves the solve there)
+ * Synchronous on `stream`, like the reference.  length >= 3 (the reference
+ * reads out of bounds for length < 3).
+ */
+int few_interpolate_arrays(double *t_arr, double *interp_array, int ninterps, int length,
+                           double *B, double *upper_diag, double *diag, double *lower_diag,
+                           void *stream);
+
+/*
+ * Replaces get_waveform (src/few/cutils/interpolate.hh:29-31,
+ * src/few/cutils/interpolate.cu:726-797; bound at src/few/cutils/pyinterp.pyx:41-61).
+ * Same arguments and host/device split:
+ *   d_waveform          device complex128 (out_len,), ACCUMULATED INTO
+ *   interp_array        device (4, init_len, 2*num_teuk_modes) coefficients
+ *   phase_interp_t      HOST (init_len,) dense-output segment times
+ *   phase_interp_coeffs device (8, init_len-1, 3)
+ *   d_m, d_k, d_n       device int32 (num_teuk_modes,)
+ *   d_Ylms              device complex128 (2*num_teuk_modes,), Y+ then Y-
+ *   h_t                 HOST (init_len,) amplitude knot times
+ *   dev                 device ordinal (hipSetDevice; restored on return)
+ * Plain FP64 arithmetic with the reference's phase rounding.  Synchronous.
+ */
+int few_get_waveform(void *d_waveform, double *interp_array, double *phase_interp_t,
+                     double *phase_interp_coeffs, int *d_m, int *d_k, int *d_n, int init_len,
+                     int out_len, int num_teuk_modes, void *d_Ylms, double delta_t, double *h_t,
+                     int dev, void *stream);
+
+/*
+ * Asynchronous get_waveform on device-resident time arrays (the caller's
+ * CubicSplineInterpolant / InterpolatedModeSum keep t on the device; the
+ * reference copies it to the host, src/few/summation/interpolatedmodesum.py:339).
+ * `precision` selects the arithmetic (FEW_PRECISION_*); `workspace` is a
+ * device buffer of few_waveform_workspace_bytes() bytes, owned by the
+ * caller, that must not be shared by calls in flight at the same time.
+ * No host synchronisation and no allocation: safe to capture in a hipGraph.
+ */
+int few_get_waveform_async(void *d_waveform, const double *interp_array,
+                           const double *d_phase_interp_t, const double *phase_interp_coeffs,
+                           const int *d_m, const int *d_k, const int *d_n, int init_len,
+                           int out_len, int num_teuk_modes, const void *d_Ylms, double delta_t,
+                           const double *d_h_t, int precision, void *workspace,
+                           size_t workspace_bytes, void *stream);
+
+size_t few_waveform_workspace_bytes(int init_len, int out_len, int num_teuk_modes, int precision);
+
+/*
+ * Strided batched not-a-knot spline build (the computation of
+ * interpolate_arrays, src/few/cutils/interpolate.cu:284-325, on arbitrary
+ * layouts).  System s, knot i:
+ *   y[s*y_sys + i*y_knot], t[s*t_sys + i*t_knot]   (t_sys = 0: shared knots)
+ *   dense == 0: out[c*o_coef + s*o_sys + i*o_knot] = c-th cubic coefficient
+ *   dense == 1: out[c*(length-1)*nsys + i*nsys + s] = c-th dense-output
+ *               coefficient (8 per interval, the layout get_waveform reads
+ *               for phases, src/few/cutils/interpolate.cu:516-545)
+ * scratch: 4*length*nsys doubles.  Asynchronous.
+ */
+int few_spline_build_async(const double *y, long y_sys, long y_knot, const double *t,
+                           long t_sys, long t_knot, double *out, long o_coef, long o_sys,
+                           long o_knot, int nsys, int length, int dense, double *scratch,
+                           void *stream);
+
+/*
+ * One launch that prepares everything get_waveform reads from knot values:
+ * the 2K amplitude splines of teuk (length, K) complex128 written to
+ * interp_out (4, length, 2K) -- exactly what InterpolatedModeSum.sum builds
+ * (src/few/summation/interpolatedmodesum.py:326-337, 99-101) -- and the phase
+ * splines of phase_knots (3, length) [Phi_phi, Phi_theta, Phi_r] written as
+ * dense-output coefficients pcoef_out (8, length-1, 3).  t (length,) is
+ * shared; scratch holds 4*length*(2K+3) doubles.  phase_knots may be NULL
+ * (amplitudes only; scratch 4*length*2K doubles).  Asynchronous.
+ */
+int few_prepare_modes_async(const double *teuk, const double *phase_knots, const double *t,
+                            int length, int num_modes, double *interp_out, double *pcoef_out,
+                            double *scratch, void *stream);
+
+/* Frees the library-owned workspaces of the synchronous drop-in entries. */
+int few_release_workspace(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEW_ROCM_H */

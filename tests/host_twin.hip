@@ -1,0 +1,148 @@
+// Host twin of the HIP kernels' scalar arithmetic (test infrastructure).
+//
+// Compiles the product's kernel sources and runs their host-callable
+// per-system / per-sample routines on the CPU, so the CPU test suite can
+// check the exact code the GPU runs (spline recurrence, dense-output phase,
+// the direct mode term, the interval planning rule) against the oracle
+// without a GPU.  Only the orchestration (tiling, LDS staging, launch
+// geometry) is left to the GPU tests.
+
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <vector>
+
+#include "spline.hip"
+#include "modesum_direct.hpp"
+
+namespace few {
+static char g_twin_err[512];
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_twin_err, sizeof(g_twin_err), fmt, ap);
+    va_end(ap);
+}
+int check_hip(hipError_t e, const char *what)
+{
+    (void)e;
+    set_error("%s", what);
+    return FEW_ERR_HIP;
+}
+}  // namespace few
+
+using namespace few;
+
+extern "C" {
+
+int twin_spline_build(const double *y, long y_sys, long y_knot, const double *t, long t_sys,
+                      long t_knot, double *out, long o_coef, long o_sys, long o_knot, int nsys,
+                      int length, int dense, double *scratch, int *info)
+{
+    SplineGroup g{y, y_sys, y_knot, t, t_sys, t_knot, out, o_coef, o_sys, o_knot, nsys, dense};
+    const long f = (long)length * nsys;
+    double *sc[4] = {scratch, scratch + f, scratch + 2 * f, scratch + 3 * f};
+    SplineArgs a;
+    fill_spline_args(&g, 1, length, sc, info, a);
+    for (int s = 0; s < nsys; s++) spline_one_system(a, 0, s);
+    return 0;
+}
+
+int twin_prepare_modes(const double *teuk, const double *phase_knots, const double *t, int length,
+                       int num_modes, double *interp_out, double *pcoef_out, double *scratch)
+{
+    const long K = num_modes, L = length;
+    SplineGroup g[3];
+    g[0] = SplineGroup{teuk, 2, 2 * K, t, 0, 1, interp_out, L * 2 * K, 1, 2 * K, num_modes, 0};
+    g[1] = SplineGroup{teuk + 1, 2, 2 * K, t, 0, 1, interp_out + K, L * 2 * K, 1, 2 * K, num_modes, 0};
+    g[2] = SplineGroup{phase_knots, L, 1, t, 0, 1, pcoef_out, 0, 0, 0, 3, 1};
+    const long f = L * (2 * K + 3);
+    double *sc[4] = {scratch, scratch + f, scratch + 2 * f, scratch + 3 * f};
+    SplineArgs a;
+    fill_spline_args(g, 3, length, sc, nullptr, a);
+    for (int gi = 0; gi < 3; gi++)
+        for (int s = 0; s < g[gi].nsys; s++) spline_one_system(a, gi, s);
+    return 0;
+}
+
+// Interval planning exactly as plan_segments_kernel computes it.
+int twin_plan(const double *h_t, int L, double dt, int out_len, int *starts, int *ends)
+{
+    const double T = (out_len - 1) * dt;
+    int used = L;
+    for (int i = 1; i < L; i++)
+        if (!(h_t[i] < T)) {
+            used = i + 1;
+            break;
+        }
+    for (int j = 0; j < L - 1; j++) {
+        int st = 0, en = 0;
+        if (j < used - 1) {
+            st = plan_start(h_t, j, T, dt, out_len);
+            en = plan_start(h_t, j + 1, T, dt, out_len);
+            if (j + 1 == used - 1 && used < L) en = out_len;
+            st = st < 0 ? 0 : (st > out_len ? out_len : st);
+            en = en < 0 ? 0 : (en > out_len ? out_len : en);
+            if (en < st) en = st;
+        }
+        starts[j] = st;
+        ends[j] = en;
+    }
+    return used;
+}
+
+// Direct FP64 mode sum at chosen samples with the kernel's per-mode code.
+int twin_direct_samples(double *out, const long long *idx, long nidx, const double *interp,
+                        const double *pt, const double *pcoef, const int *m, const int *k,
+                        const int *n, int L, int out_len, int K, const double *ylm, double dt,
+                        const double *h_t)
+{
+    std::vector<int> st(L), en(L);
+    twin_plan(h_t, L, dt, out_len, st.data(), en.data());
+    const long np = (long)(L - 1) * 3;
+    const long nb = (long)L * 2 * K;
+    for (long q = 0; q < nidx; q++) {
+        const int i = (int)idx[q];
+        int j = -1;
+        for (int jj = 0; jj < L - 1; jj++)
+            if (i >= st[jj] && i < en[jj]) {
+                j = jj;
+                break;
+            }
+        double w_re = 0.0, w_im = 0.0;
+        if (j >= 0) {
+            double rp[8], rt[8], rr[8];
+            for (int c = 0; c < 8; c++) {
+                rp[c] = pcoef[c * np + (long)j * 3 + 0];
+                rt[c] = pcoef[c * np + (long)j * 3 + 1];
+                rr[c] = pcoef[c * np + (long)j * 3 + 2];
+            }
+            SampleState ss;
+            sample_state(dt, i, h_t[j], pt[j], pt[j + 1] - pt[j], rp, rt, rr, ss);
+            double tr_re = 0.0, tr_im = 0.0;
+            for (int mode = 0; mode < K; mode++) {
+                if (mode > 0 && mode % 5000 == 0) {
+                    w_re = w_re + tr_re;
+                    w_im = w_im + tr_im;
+                    tr_re = tr_im = 0.0;
+                }
+                const long br = (long)j * 2 * K + mode, bi = br + K;
+                ModeCoef c;
+                c.ry = interp[br]; c.rc1 = interp[nb + br]; c.rc2 = interp[2 * nb + br]; c.rc3 = interp[3 * nb + br];
+                c.iy = interp[bi]; c.ic1 = interp[nb + bi]; c.ic2 = interp[2 * nb + bi]; c.ic3 = interp[3 * nb + bi];
+                c.yp_re = ylm[2 * mode]; c.yp_im = ylm[2 * mode + 1];
+                c.ym_re = ylm[2 * (K + mode)]; c.ym_im = ylm[2 * (K + mode) + 1];
+                c.m = m[mode]; c.k = k[mode]; c.n = n[mode];
+                mode_term_fp64(c, ss.x, ss.x2, ss.x3, ss.Pp, ss.Pt, ss.Pr, tr_re, tr_im);
+            }
+            w_re = w_re + tr_re;
+            w_im = w_im + tr_im;
+        }
+        out[2 * q] = w_re;
+        out[2 * q + 1] = w_im;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+"""GPU parity of the HIP spline build and the plain FP64 mode sum against the oracle.
+
+Bars: spline coefficients bit-identical to the oracle (same recurrence, same
+rounding); FP64 waveforms within rel-L2 1e-12 of the oracle (the only
+difference allowed is the last bit of sin/cos and FMA in the amplitude /
+complex products -- the phase argument is bit-identical).
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import oracle_coefficients, oracle_waveform, spread_indices
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _problem(inp):
+    from fastemriwaveforms_amd.pipeline import ModeSumProblem
+    return ModeSumProblem(inp)
+
+
+def test_interpolate_arrays_dropin_bitwise(gpu):
+    """few_interpolate_arrays through the reference-shaped wrapper, numpy in/out."""
+    from fastemriwaveforms_amd import pyinterp
+    rng = np.random.default_rng(3)
+    for ninterps, length in [(1, 3), (5, 4), (70, 120), (129, 37)]:
+        t = np.sort(rng.uniform(0, 1e6, (ninterps, length)), axis=1)
+        y = rng.normal(size=(ninterps, length))
+        interp = np.zeros((4, ninterps, length))
+        interp[0] = y
+        interp = interp.flatten()
+        B, ud, dg, ld = (np.zeros(ninterps * length) for _ in range(4))
+        pyinterp.interpolate_arrays_wrap(t.flatten(), interp, ninterps, length, B, ud, dg, ld)
+        ref, _, info = oracle.interpolate_arrays(t, y)
+        assert info == (3 if length == 3 else 0)  # 3 knots: singular, LAPACK's early return
+        got = interp.reshape(4, ninterps, length)
+        assert np.array_equal(got, ref), (ninterps, length, np.max(np.abs(got - ref)))
+
+
+def test_cubic_spline_native(gpu):
+    """Restates tests/test_mode_summation.py:23-52 on the native class."""
+    from scipy.interpolate import CubicSpline
+    from fastemriwaveforms_amd.summation import CubicSplineInterpolant
+    t_test = np.linspace(0, 100, 1000)
+    f_test = np.vstack((np.cos(t_test), np.sin(t_test)))
+    few_spl = CubicSplineInterpolant(t_test, f_test)
+    scipy_spl = CubicSpline(t_test, f_test, axis=-1)
+    np.testing.assert_allclose(few_spl(t_test).cpu().numpy(), f_test, rtol=1e-10)
+    for d in range(1, 4):
+        np.testing.assert_allclose(few_spl(t_test, deriv_order=d).cpu().numpy(),
+                                   scipy_spl(t_test, nu=d), rtol=1e-10)
+    t_eval = np.linspace(0, 100, 1852)
+    for d in range(4):
+        np.testing.assert_allclose(few_spl(t_eval, deriv_order=d).cpu().numpy(),
+                                   scipy_spl(t_eval, nu=d), rtol=1e-10)
+
+
+def test_prepare_modes_bitwise(gpu):
+    from fastemriwaveforms_amd import synthetic
+    inp = synthetic.make_workload("cpu0")
+    p = _problem(inp)
+    p.build_splines()
+    interp_tl, pcoef = oracle_coefficients(inp)
+    assert np.array_equal(p.interp.cpu().numpy(), interp_tl)
+    assert np.array_equal(p.pcoef.cpu().numpy(), pcoef)
+
+
+def test_direct_fp64_cpu0_full(gpu):
+    """Workload cpu0 (0.1 yr, 500 modes, 100 knots): every sample against the oracle."""
+    from fastemriwaveforms_amd import synthetic
+    inp = synthetic.make_workload("cpu0")
+    h = _problem(inp).run("fp64").cpu().numpy()
+    ref = oracle_waveform(inp)
+    err = oracle.rel_l2(h, ref)
+    print(f"cpu0 fp64 rel-L2 {err:.3e}")
+    assert err <= FP64_TOL
+
+
+def test_direct_fp64_full1_samples(gpu):
+    """Workload full1 (1 yr, 3843 modes, 120 knots): 2000+ spread samples."""
+    from fastemriwaveforms_amd import synthetic
+    inp = synthetic.make_workload("full1")
+    h = _problem(inp).run("fp64").cpu().numpy()
+    idx = spread_indices(inp.num_pts, count=2000)
+    ref = oracle_waveform(inp, idx)
+    err = oracle.rel_l2(h[idx], ref)
+    print(f"full1 fp64 rel-L2 {err:.3e} over {len(idx)} samples")
+    assert err <= FP64_TOL
+
+
+def test_dropin_get_waveform_numpy(gpu):
+    """The reference-shaped wrappers driven exactly as InterpolatedModeSum.sum drives them
+    (interpolatedmodesum.py:326-375), with numpy arrays (Feature.NUMPY drop-in)."""
+    from fastemriwaveforms_amd import pyinterp, synthetic
+    inp = synthetic.small_input(seed=11, L=15, K=40, T_years=0.003)
+    L, K = inp.L, inp.K
+    y_all = np.concatenate([inp.teuk.T.real, inp.teuk.T.imag])
+    interp = np.zeros((4, 2 * K, L))
+    interp[0] = y_all
+    interp = interp.flatten()
+    tt = np.tile(inp.t, (2 * K, 1)).flatten()
+    B, ud, dg, ld = (np.zeros(2 * K * L) for _ in range(4))
+    pyinterp.interpolate_arrays_wrap(tt, interp, 2 * K, L, B, ud, dg, ld)
+    interp_tl = np.transpose(interp.reshape(4, 2 * K, L), [0, 2, 1]).flatten()
+    _, pcoef = oracle_coefficients(inp)
+    wave = np.zeros(inp.num_pts, dtype=np.complex128)
+    pyinterp.get_waveform_wrap(wave, interp_tl, inp.t, pcoef.flatten(), inp.m, inp.k, inp.n, L,
+                               inp.num_pts, K, inp.ylms, inp.dt, inp.t, 0)
+    ref = oracle_waveform(inp)
+    assert oracle.rel_l2(wave, ref) <= FP64_TOL
+
+
+def test_native_interpolated_mode_sum(gpu):
+    """Native InterpolatedModeSum.__call__ (summation/base.py:66 + interpolatedmodesum.py:283)."""
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.summation import InterpolatedModeSum
+    inp = synthetic.small_input(seed=5, L=20, K=60, T_years=0.004)
+    _, pcoef = oracle_coefficients(inp)
+    dopr = np.transpose(pcoef, (1, 2, 0))  # (L-1, 3, 8): the trajectory's layout
+    ylms = inp.ylms.copy()
+    ylms[inp.K:] *= (-1.0) ** inp.l  # sum() applies (-1)^l again (interpolatedmodesum.py:356-358)
+    summ = InterpolatedModeSum()
+    h = summ(inp.t, inp.teuk, ylms, inp.t, dopr, inp.l, inp.m, inp.k, inp.n,
+             T=inp.T_years, dt=inp.dt).cpu().numpy()
+    assert summ.num_pts == inp.num_pts
+    ref = oracle_waveform(inp)
+    assert oracle.rel_l2(h, ref) <= FP64_TOL
+
+
+def test_brute_force_small(gpu):
+    """Direct numpy sum over modes (directmodesum.py:91-126 pattern) on dense
+    spline evaluations: pins the whole FP64 pipeline on a tiny case."""
+    from fastemriwaveforms_amd import synthetic
+    inp = synthetic.small_input(seed=2, L=10, K=12, T_years=0.001)
+    h = _problem(inp).run("fp64").cpu().numpy()
+    interp_tl, pcoef = oracle_coefficients(inp)
+    N = inp.num_pts
+    tt = np.arange(N) * inp.dt
+    j = np.clip(np.searchsorted(inp.t, tt, side="right") - 1, 0, inp.L - 2)
+    x = tt - inp.t[j]
+    K = inp.K
+    A = np.zeros((N, K), dtype=np.complex128)
+    for p in range(4):
+        A += (interp_tl[p][j, :K] + 1j * interp_tl[p][j, K:]) * x[:, None] ** p
+    s = (tt - inp.t[j]) / (inp.t[j + 1] - inp.t[j])
+    ph = []
+    for q in range(3):
+        r = pcoef[:, j, q]
+        ph.append(r[0] + s * (r[1] + (1 - s) * (r[2] + s * r[3])))
+    phase = inp.m[None] * ph[0][:, None] + inp.k[None] * ph[1][:, None] + inp.n[None] * ph[2][:, None]
+    P = A * np.exp(-1j * phase)
+    yp, ym = inp.ylms[:K], inp.ylms[K:]
+    ref = (P * yp).sum(1) + ((inp.m != 0) * np.conj(P) * ym).sum(1)
+    assert oracle.rel_l2(h, ref) < 1e-10
+
+
+@pytest.mark.parametrize("case", ["many_modes", "knots_on_grid", "knot_past_end", "accumulate"])
+def test_direct_fp64_edges(gpu, case):
+    import torch
+    from fastemriwaveforms_amd import synthetic
+    if case == "many_modes":
+        # > 5000 modes: exercises the 5000-mode flush of the CPU chunking
+        inp = synthetic.make_input(0.0015, 10.0, 8, "m_all_55", 5500, 17, name=case)
+    else:
+        inp = synthetic.small_input(seed=23, L=14, K=30, T_years=0.002)
+    if case == "knots_on_grid":
+        inp.t[1:-1] = np.round(inp.t[1:-1] / inp.dt) * inp.dt  # samples exactly on knots
+    if case == "knot_past_end":
+        inp.num_pts = inp.num_pts - 1000  # a middle knot lies past (N-1)*dt: find_start_inds breaks
+    p = _problem(inp)
+    if case == "accumulate":
+        init = (torch.arange(inp.num_pts, dtype=torch.float64, device=p.wave.device) * 1e-3).to(torch.complex128)
+        p.wave.copy_(init)
+        p.build_splines()
+        p.mode_sum("fp64")
+        h = (p.wave - init).cpu().numpy()
+    else:
+        h = p.run("fp64").cpu().numpy()
+    ref = oracle_waveform(inp)
+    err = oracle.rel_l2(h, ref)
+    assert err <= FP64_TOL, (case, err)
