@@ -2,12 +2,14 @@
 """Benchmark of the interpolated mode sum on one MI355X.
 
 Workload (default): full1 -- T = 1 yr, dt = 10 s (3,155,815 samples), 120
-knots, all 3843 modes (l = 2..10, m = 0..l, n = -30..30), synthetic seeded
+knots, all 3843 modes (l = 2..10, m = 0..l, n = -30..30), seeded synthetic
 inputs resident in HBM.  One step = zero the output + spline build (2K+3
-systems) + interval planning + mode sum, i.e. the whole path from knot
-values to the complex128 waveform.
+systems) + interval planning / coefficient tiles + mode sum: the whole path
+from knot values to the complex128 waveform.
 
-Prints ONE JSON line (rank 0).  value = mode-samples/s = K*N / step time.
+The headline path is the mixed-precision factorised sum (FP64 phases, FP32
+MFMA); the plain FP64 GPU sum (the reference's arithmetic) is timed beside it
+as the denominator of the speed-up.  Prints ONE JSON line.
 """
 
 from __future__ import annotations
@@ -23,8 +25,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAKS = {"fp64": 78.6, "fp32": 157.3, "hbm": 8000.0}  # TFLOP/s, TFLOP/s, GB/s (MI355X)
-FLOP_PER_MODE_SAMPLE = 37  # algorithmic FLOP of the direct sum per mode-sample (sincos counted 0)
+PEAK = {"fp64": 78.6, "fp32_mfma": 157.3}  # MI355X TFLOP/s (dense)
+DIRECT_FLOP = 37  # algorithmic FLOP per mode-sample of the direct sum (sincos counted 0)
 
 
 def log(msg: str) -> None:
@@ -32,7 +34,7 @@ def log(msg: str) -> None:
 
 
 def cpu_baseline(inp, seconds_target: float = 12.0):
-    """The oracle (single thread, the reference CPU backend is single-threaded) on a
+    """The oracle, single thread (the reference CPU backend is single-threaded), on a
     bounded sample of the same workload: evenly spread output samples, all modes."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle
@@ -40,7 +42,6 @@ def cpu_baseline(inp, seconds_target: float = 12.0):
     interp_tl, pcoef = oracle_coefficients(inp)
     args = (interp_tl, inp.t, pcoef, inp.m, inp.k, inp.n, inp.L, inp.num_pts, inp.K, inp.ylms,
             inp.dt, inp.t)
-    # calibrate on a small sample, then size the timed sample to ~seconds_target
     probe = np.linspace(0, inp.num_pts - 1, 200).astype(np.int64)
     t0 = time.perf_counter()
     oracle.waveform_samples(probe, *args, nthreads=1)
@@ -50,20 +51,35 @@ def cpu_baseline(inp, seconds_target: float = 12.0):
     t0 = time.perf_counter()
     oracle.waveform_samples(idx, *args, nthreads=1)
     el = time.perf_counter() - t0
-    rate = nsamp * inp.K / el
-    return {"value": rate, "unit": "mode-samples/s", "cores": 1, "kind": "port",
+    return {"value": nsamp * inp.K / el, "unit": "mode-samples/s", "cores": 1, "kind": "port",
             "sample": f"{nsamp} evenly spread output samples x {inp.K} modes of the same workload "
-                      f"({nsamp * inp.K:.3g} mode-samples, {el:.1f} s), oracle/few_oracle.c "
-                      f"restating interpolate.cu's CPU branch, -O2 -ffp-contract=off"}
+                      f"({nsamp * inp.K:.3g} mode-samples, {el:.1f} s); oracle/few_oracle.c "
+                      "restating interpolate.cu's CPU branch (-O2 -ffp-contract=off), 1 thread"}
 
 
-def rel_l2_vs_oracle(inp, h, count=1500):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle
-    from helpers import oracle_waveform, spread_indices
-    idx = spread_indices(inp.num_pts, count=count, seed=5)
-    ref = oracle_waveform(inp, idx, nthreads=min(16, os.cpu_count() or 1))
-    return oracle.rel_l2(h[idx], ref)
+def time_path(prob, precision, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        prob.run(precision)
+    torch.cuda.synchronize()
+    ev = {k: [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)] for k in ("spline", "prepare", "sum")}
+    t0 = time.perf_counter()
+    for s in range(steps):
+        prob.wave.zero_()
+        ev["spline"][s][0].record()
+        prob.build_splines()
+        ev["spline"][s][1].record()
+        ev["prepare"][s][0].record()
+        prob.mode_sum(precision, stage=1)
+        ev["prepare"][s][1].record()
+        ev["sum"][s][0].record()
+        prob.mode_sum(precision, stage=2)
+        ev["sum"][s][1].record()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms = {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in ev.items()}
+    return el / steps * 1e3, ms
 
 
 def main():
@@ -72,67 +88,72 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="full1")
-    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--precision", default="mixed", help="headline path: mixed | fp64")
+    ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sum shards by "
-              "time with no exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path in this build (the sum "
+              "shards by time with no exchange; see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    from helpers import oracle_waveform, spread_indices
     from fastemriwaveforms_amd import synthetic
     from fastemriwaveforms_amd.pipeline import ModeSumProblem
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    log(f"[bench] building workload {args.workload}")
+    log(f"[bench] workload {args.workload}")
     inp = synthetic.make_workload(args.workload)
     prob = ModeSumProblem(inp, dev)
+    prob.variant = args.variant
     K, N = inp.K, inp.num_pts
-    log(f"[bench] K={K} N={N} L={inp.L} mode-samples={K * N:.4g}")
+    log(f"[bench] K={K} N={N} L={inp.L} mode-samples={K * N:.4g} plan rows={prob.plan.rows} "
+        f"cols={prob.plan.cols} supported={prob.plan.supported}")
 
-    for _ in range(args.warmup):
-        prob.run(args.precision)
-    torch.cuda.synchronize()
-    log("[bench] warmup done")
-
-    # live per-kernel timing of the dominant kernel (mode sum) with events on
-    # the stream the kernels are launched on (torch's current stream)
-    ev_sum = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
-    ev_spl = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        prob.wave.zero_()
-        ev_spl[s][0].record()
-        prob.build_splines()
-        ev_spl[s][1].record()
-        ev_sum[s][0].record()
-        prob.mode_sum(args.precision)
-        ev_sum[s][1].record()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    ms_step = el / args.steps * 1e3
-    sum_ms = float(np.mean([a.elapsed_time(b) for a, b in ev_sum]))
-    spl_ms = float(np.mean([a.elapsed_time(b) for a, b in ev_spl]))
-    value = K * N / (el / args.steps)
-    log(f"[bench] step {ms_step:.3f} ms, mode-sum {sum_ms:.3f} ms, spline {spl_ms:.4f} ms")
-
+    ms_step, ms = time_path(prob, args.precision, args.steps, args.warmup)
     h = prob.wave.cpu().numpy()
-    err = rel_l2_vs_oracle(inp, h)
-    log(f"[bench] rel-L2 vs oracle {err:.3e}")
+    value = K * N / (ms_step * 1e-3)
+    log(f"[bench] {args.precision}: step {ms_step:.3f} ms | spline {ms['spline']:.3f} "
+        f"prepare {ms['prepare']:.3f} sum {ms['sum']:.3f} ms | {value:.4g} mode-samples/s")
 
-    flops = FLOP_PER_MODE_SAMPLE * K * N
-    achieved = flops / (sum_ms * 1e-3) / 1e12
-    peak = PEAKS["fp64"]
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None,
-                "kernel": "direct_fp64_kernel", "pipe": "FP64 VALU (no MFMA in this kernel)",
-                "algorithmic": f"{FLOP_PER_MODE_SAMPLE} FLOP/mode-sample x {K * N:.4g} "
-                               "mode-samples per launch, sincos counted 0"}
+    idx = spread_indices(N, count=1500, seed=5)
+    ref = oracle_waveform(inp, idx, nthreads=min(16, os.cpu_count() or 1))
+    err = oracle.rel_l2(h[idx], ref)
+    log(f"[bench] rel-L2 vs oracle ({len(idx)} samples) {err:.3e}")
+
+    fp64 = None
+    if not args.no_fp64 and args.precision != "fp64":
+        ms64_step, ms64 = time_path(prob, "fp64", max(2, args.steps // 3), 1)
+        h64 = prob.wave.cpu().numpy()
+        fp64 = {"ms_per_step": ms64_step, "mode_samples_per_s": K * N / (ms64_step * 1e-3),
+                "kernel_ms": ms64, "rel_l2_vs_oracle": oracle.rel_l2(h64[idx], ref),
+                "headline_rel_l2_vs_fp64_all_samples": oracle.rel_l2(h, h64),
+                "headline_mismatch_vs_fp64": float(oracle.mismatch(h, h64))}
+        log(f"[bench] fp64: step {ms64_step:.3f} ms, speed-up {ms64_step / ms_step:.2f}x, "
+            f"headline rel-L2 vs fp64 {fp64['headline_rel_l2_vs_fp64_all_samples']:.3e}")
+
+    if args.precision == "mixed" and prob.plan.supported:
+        flop_per_sample = 8 * 4 * prob.plan.rows * prob.plan.cols
+        flops = flop_per_sample * N
+        achieved = flops / (ms["sum"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma"],
+                    "unit": "TFLOP/s", "frac": achieved / PEAK["fp32_mfma"], "traffic": None,
+                    "kernel": "fact_f32_kernel",
+                    "algorithmic": f"8 FLOP x 4 powers x {prob.plan.rows} rows x {prob.plan.cols} "
+                                   f"cols = {flop_per_sample} FLOP/sample x {N} samples per launch"}
+    else:
+        flops = DIRECT_FLOP * K * N
+        achieved = flops / (ms["sum"] * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                    "frac": achieved / PEAK["fp64"], "traffic": None, "kernel": "direct_fp64_kernel",
+                    "pipe": "FP64 VALU", "algorithmic": f"{DIRECT_FLOP} FLOP/mode-sample x {K * N} "
+                                                        "mode-samples, sincos counted 0"}
 
     cpu = None
     if not args.no_cpu:
@@ -141,7 +162,7 @@ def main():
         log(f"[bench] CPU {cpu['value']:.4g} mode-samples/s")
 
     out = {
-        "metric": "mode-samples/s (interpolated mode sum, spline build included)",
+        "metric": "mode-samples/s of the interpolated mode sum at rel-L2 < 1e-6 vs the FP64 oracle",
         "value": value,
         "unit": "mode-samples/s",
         "n_gpus": 1,
@@ -151,14 +172,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded; knots, amplitudes, phases, Y_lm as described in synthetic.py)",
+        "dtype": "f64 phases + f32 MFMA" if args.precision == "mixed" else "f64",
+        "data": "synthetic, seeded (fastemriwaveforms_amd/synthetic.py)",
         "config": {"workload": f"{args.workload}: T={inp.T_years} yr, dt={inp.dt} s, N={N} samples, "
                                f"L={inp.L} knots, K={K} modes",
                    "precision": args.precision, "mode_samples": K * N},
         "waveforms_per_s": 1e3 / ms_step,
         "rel_l2_vs_oracle": err,
-        "kernel_ms": {"mode_sum": sum_ms, "spline_build": spl_ms},
+        "kernel_ms": ms,
+        "fp64_plain_gpu": fp64,
+        "speedup_vs_fp64_plain_gpu": (fp64["ms_per_step"] / ms_step) if fp64 else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

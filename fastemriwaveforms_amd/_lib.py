@@ -38,6 +38,12 @@ def _declare(lib: ctypes.CDLL) -> None:
                                     c_long, c_int, c_int, c_int, _d, _d], c_int),
         "few_prepare_modes_async": ([_d, _d, _d, c_int, c_int, _d, _d, _d, _d], c_int),
         "few_release_workspace": ([], c_int),
+        "few_mode_plan_create": ([_d, _d, _d, c_int], ctypes.c_void_p),
+        "few_mode_plan_info": ([_d, _d], c_int),
+        "few_mode_plan_destroy": ([_d], None),
+        "few_plan_workspace_bytes": ([_d, c_int, c_int, c_int], c_size),
+        "few_get_waveform_plan_async": ([_d, _d, _d, _d, _d, _d, _d, _d, c_int, c_int, c_int, _d,
+                                         c_double, _d, c_int, c_int, c_int, _d, c_size, _d], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -72,5 +78,7 @@ def exported_symbols() -> list[str]:
     return [
         "few_last_error", "few_abi_version", "few_interpolate_arrays", "few_get_waveform",
         "few_get_waveform_async", "few_waveform_workspace_bytes", "few_spline_build_async",
-        "few_prepare_modes_async", "few_release_workspace",
+        "few_prepare_modes_async", "few_release_workspace", "few_mode_plan_create",
+        "few_mode_plan_info", "few_mode_plan_destroy", "few_plan_workspace_bytes",
+        "few_get_waveform_plan_async",
     ]

@@ -18,6 +18,7 @@
 #include "few_common.hpp"
 #include "few_rocm.h"
 #include "modesum_direct.hpp"
+#include "modesum_fact.hpp"
 
 namespace few {
 
@@ -87,6 +88,14 @@ static PlanLayout plan_layout(int L)
     return p;
 }
 
+// Workspace of the factorised path: plan layout, then the coefficient tiles.
+static size_t fact_coef_offset(int L) { return plan_layout(L).total; }
+static size_t fact_ws_bytes(const FactPlanHost &p, int L)
+{
+    return fact_coef_offset(L) +
+           align256(sizeof(float) * fact_coef_floats_per_interval(p) * (size_t)(L > 1 ? L - 1 : 1));
+}
+
 struct DeviceGuard {
     int prev = -1;
     int ok = FEW_OK;
@@ -123,7 +132,8 @@ static int check_mode_args(const void *wave, const double *interp, const double 
 static int run_waveform(double2 *wave, const double *interp, const double *pt, const double *pcoef,
                         const int *m, const int *k, const int *n, int L, int out_len, int K,
                         const double2 *ylm, double dt, const double *h_t, int precision, char *ws,
-                        size_t ws_bytes, hipStream_t stream)
+                        size_t ws_bytes, hipStream_t stream, const FactPlan *plan = nullptr,
+                        int variant = 0, int stage = 3)
 {
     if (out_len == 0 || K == 0) return FEW_OK;
     const PlanLayout pl = plan_layout(L);
@@ -133,14 +143,52 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
     }
     Segment *segs = reinterpret_cast<Segment *>(ws + pl.segs);
     int *prefix = reinterpret_cast<int *>(ws + pl.prefix);
+    if (precision == FEW_PRECISION_MIXED) {
+        if (!plan || !plan->host.ok) {
+            set_error("get_waveform: mixed precision needs a supported mode plan");
+            return FEW_ERR_UNSUPPORTED;
+        }
+        const FactPlanHost &ph = plan->host;
+        if (ph.K != K) {
+            set_error("get_waveform: plan built for %d modes, called with %d", ph.K, K);
+            return FEW_ERR_ARG;
+        }
+        if (ws_bytes < fact_ws_bytes(ph, L)) {
+            set_error("get_waveform: workspace of %zu bytes, need %zu", ws_bytes,
+                      fact_ws_bytes(ph, L));
+            return FEW_ERR_ARG;
+        }
+        float *C = reinterpret_cast<float *>(ws + fact_coef_offset(L));
+        const long c_stride = (long)fact_coef_floats_per_interval(ph);
+        const int tile = fact_tile(ph, variant);
+        const long max_blocks = ((long)out_len + tile - 1) / tile + (L - 1);
+        if (max_blocks > 0x7fffffffL) {
+            set_error("get_waveform: output too long (%d samples)", out_len);
+            return FEW_ERR_LIMIT;
+        }
+        if (stage & 1) {
+            PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
+            int rc = launch_plan(p, stream);
+            if (rc) return rc;
+            FactPrepArgs pa{interp, ylm, h_t, L, C, c_stride, plan->dev};
+            rc = launch_fact_prep(pa, stream);
+            if (rc) return rc;
+        }
+        if (!(stage & 2)) return FEW_OK;
+        FactArgs fa{wave, pcoef, segs, prefix, L - 1, dt, C, c_stride, plan->dev};
+        return launch_fact_sum(fa, ph, (int)max_blocks, variant, stream);
+    }
     if (precision != FEW_PRECISION_FP64) {
         set_error("get_waveform: precision %d is not available in this build", precision);
         return FEW_ERR_UNSUPPORTED;
     }
     const int tile = direct_tile(precision);
-    PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
-    int rc = launch_plan(p, stream);
-    if (rc) return rc;
+    if (stage & 1) {
+        PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
+        int rc = launch_plan(p, stream);
+        if (rc) return rc;
+    }
+    if (!(stage & 2)) return FEW_OK;
     DirectArgs a;
     a.wave = wave;
     a.interp = interp;
@@ -314,6 +362,112 @@ int few_get_waveform_async(void *d_waveform, const double *interp_array, const d
                         phase_interp_coeffs, d_m, d_k, d_n, init_len, out_len, num_teuk_modes,
                         static_cast<const double2 *>(d_Ylms), delta_t, d_h_t, precision, w,
                         workspace_bytes, reinterpret_cast<hipStream_t>(stream));
+}
+
+void *few_mode_plan_create(const int *h_m, const int *h_k, const int *h_n, int num_modes)
+{
+    if (!h_m || !h_k || !h_n || num_modes <= 0) {
+        set_error("mode_plan_create: need host m/k/n arrays and modes > 0");
+        return nullptr;
+    }
+    FactPlan *pl = new FactPlan();
+    build_fact_plan(h_m, h_k, h_n, num_modes, pl->host);
+    FactPlanHost &h = pl->host;
+    pl->dev.K = h.K;
+    pl->dev.Nr = h.Nr;
+    pl->dev.Nc = h.Nc;
+    pl->dev.n_lo = h.n_lo;
+    pl->dev.RB = h.RB;
+    pl->dev.KS = h.KS;
+    pl->dev.CST = h.CST;
+    if (!h.ok) return pl;  // usable for the direct paths; mixed reports unsupported
+    std::vector<int> buf;
+    for (int r = 0; r < h.Nr; r++) {
+        buf.push_back(h.row_m[r]);
+        buf.push_back(h.row_k[r]);
+    }
+    const size_t o_cell = buf.size();
+    buf.insert(buf.end(), h.cell_start.begin(), h.cell_start.end());
+    const size_t o_ent = buf.size();
+    buf.insert(buf.end(), h.entries.begin(), h.entries.end());
+    if (buf.empty()) buf.push_back(0);
+    pl->dev_bytes = buf.size() * sizeof(int);
+    if (hipGetDevice(&pl->device) != hipSuccess ||
+        hipMalloc(&pl->dev_buf, pl->dev_bytes) != hipSuccess ||
+        hipMemcpy(pl->dev_buf, buf.data(), pl->dev_bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("mode_plan_create: device upload failed");
+        if (pl->dev_buf) (void)hipFree(pl->dev_buf);
+        delete pl;
+        return nullptr;
+    }
+    int *d = static_cast<int *>(pl->dev_buf);
+    pl->dev.row_mk = reinterpret_cast<const int2 *>(d);
+    pl->dev.cell_start = d + o_cell;
+    pl->dev.entries = d + o_ent;
+    return pl;
+}
+
+int few_mode_plan_info(const void *plan, int *info7)
+{
+    if (!plan || !info7) {
+        set_error("mode_plan_info: null argument");
+        return FEW_ERR_ARG;
+    }
+    const FactPlanHost &h = static_cast<const FactPlan *>(plan)->host;
+    info7[0] = h.K;
+    info7[1] = h.Nr;
+    info7[2] = h.Nc;
+    info7[3] = h.n_lo;
+    info7[4] = h.RB;
+    info7[5] = h.KS;
+    info7[6] = h.ok ? 1 : 0;
+    return FEW_OK;
+}
+
+void few_mode_plan_destroy(void *plan)
+{
+    if (!plan) return;
+    FactPlan *pl = static_cast<FactPlan *>(plan);
+    if (pl->dev_buf) (void)hipFree(pl->dev_buf);
+    delete pl;
+}
+
+size_t few_plan_workspace_bytes(const void *plan, int init_len, int out_len, int precision)
+{
+    (void)out_len;
+    const int L = init_len < 2 ? 2 : init_len;
+    size_t b = plan_layout(L).total;
+    if (plan && precision == FEW_PRECISION_MIXED) {
+        const FactPlanHost &h = static_cast<const FactPlan *>(plan)->host;
+        if (h.ok) b = fact_ws_bytes(h, L);
+    }
+    return b;
+}
+
+int few_get_waveform_plan_async(void *d_waveform, const double *interp_array,
+                                const double *d_phase_interp_t, const double *phase_interp_coeffs,
+                                const void *plan, const int *d_m, const int *d_k, const int *d_n,
+                                int init_len, int out_len, int num_teuk_modes, const void *d_Ylms,
+                                double delta_t, const double *d_h_t, int precision, int variant,
+                                int stage, void *workspace, size_t workspace_bytes, void *stream)
+{
+    int rc = check_mode_args(d_waveform, interp_array, phase_interp_coeffs, d_m, d_k, d_n, init_len,
+                             out_len, num_teuk_modes, d_Ylms, delta_t);
+    if (rc) return rc;
+    if (!d_phase_interp_t || !d_h_t || !workspace) {
+        set_error("get_waveform_plan_async: null device time array or workspace");
+        return FEW_ERR_ARG;
+    }
+    if (stage < 1 || stage > 3) {
+        set_error("get_waveform_plan_async: stage must be 1 (prepare), 2 (sum) or 3 (both)");
+        return FEW_ERR_ARG;
+    }
+    return run_waveform(static_cast<double2 *>(d_waveform), interp_array, d_phase_interp_t,
+                        phase_interp_coeffs, d_m, d_k, d_n, init_len, out_len, num_teuk_modes,
+                        static_cast<const double2 *>(d_Ylms), delta_t, d_h_t, precision,
+                        static_cast<char *>(workspace), workspace_bytes,
+                        reinterpret_cast<hipStream_t>(stream), static_cast<const FactPlan *>(plan),
+                        variant, stage);
 }
 
 int few_release_workspace(void)

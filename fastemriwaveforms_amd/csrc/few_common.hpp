@@ -51,6 +51,7 @@ struct Segment {
     double h_t;       // amplitude-spline knot time of the interval
     double pt;        // phase dense-output segment start
     double pw;        // phase dense-output segment width
+    double hw;        // amplitude knot spacing h_t[j+1] - h_t[j]
 };
 
 }  // namespace few

@@ -62,6 +62,7 @@ __global__ __launch_bounds__(256) void plan_segments_kernel(PlanArgs a)
             sg.h_t = a.h_t[j];
             sg.pt = a.pt[j];
             sg.pw = a.pt[j + 1] - a.pt[j];
+            sg.hw = a.h_t[j + 1] - a.h_t[j];
             a.segs[j] = sg;
             nt = (en - st + a.tile - 1) / a.tile;
         }

@@ -1,0 +1,95 @@
+// Factorised mode sum: shared declarations.
+//
+// The phase of mode q is m_q Phi_phi + k_q Phi_theta + n_q Phi_r, so
+//   exp(-i phase_q) = u_r(t) * v_n(t),
+//   u_r = exp(-i (m Phi_phi + k Phi_theta)),  v_n = exp(-i n Phi_r).
+// Within knot interval j the amplitude of mode q is a cubic in
+// s = (t - t_j) / (t_{j+1} - t_j).  Grouping modes (and the conjugate
+// -m partners of interpolate.cu:665-677) by row r = signed (m, k) and
+// column n gives
+//   h(t) = sum_p s^p sum_r u_r(t) sum_n C_p[r, n] v_n(t),
+// where C_p[r, n] (complex) collects Y+ c_{p,q} h_j^p and Y- conj(c_{p,q}) h_j^p.
+// For a tile of samples inside one interval the inner sum over n is a dense
+// GEMM (rows (r, p) x columns n) x (n x samples), run on MFMA; the outer sums
+// are an elementwise epilogue.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "few_common.hpp"
+
+namespace few {
+
+// Host-built description of a mode set (rows / columns / gather lists).
+struct FactPlanHost {
+    int K = 0;              // modes
+    int Nr = 0;             // rows: distinct signed (m, k)
+    int Nc = 0;             // columns: n_lo .. n_lo + Nc - 1
+    int n_lo = 0;
+    int RB = 0;             // 16-row MFMA blocks: ceil(8 * Nr / 16) (4 powers x re/im)
+    int KS = 0;             // 4-deep k-steps: ceil(2 * Nc / 4)
+    int CST = 0;            // LDS row stride in complex entries (== 2 mod 16)
+    std::vector<int> row_m, row_k;        // (Nr,)
+    std::vector<int> cell_start;          // (Nr*Nc + 1,)
+    std::vector<int> entries;             // q*2 + conj, grouped by cell, q ascending
+    bool ok = false;
+};
+
+// Device-side view of an uploaded plan (all pointers into one allocation).
+struct FactPlanDev {
+    int K, Nr, Nc, n_lo, RB, KS, CST;
+    const int2 *row_mk;      // (Nr,)
+    const int *cell_start;   // (Nr*Nc + 1,)
+    const int *entries;
+};
+
+struct FactPlan {
+    FactPlanHost host;
+    FactPlanDev dev;
+    void *dev_buf = nullptr;
+    size_t dev_bytes = 0;
+    int device = -1;
+};
+
+// Largest supported sizes of the MFMA kernels (others fall back to the direct sum).
+#define FACT_MAX_RB 16
+#define FACT_MAX_NC 512
+
+bool build_fact_plan(const int *m, const int *k, const int *n, int K, FactPlanHost &p);
+
+// Coefficient tiles: per interval, (RB*8 rows) x (CST complex) float2.
+inline size_t fact_coef_floats_per_interval(const FactPlanHost &p)
+{
+    return (size_t)p.RB * 8 * p.CST * 2;
+}
+
+struct FactArgs {
+    double2 *wave;
+    const double *pcoef;          // (8, L-1, 3)
+    const Segment *segs;
+    const int *prefix;
+    int nseg;
+    double dt;
+    const float *C;               // (L-1) x RB*8 x CST x 2
+    long c_stride;                // floats per interval
+    FactPlanDev plan;
+};
+
+struct FactPrepArgs {
+    const double *interp;         // (4, L, 2K)
+    const double2 *ylm;           // (2K,)
+    const double *h_t;            // (L,) amplitude knots (device)
+    int L;
+    float *C;
+    long c_stride;
+    FactPlanDev plan;
+};
+
+int launch_fact_prep(const FactPrepArgs &a, hipStream_t stream);
+int fact_tile(const FactPlanHost &p, int variant);
+int launch_fact_sum(const FactArgs &a, const FactPlanHost &p, int max_blocks, int variant,
+                    hipStream_t stream);
+
+}  // namespace few

@@ -22,7 +22,36 @@ from . import _lib
 from ._lib import check, lib
 from .synthetic import ModeSumInput
 
-PRECISION_CODES = {"fp64": _lib.FEW_PRECISION_FP64}
+PRECISION_CODES = {"fp64": _lib.FEW_PRECISION_FP64, "mixed": _lib.FEW_PRECISION_MIXED}
+
+
+class ModePlan:
+    """Host-built, device-uploaded description of a mode set (rows = signed (m, k),
+    columns = n, gather lists) used by the factorised sum.  Create once per mode set."""
+
+    def __init__(self, m, k, n):
+        self.m = np.ascontiguousarray(m, np.int32)
+        self.k = np.ascontiguousarray(k, np.int32)
+        self.n = np.ascontiguousarray(n, np.int32)
+        self.K = len(self.m)
+        self.handle = lib().few_mode_plan_create(self.m.ctypes.data, self.k.ctypes.data,
+                                                 self.n.ctypes.data, self.K)
+        if not self.handle:
+            check(1, "mode_plan_create")
+        info = np.zeros(7, np.int32)
+        check(lib().few_mode_plan_info(self.handle, info.ctypes.data), "mode_plan_info")
+        self.rows, self.cols, self.n_lo = int(info[1]), int(info[2]), int(info[3])
+        self.row_blocks, self.ksteps = int(info[4]), int(info[5])
+        self.supported = bool(info[6])
+
+    def workspace_bytes(self, L, N, precision) -> int:
+        return int(lib().few_plan_workspace_bytes(self.handle, L, N, PRECISION_CODES[precision]))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            lib().few_mode_plan_destroy(h)
+            self.handle = None
 
 
 def _stream() -> int:
@@ -48,9 +77,10 @@ class ModeSumProblem:
         self.pcoef = torch.empty((8, L - 1, 3), dtype=f64, device=dev)
         self.scratch = torch.empty(4 * L * (2 * K + 3), dtype=f64, device=dev)
         self.wave = torch.zeros(self.N, dtype=torch.complex128, device=dev)
-        nbytes = max(int(lib().few_waveform_workspace_bytes(L, self.N, K, code))
-                     for code in PRECISION_CODES.values())
+        self.plan = ModePlan(inp.m, inp.k, inp.n)
+        nbytes = max(self.plan.workspace_bytes(L, self.N, p) for p in PRECISION_CODES)
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.variant = 0
 
     @property
     def mode_samples(self) -> int:
@@ -63,12 +93,14 @@ class ModeSumProblem:
             self.scratch.data_ptr(), _stream())
         check(rc, "prepare_modes")
 
-    def mode_sum(self, precision: str = "fp64") -> None:
-        rc = lib().few_get_waveform_async(
+    def mode_sum(self, precision: str = "fp64", stage: int = 3) -> None:
+        """stage: 1 = plan intervals + coefficient tiles, 2 = the summation kernel, 3 = both."""
+        rc = lib().few_get_waveform_plan_async(
             self.wave.data_ptr(), self.interp.data_ptr(), self.t.data_ptr(), self.pcoef.data_ptr(),
-            self.m.data_ptr(), self.k.data_ptr(), self.n.data_ptr(), self.L, self.N, self.K,
-            self.ylm.data_ptr(), self.dt, self.t.data_ptr(), PRECISION_CODES[precision],
-            self.ws.data_ptr(), self.ws.numel(), _stream())
+            self.plan.handle, self.m.data_ptr(), self.k.data_ptr(), self.n.data_ptr(), self.L,
+            self.N, self.K, self.ylm.data_ptr(), self.dt, self.t.data_ptr(),
+            PRECISION_CODES[precision], self.variant, stage, self.ws.data_ptr(), self.ws.numel(),
+            _stream())
         check(rc, "get_waveform")
 
     def run(self, precision: str = "fp64") -> torch.Tensor:

@@ -27,6 +27,7 @@ extern "C" {
 #define FEW_ROCM_ABI_VERSION 1
 
 #define FEW_PRECISION_FP64 0   /* plain FP64 direct sum (reference arithmetic) */
+#define FEW_PRECISION_MIXED 2  /* factorised MFMA sum: FP64 phases, FP32 MFMA, FP64 output */
 
 const char *few_last_error(void);
 int few_abi_version(void);
@@ -113,6 +114,39 @@ int few_spline_build_async(const double *y, long y_sys, long y_knot, const doubl
 int few_prepare_modes_async(const double *teuk, const double *phase_knots, const double *t,
                             int length, int num_modes, double *interp_out, double *pcoef_out,
                             double *scratch, void *stream);
+
+/*
+ * Mode plan for the factorised sum (no reference counterpart; the reference
+ * re-reads m/k/n per launch, src/few/cutils/interpolate.cu:591-593).  Built
+ * on the host from HOST copies of the int32 mode indices and uploaded to the
+ * current device once; reused by every call with the same mode set.
+ * Returns NULL on error.  info7 = {K, rows, cols, n_lo, row blocks,
+ * k-steps, supported}; unsupported sets (too many rows/columns for the MFMA
+ * tiles) can still use FEW_PRECISION_FP64.
+ */
+void *few_mode_plan_create(const int *h_m, const int *h_k, const int *h_n, int num_modes);
+int few_mode_plan_info(const void *plan, int *info7);
+void few_mode_plan_destroy(void *plan);
+size_t few_plan_workspace_bytes(const void *plan, int init_len, int out_len, int precision);
+
+/*
+ * get_waveform (src/few/cutils/interpolate.cu:726-797) with a mode plan and a
+ * precision choice, asynchronous on device-resident time arrays.
+ * FEW_PRECISION_FP64 runs the reference arithmetic; FEW_PRECISION_MIXED the
+ * factorised MFMA sum (rel-L2 ~1e-7 against the FP64 result).  `variant`
+ * selects the tile shape (0 = default, 2 = narrow).  `stage` is a bitmask:
+ * 1 = prepare (interval planning, coefficient tiles), 2 = sum (the output
+ * kernel), 3 = both; stage 2 reuses what the last stage-1 call on the same
+ * workspace prepared.  Workspace:
+ * few_plan_workspace_bytes() bytes, caller-owned, one call in flight at a
+ * time.  No host synchronisation, no allocation: capturable in a hipGraph.
+ */
+int few_get_waveform_plan_async(void *d_waveform, const double *interp_array,
+                                const double *d_phase_interp_t, const double *phase_interp_coeffs,
+                                const void *plan, const int *d_m, const int *d_k, const int *d_n,
+                                int init_len, int out_len, int num_teuk_modes, const void *d_Ylms,
+                                double delta_t, const double *d_h_t, int precision, int variant,
+                                int stage, void *workspace, size_t workspace_bytes, void *stream);
 
 /* Frees the library-owned workspaces of the synchronous drop-in entries. */
 int few_release_workspace(void);
