@@ -380,6 +380,9 @@ void *few_mode_plan_create(const int *h_m, const int *h_k, const int *h_n, int n
     pl->dev.RB = h.RB;
     pl->dev.KS = h.KS;
     pl->dev.CST = h.CST;
+    pl->dev.rows_arith = h.rows_arith;
+    pl->dev.m0 = h.m0;
+    pl->dev.k0 = h.k0;
     if (!h.ok) return pl;  // usable for the direct paths; mixed reports unsupported
     std::vector<int> buf;
     for (int r = 0; r < h.Nr; r++) {

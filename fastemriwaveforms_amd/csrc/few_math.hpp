@@ -1,0 +1,53 @@
+// Small host+device math helpers of the factorised sum.
+//
+// sincos_turns(f): sin and cos of 2*pi*f for f in [-1/2, 1/2] (an angle in
+// turns already reduced in FP64).  The quadrant q = rint(4f) is removed
+// exactly (f - q/4 is exact in binary floating point), leaving
+// |2*pi*r| <= pi/4, where the Taylor series of sin and cos are summed by
+// Horner's rule in FP64 up to x^17 and x^18 (truncation error below
+// (pi/4)^19/19! ~ 1e-19), then rotated back by the quadrant.
+#pragma once
+
+#include <math.h>
+
+namespace few {
+
+__host__ __device__ inline void sincos_turns(double f, double &s, double &c)
+{
+    const double q = rint(4.0 * f);
+    const double r = f - 0.25 * q;
+    const double x = r * 6.283185307179586476925286766559;  // 2*pi (double)
+    const double x2 = x * x;
+    // sin x = x (1 - x^2/3! + x^4/5! - ... + x^16/17!)
+    double ps = 2.8114572543455207632e-15;            // 1/17!
+    ps = ps * x2 - 7.6471637318198164759e-13;         // -1/15!
+    ps = ps * x2 + 1.6059043836821614599e-10;         // 1/13!
+    ps = ps * x2 - 2.5052108385441718775e-08;         // -1/11!
+    ps = ps * x2 + 2.7557319223985890653e-06;         // 1/9!
+    ps = ps * x2 - 1.9841269841269841270e-04;         // -1/7!
+    ps = ps * x2 + 8.3333333333333333333e-03;         // 1/5!
+    ps = ps * x2 - 1.6666666666666666667e-01;         // -1/3!
+    const double sn = x + x * x2 * ps;
+    // cos x = 1 - x^2/2! + x^4/4! - ... + x^18/18!
+    double pc = 1.5619206968586225351e-16;            // 1/18!
+    pc = pc * x2 - 4.7794773323873852974e-14;         // -1/16!
+    pc = pc * x2 + 1.1470745597729724714e-11;         // 1/14!
+    pc = pc * x2 - 2.0876756987868098979e-09;         // -1/12!
+    pc = pc * x2 + 2.7557319223985890653e-07;         // 1/10!
+    pc = pc * x2 - 2.4801587301587301587e-05;         // -1/8!
+    pc = pc * x2 + 1.3888888888888888889e-03;         // 1/6!
+    pc = pc * x2 - 4.1666666666666666667e-02;         // -1/4!
+    pc = pc * x2 + 0.5;
+    const double cs = 1.0 - x2 * pc;
+    // rotate by q quarter turns
+    const int qi = ((int)q) & 3;
+    if (qi == 0) { s = sn; c = cs; }
+    else if (qi == 1) { s = cs; c = -sn; }
+    else if (qi == 2) { s = -sn; c = -cs; }
+    else { s = -cs; c = sn; }
+}
+
+// Angle in turns reduced to [-1/2, 1/2]: a - rint(a).
+__host__ __device__ inline double reduce_turns(double a) { return a - rint(a); }
+
+}  // namespace few

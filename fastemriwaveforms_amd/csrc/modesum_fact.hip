@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <map>
 
+#include "few_math.hpp"
 #include "modesum_direct.hpp"
 #include "modesum_fact.hpp"
 
@@ -80,6 +81,11 @@ bool build_fact_plan(const int *m, const int *k, const int *n, int K, FactPlanHo
         p.entries[fill[cell_of(m[q], k[q], n[q])]++] = 2 * q;
         if (m[q] != 0) p.entries[fill[cell_of(-m[q], -k[q], -n[q])]++] = 2 * q + 1;
     }
+    p.rows_arith = 1;
+    for (int rr = 0; rr < p.Nr; rr++)
+        if (p.row_m[rr] != p.row_m[0] + rr || p.row_k[rr] != p.row_k[0]) p.rows_arith = 0;
+    p.m0 = p.row_m[0];
+    p.k0 = p.row_k[0];
     p.ok = (p.RB <= FACT_MAX_RB) && (p.Nc <= FACT_MAX_NC);
     return p.ok;
 }
@@ -152,39 +158,17 @@ __device__ __forceinline__ int fact_find_segment(const int *prefix, int nseg, in
     return lo;
 }
 
+// One 64*NB-sample tile of interval j (coefficients already in LDS).
 template <int RB, int NB>
-__global__ __launch_bounds__(256) void fact_f32_kernel(FactArgs a)
+__device__ __forceinline__ void fact_tile_body(const FactArgs &a, const FactPlanDev &P,
+                                               const float *sC, const Segment &sg, int q,
+                                               const double *sPh, int wv, int li, int g)
 {
-    extern __shared__ float sC[];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wv = tid >> 6;
-    const int li = lane & 15;
-    const int g = lane >> 4;
-    const int b = blockIdx.x;
-    const int total = a.prefix[a.nseg];
-    if (b >= total) return;
-    const int j = fact_find_segment(a.prefix, a.nseg, b);
-    const Segment sg = a.segs[j];
-    const int q = b - a.prefix[j];
-    const FactPlanDev P = a.plan;
-
-    {   // stage the interval's coefficient tile
-        const float4 *src = reinterpret_cast<const float4 *>(a.C + (long)j * a.c_stride);
-        float4 *dst = reinterpret_cast<float4 *>(sC);
-        const int nv = (int)(a.c_stride >> 2);
-        for (int i = tid; i < nv; i += 256) dst[i] = src[i];
-    }
-
-    const long np = (long)a.nseg * 3;
-    double rp[8], rt[8], rr[8];
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        rp[c] = a.pcoef[c * np + (long)j * 3 + 0];
-        rt[c] = a.pcoef[c * np + (long)j * 3 + 1];
-        rr[c] = a.pcoef[c * np + (long)j * 3 + 2];
-    }
-
+    // sPh: the interval's dense-output phase coefficients in LDS,
+    // [0..7] Phi_phi, [8..15] Phi_theta, [16..23] Phi_r
+    const double *rp = sPh;
+    const double *rt = sPh + 8;
+    const double *rr = sPh + 16;
     constexpr int TILE = 64 * NB;
     int idx[NB];
     bool ok[NB];
@@ -199,20 +183,17 @@ __global__ __launch_bounds__(256) void fact_f32_kernel(FactArgs a)
         sa[c] = (float)((t - sg.h_t) / sg.hw);
         const double s = (t - sg.pt) / sg.pw;
         const double s1 = 1.0 - s;
-        const double Pp = dense_phase(rp, s, s1);
-        const double Pt = dense_phase(rt, s, s1);
-        const double Pr = dense_phase(rr, s, s1);
-        uph[c] = Pp * INV_2PI;
-        uth[c] = Pt * INV_2PI;
+        const double ur = dense_phase(rr, s, s1) * INV_2PI;  // Phi_r in turns
+        uph[c] = dense_phase(rp, s, s1) * INV_2PI;
+        uth[c] = dense_phase(rt, s, s1) * INV_2PI;
         double sn, cs;
-        sincos((double)n0 * Pr, &sn, &cs);
-        v_re[c] = cs;
+        sincos_turns(reduce_turns((double)n0 * ur), sn, cs);
+        v_re[c] = cs;   // v_n = exp(-i n Phi_r)
         v_im[c] = -sn;
-        sincos(2.0 * Pr, &sn, &cs);
-        w_re[c] = cs;
+        sincos_turns(reduce_turns(2.0 * ur), sn, cs);
+        w_re[c] = cs;   // exp(-2 i Phi_r)
         w_im[c] = -sn;
     }
-    __syncthreads();
 
     v4f acc[RB][NB];
 #pragma unroll
@@ -250,7 +231,8 @@ __global__ __launch_bounds__(256) void fact_f32_kernel(FactArgs a)
         }
     }
 
-    // epilogue: h(t) = sum_{r,p} s^p u_r(t) W[(r,p), t]
+    // epilogue: h(t) = sum_{r,p} s^p u_r(t) W[(r,p), t]; this lane holds rows
+    // r = 2 rb + (g >> 1), powers p0 = 2 (g & 1) and p0 + 1
     const int p0 = 2 * f;
 #pragma unroll
     for (int c = 0; c < NB; c++) {
@@ -259,21 +241,43 @@ __global__ __launch_bounds__(256) void fact_f32_kernel(FactArgs a)
         const float spa = p0 == 0 ? 1.f : s2f;
         const float spb = p0 == 0 ? s1f : s2f * s1f;
         double hre = 0.0, him = 0.0;
+        double u_re = 0.0, u_im = 0.0, e_re = 0.0, e_im = 0.0;
+        if (P.rows_arith) {
+            // u_r = exp(-2 pi i (m_r uph + k0 uth)), m_r = m0 + r: step by 2 rows
+            double sn, cs;
+            const double m_first = (double)(P.m0 + (g >> 1));
+            sincos_turns(reduce_turns(m_first * uph[c] + (double)P.k0 * uth[c]), sn, cs);
+            u_re = cs;
+            u_im = -sn;
+            sincos_turns(reduce_turns(2.0 * uph[c]), sn, cs);
+            e_re = cs;
+            e_im = -sn;
+        }
 #pragma unroll
         for (int rb = 0; rb < RB; rb++) {
             const int r = 2 * rb + (g >> 1);
+            float ur, ui;
+            if (P.rows_arith) {
+                ur = (float)u_re;
+                ui = (float)u_im;
+                const double nr = u_re * e_re - u_im * e_im;
+                const double ni = u_re * e_im + u_im * e_re;
+                u_re = nr;
+                u_im = ni;
+            } else {
+                const int2 mk = r < P.Nr ? P.row_mk[r] : make_int2(0, 0);
+                const double tu = reduce_turns((double)mk.x * uph[c] + (double)mk.y * uth[c]);
+                float sn, cs;
+                sincospif(2.0f * (float)tu, &sn, &cs);
+                ur = cs;
+                ui = -sn;
+            }
             if (r < P.Nr) {
                 const v4f W = acc[rb][c];
                 const float zr = spa * W[0] + spb * W[2];
                 const float zi = spa * W[1] + spb * W[3];
-                const int2 mk = P.row_mk[r];
-                double tu = (double)mk.x * uph[c] + (double)mk.y * uth[c];
-                tu -= rint(tu);
-                float sn, cs;
-                sincospif(2.0f * (float)tu, &sn, &cs);
-                // u = cs - i sn
-                hre += (double)(cs * zr + sn * zi);
-                him += (double)(cs * zi - sn * zr);
+                hre += (double)(ur * zr - ui * zi);
+                him += (double)(ur * zi + ui * zr);
             }
         }
         hre += __shfl_xor(hre, 16);
@@ -289,7 +293,66 @@ __global__ __launch_bounds__(256) void fact_f32_kernel(FactArgs a)
     }
 }
 
-int fact_tile(const FactPlanHost & /*p*/, int variant) { return 64 * (variant == 2 ? 2 : 4); }
+// Persistent: block b owns the contiguous tile range [b*per, (b+1)*per) of
+// the (interval, tile) sequence, so consecutive tiles share an interval and
+// the LDS coefficient tile is reloaded only when the interval changes.
+template <int RB, int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB <= 2 ? 2 : 1, NB <= 2 ? 2 : 1)))
+void fact_f32_kernel(FactArgs a)
+{
+    extern __shared__ float sC[];
+    double *sPh = reinterpret_cast<double *>(sC + a.c_stride);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = tid >> 6;
+    const int li = lane & 15;
+    const int g = lane >> 4;
+    const FactPlanDev P = a.plan;
+    const int total = a.prefix[a.nseg];
+    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * per;
+    const int t1 = min(total, t0 + per);
+    if (t0 >= t1) return;
+    int j = fact_find_segment(a.prefix, a.nseg, t0);
+    int jnext_start = a.prefix[j + 1];
+    const long np = (long)a.nseg * 3;
+    bool stale = true;
+    for (int tile = t0; tile < t1; tile++) {
+        while (tile >= jnext_start) {  // advance to the interval holding this tile
+            j++;
+            jnext_start = a.prefix[j + 1];
+            stale = true;
+        }
+        if (stale) {
+            __syncthreads();  // every wave is done reading the previous tile
+            const float4 *src = reinterpret_cast<const float4 *>(a.C + (long)j * a.c_stride);
+            float4 *dst = reinterpret_cast<float4 *>(sC);
+            const int nv = (int)(a.c_stride >> 2);
+            for (int i = tid; i < nv; i += 256) dst[i] = src[i];
+            if (tid < 24) sPh[tid] = a.pcoef[(tid & 7) * np + (long)j * 3 + (tid >> 3)];
+            __syncthreads();
+            stale = false;
+        }
+        const Segment sg = a.segs[j];
+        fact_tile_body<RB, NB>(a, P, sC, sg, tile - a.prefix[j], sPh, wv, li, g);
+    }
+}
+
+int fact_tile(const FactPlanHost & /*p*/, int variant) { return 64 * (variant == 4 ? 4 : 2); }
+
+static int device_cus()
+{
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cached[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
 
 template <int NB>
 static int launch_fact_nb(const FactArgs &a, int RB, int blocks, size_t lds, hipStream_t stream)
@@ -318,13 +381,17 @@ int launch_fact_sum(const FactArgs &a, const FactPlanHost &p, int max_blocks, in
                     hipStream_t stream)
 {
     if (max_blocks <= 0) return FEW_OK;
-    const size_t lds = (size_t)fact_coef_floats_per_interval(p) * sizeof(float);
+    const size_t lds = (size_t)fact_coef_floats_per_interval(p) * sizeof(float) + 24 * sizeof(double);
     if (lds > 160 * 1024) {
         set_error("factorised sum: coefficient tile of %zu bytes exceeds LDS", lds);
         return FEW_ERR_UNSUPPORTED;
     }
-    if (variant == 2) return launch_fact_nb<2>(a, p.RB, max_blocks, lds, stream);
-    return launch_fact_nb<4>(a, p.RB, max_blocks, lds, stream);
+    // persistent grid: enough resident blocks to fill every CU
+    const int per_cu = (variant == 4) ? 1 : (lds <= 80 * 1024 ? 2 : 1);
+    int blocks = device_cus() * per_cu;
+    if (blocks > max_blocks) blocks = max_blocks;
+    if (variant == 4) return launch_fact_nb<4>(a, p.RB, blocks, lds, stream);
+    return launch_fact_nb<2>(a, p.RB, blocks, lds, stream);
 }
 
 }  // namespace few

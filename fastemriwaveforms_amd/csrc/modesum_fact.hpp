@@ -31,6 +31,8 @@ struct FactPlanHost {
     int RB = 0;             // 16-row MFMA blocks: ceil(8 * Nr / 16) (4 powers x re/im)
     int KS = 0;             // 4-deep k-steps: ceil(2 * Nc / 4)
     int CST = 0;            // LDS row stride in complex entries (== 2 mod 16)
+    int rows_arith = 0;     // rows are m0, m0+1, ... with one k: phasors by recurrence
+    int m0 = 0, k0 = 0;
     std::vector<int> row_m, row_k;        // (Nr,)
     std::vector<int> cell_start;          // (Nr*Nc + 1,)
     std::vector<int> entries;             // q*2 + conj, grouped by cell, q ascending
@@ -40,6 +42,7 @@ struct FactPlanHost {
 // Device-side view of an uploaded plan (all pointers into one allocation).
 struct FactPlanDev {
     int K, Nr, Nc, n_lo, RB, KS, CST;
+    int rows_arith, m0, k0;
     const int2 *row_mk;      // (Nr,)
     const int *cell_start;   // (Nr*Nc + 1,)
     const int *entries;

@@ -25,6 +25,7 @@
 namespace few {
 
 #define SPLINE_MAX_GROUPS 4
+#define SPLINE_PF 8  // prefetch distance (steps) of the sequential sweeps
 
 struct SplineArgs {
     SplineGroup g[SPLINE_MAX_GROUPS];
@@ -93,22 +94,21 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
 #define YK(i) yv[(long)(i) * yk]
 
     // ---------------- forward elimination (dgtsv, NRHS = 1) ----------------
+    // The knot loads are issued SPLINE_PF steps ahead through a register ring
+    // so the sequential recurrence does not wait on memory each step; the
+    // arithmetic is unchanged.
     double xa = XK(0), xb = XK(1), xc = XK(2);
     double ya = YK(0), yb = YK(1), yc = YK(2);
     double cb, cdu, cd, cld;
     prep_row(0, n, xa, xb, xc, ya, yb, yc, cb, cdu, cd, cld);
     int info = 0;
     int stop = n - 1;  // rows [0, stop) are stored in scratch
-    for (int i = 0; i <= n - 2; i++) {
-        // row r = i+1: interior rows read (x[r-1], x[r], x[r+1]) -- row 1 reads
-        // the initial window; the last row reads (x[n-3], x[n-2], x[n-1]),
-        // the same window as row n-2.
+
+    // One dgtsv elimination step i, with row r = i+1 assembled from the
+    // current window.  Returns false when LAPACK would stop (zero pivot).
+    auto elim = [&](int i) -> bool {
         const int r = i + 1;
         double nb, ndu, nd, nld;
-        if (r >= 2 && r < n - 1) {
-            xa = xb; xb = xc; xc = XK(r + 1);
-            ya = yb; yb = yc; yc = YK(r + 1);
-        }
         prep_row(r, n, xa, xb, xc, ya, yb, yc, nb, ndu, nd, nld);
         double Di, DUi, DLi, Bi;
         if (fabs(cd) >= fabs(nld)) {
@@ -121,7 +121,7 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
                 a.sB[(long)i * ns + col] = cb;
                 a.sB[(long)r * ns + col] = nb;
                 stop = r + 1;
-                break;
+                return false;
             }
             Di = cd; DUi = cdu; DLi = (i < n - 2) ? 0.0 : nld; Bi = cb;
         } else {
@@ -144,7 +144,44 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
         a.sDL[(long)i * ns + col] = DLi;
         a.sB[(long)i * ns + col] = Bi;
         cd = nd; cdu = ndu; cb = nb;
+        return true;
+    };
+
+    // row 1 reads the initial window (x0, x1, x2)
+    bool go = elim(0);
+    // rows 2 .. n-2 read (x[r-1], x[r], x[r+1]): shift in x[i+2] at step i
+    int i = 1;
+    if (go && n - 3 >= SPLINE_PF) {
+        double px[SPLINE_PF], py[SPLINE_PF];
+#pragma unroll
+        for (int u = 0; u < SPLINE_PF; u++) {
+            px[u] = XK(3 + u);
+            py[u] = YK(3 + u);
+        }
+        for (; go && i + SPLINE_PF - 1 <= n - 3; i += SPLINE_PF) {
+#pragma unroll
+            for (int u = 0; u < SPLINE_PF; u++) {
+                xa = xb; xb = xc; xc = px[u];
+                ya = yb; yb = yc; yc = py[u];
+                const int id = i + u + 2 + SPLINE_PF;
+                if (id < n) {
+                    px[u] = XK(id);
+                    py[u] = YK(id);
+                }
+                if (!elim(i + u)) {
+                    go = false;
+                    break;
+                }
+            }
+        }
     }
+    for (; go && i <= n - 3; i++) {
+        xa = xb; xb = xc; xc = XK(i + 2);
+        ya = yb; yb = yc; yc = YK(i + 2);
+        go = elim(i);
+    }
+    // the last row reads (x[n-3], x[n-2], x[n-1]), the window of row n-2
+    if (go) go = elim(n - 2);
     if (info == 0 && cd == 0.0) {
         info = n;
         a.sB[(long)(n - 1) * ns + col] = cb;
@@ -152,50 +189,10 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
     }
 
     double *out = G.out;
-    if (info != 0) {
-        // LAPACK returns before back substitution; the reference then builds
-        // coefficients from the partially eliminated right-hand side.  Rows
-        // never reached keep their assembled values.
-        if (a.info) *a.info = 1;  // benign race: every writer stores 1
-        for (int r = stop; r < n; r++) {
-            double b_, u_, d_, l_;
-            int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
-            prep_row(r, n, XK(i0), XK(i0 + 1), XK(i0 + 2), YK(i0), YK(i0 + 1), YK(i0 + 2), b_, u_, d_,
-                     l_);
-            a.sB[(long)r * ns + col] = b_;
-        }
-    }
-
-    // ---------------- back substitution + coefficients ----------------
-    // slope(i) is the solved knot derivative; interval i needs slope(i),
-    // slope(i+1).  Walk i downwards, emitting interval i as soon as slope(i)
-    // is known.
-    double x_next, x_next2;  // slope(i+1), slope(i+2)
-    if (info == 0) {
-        x_next = cb / cd;                                     // slope(n-1)
-    } else {
-        x_next = a.sB[(long)(n - 1) * ns + col];
-    }
-    x_next2 = 0.0;
     double c1L = 0.0, c2L = 0.0, c3L = 0.0;
-    for (int i = n - 2; i >= 0; i--) {
-        double xi;
-        if (info == 0) {
-            const double Bi = a.sB[(long)i * ns + col];
-            const double DUi = a.sDU[(long)i * ns + col];
-            const double Di = a.sD[(long)i * ns + col];
-            if (i == n - 2) {
-                xi = (Bi - DUi * x_next) / Di;
-            } else {
-                const double DLi = a.sDL[(long)i * ns + col];
-                xi = (Bi - DUi * x_next - DLi * x_next2) / Di;
-            }
-        } else {
-            xi = a.sB[(long)i * ns + col];
-        }
-        // set_spline_constants / fill_coefficients (interpolate.cu:28-43)
-        const double t0 = XK(i), t1 = XK(i + 1);
-        const double y0 = YK(i), y1 = YK(i + 1);
+    // Coefficients of interval i from slope(i) = xi and slope(i+1) = x_next
+    // (set_spline_constants / fill_coefficients, interpolate.cu:28-43).
+    auto emit = [&](int i, double xi, double x_next, double t0, double t1, double y0, double y1) {
         const double dt = t1 - t0;
         const double slope = (y1 - y0) / dt;
         const double tt = (xi + x_next - 2 * slope) / dt;
@@ -228,8 +225,84 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
             out[3 * G.o_coef + o] = c3;
         }
         if (i == n - 2) { c1L = c1; c2L = c2; c3L = c3; }
-        x_next2 = x_next;
-        x_next = xi;
+    };
+
+    if (info != 0) {
+        // LAPACK returns before back substitution; the reference then builds
+        // coefficients from the partially eliminated right-hand side.  Rows
+        // never reached keep their assembled values.
+        if (a.info) *a.info = 1;  // benign race: every writer stores 1
+        for (int r = stop; r < n; r++) {
+            double b_, u_, d_, l_;
+            int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
+            prep_row(r, n, XK(i0), XK(i0 + 1), XK(i0 + 2), YK(i0), YK(i0 + 1), YK(i0 + 2), b_, u_, d_,
+                     l_);
+            a.sB[(long)r * ns + col] = b_;
+        }
+        double x_next = a.sB[(long)(n - 1) * ns + col];
+        for (int ii = n - 2; ii >= 0; ii--) {
+            const double xi = a.sB[(long)ii * ns + col];
+            emit(ii, xi, x_next, XK(ii), XK(ii + 1), YK(ii), YK(ii + 1));
+            x_next = xi;
+        }
+    } else {
+        // ---------------- back substitution + coefficients ----------------
+        // slope(i) is the solved knot derivative; interval i needs slope(i)
+        // and slope(i+1): walk i downwards, emitting interval i as soon as
+        // slope(i) is known.  Scratch rows and knots are prefetched
+        // SPLINE_PF steps ahead.
+        double x_next = cb / cd;  // slope(n-1)
+        double x_next2 = 0.0;     // slope(i+2)
+        double t1 = XK(n - 1), y1 = YK(n - 1);
+        auto back = [&](int ii, double Bi, double DUi, double Di, double DLi, double t0, double y0) {
+            double xi;
+            if (ii == n - 2)
+                xi = (Bi - DUi * x_next) / Di;
+            else
+                xi = (Bi - DUi * x_next - DLi * x_next2) / Di;
+            emit(ii, xi, x_next, t0, t1, y0, y1);
+            t1 = t0;
+            y1 = y0;
+            x_next2 = x_next;
+            x_next = xi;
+        };
+        int ii = n - 2;
+        if (n - 1 >= SPLINE_PF) {
+            double rB[SPLINE_PF], rU[SPLINE_PF], rD[SPLINE_PF], rL[SPLINE_PF], rX[SPLINE_PF],
+                rY[SPLINE_PF];
+#pragma unroll
+            for (int u = 0; u < SPLINE_PF; u++) {
+                const long o = (long)(n - 2 - u) * ns + col;
+                rB[u] = a.sB[o];
+                rU[u] = a.sDU[o];
+                rD[u] = a.sD[o];
+                rL[u] = a.sDL[o];
+                rX[u] = XK(n - 2 - u);
+                rY[u] = YK(n - 2 - u);
+            }
+            for (; ii - (SPLINE_PF - 1) >= 0; ii -= SPLINE_PF) {
+#pragma unroll
+                for (int u = 0; u < SPLINE_PF; u++) {
+                    const double Bi = rB[u], DUi = rU[u], Di = rD[u], DLi = rL[u];
+                    const double t0 = rX[u], y0 = rY[u];
+                    const int id = ii - u - SPLINE_PF;
+                    if (id >= 0) {
+                        const long o = (long)id * ns + col;
+                        rB[u] = a.sB[o];
+                        rU[u] = a.sDU[o];
+                        rD[u] = a.sD[o];
+                        rL[u] = a.sDL[o];
+                        rX[u] = XK(id);
+                        rY[u] = YK(id);
+                    }
+                    back(ii - u, Bi, DUi, Di, DLi, t0, y0);
+                }
+            }
+        }
+        for (; ii >= 0; ii--) {
+            const long o = (long)ii * ns + col;
+            back(ii, a.sB[o], a.sDU[o], a.sD[o], a.sDL[o], XK(ii), YK(ii));
+        }
     }
     if (!G.dense) {
         // fill_final_derivs (interpolate.cu:242-280)

@@ -14,6 +14,7 @@
 
 #include "spline.hip"
 #include "modesum_direct.hpp"
+#include "few_math.hpp"
 
 namespace few {
 static char g_twin_err[512];
@@ -143,6 +144,11 @@ int twin_direct_samples(double *out, const long long *idx, long nidx, const doub
         out[2 * q + 1] = w_im;
     }
     return 0;
+}
+
+void twin_sincos_turns(const double *f, double *s, double *c, long n)
+{
+    for (long i = 0; i < n; i++) sincos_turns(reduce_turns(f[i]), s[i], c[i]);
 }
 
 }  // extern "C"

@@ -32,7 +32,7 @@ def tw(twin):
     return twin
 
 
-@pytest.mark.parametrize("ninterps,length", [(1, 3), (3, 4), (33, 120), (7, 200)])
+@pytest.mark.parametrize("ninterps,length", [(1, 3), (3, 4), (2, 9), (2, 10), (2, 11), (2, 12), (5, 13), (33, 120), (7, 200)])
 def test_twin_spline_reference_layout_bitwise(tw, ninterps, length):
     rng = np.random.default_rng(length)
     t = np.sort(rng.uniform(0, 1e7, (ninterps, length)), axis=1)
@@ -109,3 +109,18 @@ def test_twin_direct_fp64_vs_oracle(tw):
                            _p(k), _p(n), inp.L, inp.num_pts, inp.K, _p(yl), inp.dt, _p(inp.t))
     ref = oracle_waveform(inp, idx)
     assert oracle.rel_l2(out.view(np.complex128), ref) < 1e-14
+
+
+def test_twin_sincos_turns(tw):
+    """FP64 sin/cos of 2*pi*f (few_math.hpp) against numpy, over reduced and
+    unreduced turns up to 1e6 (phases of a two-year inspiral)."""
+    tw.twin_sincos_turns.argtypes = [P, P, P, ctypes.c_long]
+    rng = np.random.default_rng(0)
+    f = np.concatenate([np.linspace(-0.5, 0.5, 20001), rng.uniform(-1e6, 1e6, 20000),
+                        np.array([0.0, 0.25, -0.25, 0.125, 0.5, -0.5, 1e-300])])
+    s = np.zeros_like(f)
+    c = np.zeros_like(f)
+    tw.twin_sincos_turns(_p(f), _p(s), _p(c), len(f))
+    r = f - np.rint(f)  # the reduction the kernels apply
+    np.testing.assert_allclose(s, np.sin(2 * np.pi * r), rtol=0, atol=4e-16)
+    np.testing.assert_allclose(c, np.cos(2 * np.pi * r), rtol=0, atol=4e-16)
