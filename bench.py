@@ -7,9 +7,11 @@ inputs resident in HBM.  One step = zero the output + spline build (2K+3
 systems) + interval planning / coefficient tiles + mode sum: the whole path
 from knot values to the complex128 waveform.
 
-The headline path is the mixed-precision factorised sum (FP64 phases, FP32
-MFMA); the plain FP64 GPU sum (the reference's arithmetic) is timed beside it
-as the denominator of the speed-up.  Prints ONE JSON line.
+The headline path is the mixed-precision factorised sum (FP64 phases and
+spline build; the n-contraction on FP16 MFMA with hi/lo split operands and
+FP32 accumulation, or FP32 MFMA when the mode set's tile does not fit LDS);
+the plain FP64 GPU sum (the reference's arithmetic) is timed beside it as the
+denominator of the speed-up.  Prints ONE JSON line.
 """
 
 from __future__ import annotations
@@ -25,7 +27,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6, "fp32_mfma": 157.3}  # MI355X TFLOP/s (dense)
+PEAK = {"fp64": 78.6, "fp32_mfma": 157.3, "fp16_mfma": 2500.0}  # MI355X TFLOP/s (dense)
 DIRECT_FLOP = 37  # algorithmic FLOP per mode-sample of the direct sum (sincos counted 0)
 
 
@@ -88,10 +90,15 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="full1")
-    ap.add_argument("--precision", default="mixed", help="headline path: mixed | fp64")
+    ap.add_argument("--precision", default="auto",
+                    help="headline path: auto (mixed16 when the mode set fits, else mixed) | "
+                         "mixed16 | mixed | fp64")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a known-traffic "
+                         "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path in this build (the sum "
@@ -116,6 +123,18 @@ def main():
     log(f"[bench] K={K} N={N} L={inp.L} mode-samples={K * N:.4g} plan rows={prob.plan.rows} "
         f"cols={prob.plan.cols} supported={prob.plan.supported}")
 
+    if args.copy_mb > 0:
+        nel = args.copy_mb * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
+        del src, dst
+
+    if args.precision == "auto":
+        args.precision = "mixed16" if prob.plan.supported16 else "mixed"
     ms_step, ms = time_path(prob, args.precision, args.steps, args.warmup)
     h = prob.wave.cpu().numpy()
     value = K * N / (ms_step * 1e-3)
@@ -138,15 +157,23 @@ def main():
         log(f"[bench] fp64: step {ms64_step:.3f} ms, speed-up {ms64_step / ms_step:.2f}x, "
             f"headline rel-L2 vs fp64 {fp64['headline_rel_l2_vs_fp64_all_samples']:.3e}")
 
-    if args.precision == "mixed" and prob.plan.supported:
+    if args.precision in ("mixed", "mixed16") and prob.plan.supported:
         flop_per_sample = 8 * 4 * prob.plan.rows * prob.plan.cols
         flops = flop_per_sample * N
         achieved = flops / (ms["sum"] * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma"],
-                    "unit": "TFLOP/s", "frac": achieved / PEAK["fp32_mfma"], "traffic": None,
-                    "kernel": "fact_f32_kernel",
-                    "algorithmic": f"8 FLOP x 4 powers x {prob.plan.rows} rows x {prob.plan.cols} "
-                                   f"cols = {flop_per_sample} FLOP/sample x {N} samples per launch"}
+        alg = (f"8 FLOP x 4 powers x {prob.plan.rows} rows x {prob.plan.cols} cols = "
+               f"{flop_per_sample} FLOP/sample x {N} samples per launch")
+        if args.precision == "mixed":
+            roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma"],
+                        "unit": "TFLOP/s", "frac": achieved / PEAK["fp32_mfma"], "traffic": None,
+                        "kernel": "fact_f32_kernel", "algorithmic": alg}
+        else:
+            peak = PEAK["fp16_mfma"] / 3.0  # three FP16 products per FP32-accurate product
+            roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                        "frac": achieved / peak, "traffic": None, "kernel": "fact16_kernel",
+                        "algorithmic": alg,
+                        "peak_basis": "FP16 dense MFMA 2500 TFLOP/s / 3 split products",
+                        "frac_of_fp32_mfma_peak": achieved / PEAK["fp32_mfma"]}
     else:
         flops = DIRECT_FLOP * K * N
         achieved = flops / (ms["sum"] * 1e-3) / 1e12
@@ -172,7 +199,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 phases + f32 MFMA" if args.precision == "mixed" else "f64",
+        "dtype": {"mixed": "f64 phases + f32 MFMA", "mixed16": "f64 phases + split-f16 MFMA (f32 acc)",
+                  "fp64": "f64"}[args.precision],
         "data": "synthetic, seeded (fastemriwaveforms_amd/synthetic.py)",
         "config": {"workload": f"{args.workload}: T={inp.T_years} yr, dt={inp.dt} s, N={N} samples, "
                                f"L={inp.L} knots, K={K} modes",

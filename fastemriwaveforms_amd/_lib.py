@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libfew_rocm.so")
 FEW_PRECISION_FP64 = 0
 FEW_PRECISION_FP64_FACT = 1
 FEW_PRECISION_MIXED = 2
+FEW_PRECISION_MIXED16 = 3
 
 _lib = None
 

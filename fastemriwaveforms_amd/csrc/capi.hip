@@ -92,8 +92,14 @@ static PlanLayout plan_layout(int L)
 static size_t fact_coef_offset(int L) { return plan_layout(L).total; }
 static size_t fact_ws_bytes(const FactPlanHost &p, int L)
 {
-    return fact_coef_offset(L) +
-           align256(sizeof(float) * fact_coef_floats_per_interval(p) * (size_t)(L > 1 ? L - 1 : 1));
+    const size_t n = (size_t)(L > 1 ? L - 1 : 1);
+    const size_t f32 = align256(sizeof(float) * fact_coef_floats_per_interval(p) * n);
+    const size_t f16 = align256(fact16_tile_bytes(p) * n);
+    return fact_coef_offset(L) + (f32 > f16 ? f32 : f16);
+}
+static bool fact16_supported(const FactPlanHost &p)
+{
+    return p.ok && fact16_tile_bytes(p) + 24 * sizeof(double) <= 160 * 1024;
 }
 
 struct DeviceGuard {
@@ -143,6 +149,43 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
     }
     Segment *segs = reinterpret_cast<Segment *>(ws + pl.segs);
     int *prefix = reinterpret_cast<int *>(ws + pl.prefix);
+    if (precision == FEW_PRECISION_MIXED16) {
+        if (!plan || !fact16_supported(plan->host)) {
+            set_error("get_waveform: mixed16 precision needs a supported mode plan");
+            return FEW_ERR_UNSUPPORTED;
+        }
+        const FactPlanHost &ph = plan->host;
+        if (ph.K != K) {
+            set_error("get_waveform: plan built for %d modes, called with %d", ph.K, K);
+            return FEW_ERR_ARG;
+        }
+        if (ws_bytes < fact_ws_bytes(ph, L)) {
+            set_error("get_waveform: workspace of %zu bytes, need %zu", ws_bytes,
+                      fact_ws_bytes(ph, L));
+            return FEW_ERR_ARG;
+        }
+        unsigned char *tiles = reinterpret_cast<unsigned char *>(ws + fact_coef_offset(L));
+        const long tb = (long)fact16_tile_bytes(ph);
+        const int tile = fact16_tile(variant);
+        const long max_blocks = ((long)out_len + tile - 1) / tile + (L - 1);
+        if (max_blocks > 0x7fffffffL) {
+            set_error("get_waveform: output too long (%d samples)", out_len);
+            return FEW_ERR_LIMIT;
+        }
+        if (stage & 1) {
+            PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
+            int rc = launch_plan(p, stream);
+            if (rc) return rc;
+            Fact16PrepArgs pa{interp, ylm, h_t, L, tiles, tb, fact16_rstrh(ph), fact16_kb(ph), ph.RB,
+                              plan->dev};
+            rc = launch_fact16_prep(pa, stream);
+            if (rc) return rc;
+        }
+        if (!(stage & 2)) return FEW_OK;
+        Fact16Args fa{wave, pcoef, segs, prefix, L - 1, dt, tiles, tb, fact16_rstrh(ph),
+                      fact16_kb(ph), plan->dev};
+        return launch_fact16_sum(fa, ph.RB, (int)max_blocks, variant, stream);
+    }
     if (precision == FEW_PRECISION_MIXED) {
         if (!plan || !plan->host.ok) {
             set_error("get_waveform: mixed precision needs a supported mode plan");
@@ -383,6 +426,7 @@ void *few_mode_plan_create(const int *h_m, const int *h_k, const int *h_n, int n
     pl->dev.rows_arith = h.rows_arith;
     pl->dev.m0 = h.m0;
     pl->dev.k0 = h.k0;
+    pl->dev.has_k = h.has_k;
     if (!h.ok) return pl;  // usable for the direct paths; mixed reports unsupported
     std::vector<int> buf;
     for (int r = 0; r < h.Nr; r++) {
@@ -423,7 +467,7 @@ int few_mode_plan_info(const void *plan, int *info7)
     info7[3] = h.n_lo;
     info7[4] = h.RB;
     info7[5] = h.KS;
-    info7[6] = h.ok ? 1 : 0;
+    info7[6] = (h.ok ? 1 : 0) | (fact16_supported(h) ? 2 : 0);
     return FEW_OK;
 }
 
@@ -440,7 +484,7 @@ size_t few_plan_workspace_bytes(const void *plan, int init_len, int out_len, int
     (void)out_len;
     const int L = init_len < 2 ? 2 : init_len;
     size_t b = plan_layout(L).total;
-    if (plan && precision == FEW_PRECISION_MIXED) {
+    if (plan && (precision == FEW_PRECISION_MIXED || precision == FEW_PRECISION_MIXED16)) {
         const FactPlanHost &h = static_cast<const FactPlan *>(plan)->host;
         if (h.ok) b = fact_ws_bytes(h, L);
     }

@@ -39,12 +39,14 @@ __host__ __device__ inline void sincos_turns(double f, double &s, double &c)
     pc = pc * x2 - 4.1666666666666666667e-02;         // -1/4!
     pc = pc * x2 + 0.5;
     const double cs = 1.0 - x2 * pc;
-    // rotate by q quarter turns
+    // rotate by q quarter turns (branch-free): odd q swaps sin and cos,
+    // q = 2, 3 negate sin, q = 1, 2 negate cos
     const int qi = ((int)q) & 3;
-    if (qi == 0) { s = sn; c = cs; }
-    else if (qi == 1) { s = cs; c = -sn; }
-    else if (qi == 2) { s = -sn; c = -cs; }
-    else { s = -cs; c = sn; }
+    const bool swap = qi & 1;
+    const double s0 = swap ? cs : sn;
+    const double c0 = swap ? sn : cs;
+    s = (qi & 2) ? -s0 : s0;
+    c = ((qi + 1) & 2) ? -c0 : c0;
 }
 
 // Angle in turns reduced to [-1/2, 1/2]: a - rint(a).

@@ -86,6 +86,9 @@ bool build_fact_plan(const int *m, const int *k, const int *n, int K, FactPlanHo
         if (p.row_m[rr] != p.row_m[0] + rr || p.row_k[rr] != p.row_k[0]) p.rows_arith = 0;
     p.m0 = p.row_m[0];
     p.k0 = p.row_k[0];
+    p.has_k = 0;
+    for (int q = 0; q < K; q++)
+        if (k[q] != 0) p.has_k = 1;
     p.ok = (p.RB <= FACT_MAX_RB) && (p.Nc <= FACT_MAX_NC);
     return p.ok;
 }

@@ -33,6 +33,7 @@ struct FactPlanHost {
     int CST = 0;            // LDS row stride in complex entries (== 2 mod 16)
     int rows_arith = 0;     // rows are m0, m0+1, ... with one k: phasors by recurrence
     int m0 = 0, k0 = 0;
+    int has_k = 0;          // some mode has k != 0 (Phi_theta enters the phase)
     std::vector<int> row_m, row_k;        // (Nr,)
     std::vector<int> cell_start;          // (Nr*Nc + 1,)
     std::vector<int> entries;             // q*2 + conj, grouped by cell, q ascending
@@ -42,7 +43,7 @@ struct FactPlanHost {
 // Device-side view of an uploaded plan (all pointers into one allocation).
 struct FactPlanDev {
     int K, Nr, Nc, n_lo, RB, KS, CST;
-    int rows_arith, m0, k0;
+    int rows_arith, m0, k0, has_k;
     const int2 *row_mk;      // (Nr,)
     const int *cell_start;   // (Nr*Nc + 1,)
     const int *entries;
@@ -91,6 +92,45 @@ struct FactPrepArgs {
 };
 
 int launch_fact_prep(const FactPrepArgs &a, hipStream_t stream);
+
+// ---- FP16 split-operand variant (modesum_fact16.hip)
+// Tile of one interval: two FP16 planes of RB*16 rows x RSTRH halfs, then
+// RB*8 floats of inverse row scales, padded to 256 bytes.
+inline int fact16_kb(const FactPlanHost &p) { return (2 * p.Nc + 31) / 32; }
+inline int fact16_rstrh(const FactPlanHost &p) { return 32 * fact16_kb(p) + 8; }
+inline size_t fact16_tile_bytes(const FactPlanHost &p)
+{
+    const size_t b = 2 * (size_t)p.RB * 16 * fact16_rstrh(p) * 2 + (size_t)p.RB * 8 * 4;
+    return (b + 255) & ~(size_t)255;
+}
+
+struct Fact16PrepArgs {
+    const double *interp;
+    const double2 *ylm;
+    const double *h_t;
+    int L;
+    unsigned char *tiles;
+    long tile_bytes;
+    int RSTRH, KB, RB;
+    FactPlanDev plan;
+};
+
+struct Fact16Args {
+    double2 *wave;
+    const double *pcoef;
+    const Segment *segs;
+    const int *prefix;
+    int nseg;
+    double dt;
+    const unsigned char *tiles;
+    long tile_bytes;
+    int RSTRH, KB;
+    FactPlanDev plan;
+};
+
+int launch_fact16_prep(const Fact16PrepArgs &a, hipStream_t stream);
+int fact16_tile(int variant);
+int launch_fact16_sum(const Fact16Args &a, int RB, int max_blocks, int variant, hipStream_t stream);
 int fact_tile(const FactPlanHost &p, int variant);
 int launch_fact_sum(const FactArgs &a, const FactPlanHost &p, int max_blocks, int variant,
                     hipStream_t stream);

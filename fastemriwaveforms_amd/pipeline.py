@@ -22,7 +22,8 @@ from . import _lib
 from ._lib import check, lib
 from .synthetic import ModeSumInput
 
-PRECISION_CODES = {"fp64": _lib.FEW_PRECISION_FP64, "mixed": _lib.FEW_PRECISION_MIXED}
+PRECISION_CODES = {"fp64": _lib.FEW_PRECISION_FP64, "mixed": _lib.FEW_PRECISION_MIXED,
+                   "mixed16": _lib.FEW_PRECISION_MIXED16}
 
 
 class ModePlan:
@@ -42,7 +43,8 @@ class ModePlan:
         check(lib().few_mode_plan_info(self.handle, info.ctypes.data), "mode_plan_info")
         self.rows, self.cols, self.n_lo = int(info[1]), int(info[2]), int(info[3])
         self.row_blocks, self.ksteps = int(info[4]), int(info[5])
-        self.supported = bool(info[6])
+        self.supported = bool(info[6] & 1)
+        self.supported16 = bool(info[6] & 2)
 
     def workspace_bytes(self, L, N, precision) -> int:
         return int(lib().few_plan_workspace_bytes(self.handle, L, N, PRECISION_CODES[precision]))

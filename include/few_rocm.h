@@ -28,6 +28,7 @@ extern "C" {
 
 #define FEW_PRECISION_FP64 0   /* plain FP64 direct sum (reference arithmetic) */
 #define FEW_PRECISION_MIXED 2  /* factorised MFMA sum: FP64 phases, FP32 MFMA, FP64 output */
+#define FEW_PRECISION_MIXED16 3 /* factorised sum on FP16 MFMA with split (hi+lo) operands */
 
 const char *few_last_error(void);
 int few_abi_version(void);
@@ -121,8 +122,9 @@ int few_prepare_modes_async(const double *teuk, const double *phase_knots, const
  * on the host from HOST copies of the int32 mode indices and uploaded to the
  * current device once; reused by every call with the same mode set.
  * Returns NULL on error.  info7 = {K, rows, cols, n_lo, row blocks,
- * k-steps, supported}; unsupported sets (too many rows/columns for the MFMA
- * tiles) can still use FEW_PRECISION_FP64.
+ * k-steps, supported}; supported is a bit mask (1: FEW_PRECISION_MIXED,
+ * 2: FEW_PRECISION_MIXED16); unsupported sets (too many rows/columns for the
+ * MFMA tiles) can still use FEW_PRECISION_FP64.
  */
 void *few_mode_plan_create(const int *h_m, const int *h_k, const int *h_n, int num_modes);
 int few_mode_plan_info(const void *plan, int *info7);

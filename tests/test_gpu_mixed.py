@@ -22,30 +22,35 @@ def _run(inp, precision, variant=0):
     return p.run(precision).cpu().numpy(), p
 
 
-def test_mixed_small_full_oracle(gpu):
+PRECS = ["mixed", "mixed16"]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_mixed_small_full_oracle(gpu, prec):
     from fastemriwaveforms_amd import synthetic
     inp = synthetic.small_input(seed=31, L=16, K=50, T_years=0.004)
-    h, p = _run(inp, "mixed")
+    h, p = _run(inp, prec)
     assert p.plan.supported
     ref = oracle_waveform(inp)
     err = oracle.rel_l2(h, ref)
-    print(f"small mixed rel-L2 {err:.3e}")
+    print(f"small {prec} rel-L2 {err:.3e}")
     assert err < MIXED_TOL
 
 
-@pytest.mark.parametrize("variant", [0, 4])
-def test_mixed_cpu0_vs_oracle(gpu, variant):
+@pytest.mark.parametrize("prec,variant", [("mixed", 0), ("mixed", 4), ("mixed16", 0), ("mixed16", 4)])
+def test_mixed_cpu0_vs_oracle(gpu, prec, variant):
     from fastemriwaveforms_amd import synthetic
     inp = synthetic.make_workload("cpu0")
-    h, _ = _run(inp, "mixed", variant)
+    h, _ = _run(inp, prec, variant)
     ref = oracle_waveform(inp)
     err = oracle.rel_l2(h, ref)
     mm = oracle.mismatch(h, ref)
-    print(f"cpu0 mixed variant {variant}: rel-L2 {err:.3e} mismatch {mm:.3e}")
+    print(f"cpu0 {prec} variant {variant}: rel-L2 {err:.3e} mismatch {mm:.3e}")
     assert err < MIXED_TOL
 
 
-def test_mixed_full1_vs_fp64(gpu):
+@pytest.mark.parametrize("prec", PRECS)
+def test_mixed_full1_vs_fp64(gpu, prec):
     """Workload full1 (3843 modes, 3.16e6 samples): every sample against the FP64
     GPU path (itself within 1e-12 of the oracle), plus spread oracle samples."""
     from fastemriwaveforms_amd import synthetic
@@ -53,20 +58,25 @@ def test_mixed_full1_vs_fp64(gpu):
     inp = synthetic.make_workload("full1")
     p = ModeSumProblem(inp)
     h64 = p.run("fp64").cpu().numpy()
-    hm = p.run("mixed").cpu().numpy()
+    hm = p.run(prec).cpu().numpy()
     err = oracle.rel_l2(hm, h64)
     idx = spread_indices(inp.num_pts, count=1500)
     ref = oracle_waveform(inp, idx)
     err_o = oracle.rel_l2(hm[idx], ref)
-    print(f"full1 mixed: rel-L2 vs fp64 GPU {err:.3e}, vs oracle samples {err_o:.3e}")
+    print(f"full1 {prec}: rel-L2 vs fp64 GPU {err:.3e}, vs oracle samples {err_o:.3e}")
     assert err < MIXED_TOL and err_o < MIXED_TOL
 
 
 def test_mixed_explicit_pm_modes(gpu):
-    """Workload mixed2 shape (|m| <= l, n in [-55, 55], Y- = 0) on a short duration."""
+    """Workload mixed2 shape (|m| <= l, n in [-55, 55], Y- = 0) on a short duration.
+    Its FP16 tile (21 rows x 111 columns) exceeds LDS: mixed16 reports unsupported."""
     from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.exceptions import FewRocmError
     inp = synthetic.make_input(0.05, 5.0, 60, "m_all_55", 2000, 1002, minus_m_ylm=False)
     h, p = _run(inp, "mixed")
+    assert not p.plan.supported16
+    with pytest.raises(FewRocmError):
+        p.mode_sum("mixed16")
     assert p.plan.cols == 111
     idx = spread_indices(inp.num_pts, count=1500)
     ref = oracle_waveform(inp, idx)
@@ -75,7 +85,8 @@ def test_mixed_explicit_pm_modes(gpu):
     assert err < MIXED_TOL
 
 
-def test_mixed_accumulates_and_edges(gpu):
+@pytest.mark.parametrize("prec", PRECS)
+def test_mixed_accumulates_and_edges(gpu, prec):
     """Accumulate-into semantics and interval edges (knots on the sample grid,
     a knot past the end) for the factorised path."""
     import torch
@@ -88,7 +99,7 @@ def test_mixed_accumulates_and_edges(gpu):
     init = (torch.arange(inp.num_pts, dtype=torch.float64, device=p.wave.device) * 1e-3).to(torch.complex128)
     p.wave.copy_(init)
     p.build_splines()
-    p.mode_sum("mixed")
+    p.mode_sum(prec)
     h = (p.wave - init).cpu().numpy()
     ref = oracle_waveform(inp)
     assert oracle.rel_l2(h, ref) < MIXED_TOL
