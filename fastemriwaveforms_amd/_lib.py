@@ -45,6 +45,13 @@ def _declare(lib: ctypes.CDLL) -> None:
         "few_plan_workspace_bytes": ([_d, c_int, c_int, c_int], c_size),
         "few_get_waveform_plan_async": ([_d, _d, _d, _d, _d, _d, _d, _d, c_int, c_int, c_int, _d,
                                          c_double, _d, c_int, c_int, c_int, _d, c_size, _d], c_int),
+        "few_batch_create": (
+                             [c_int, _d, _d, _d, _d, _d, _d, _d], ctypes.c_void_p),
+        "few_batch_destroy": ([_d], None),
+        "few_batch_layout": ([_d, _d, _d], c_int),
+        "few_batch_workspace_bytes": ([_d, c_int], c_size),
+        "few_batch_run_async": ([_d, _d, _d, _d, _d, _d, _d, _d, c_int, c_int, c_int, _d, c_size,
+                                 _d], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -81,5 +88,6 @@ def exported_symbols() -> list[str]:
         "few_get_waveform_async", "few_waveform_workspace_bytes", "few_spline_build_async",
         "few_prepare_modes_async", "few_release_workspace", "few_mode_plan_create",
         "few_mode_plan_info", "few_mode_plan_destroy", "few_plan_workspace_bytes",
-        "few_get_waveform_plan_async",
+        "few_get_waveform_plan_async", "few_batch_create", "few_batch_destroy", "few_batch_layout",
+        "few_batch_workspace_bytes", "few_batch_run_async",
     ]

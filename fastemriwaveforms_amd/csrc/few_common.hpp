@@ -38,6 +38,15 @@ struct SplineGroup {
                          //    out[c*(length-1)*nsys + j*nsys + s] instead
 };
 
+// Scratch of the sequential sweeps: four [knot][column] double arrays.
+struct SplineScratch {
+    double *sD, *sDU, *sDL, *sB;
+    int *info;
+};
+
+__host__ __device__ void spline_system_core(const SplineGroup &G, int s, int n, long col, long ns,
+                                            const SplineScratch &a);
+
 // Launches the fused not-a-knot build for up to two groups of systems of a
 // common length.  scratch holds 4*length*(nsys0+nsys1) doubles.
 int launch_spline_build(const SplineGroup *groups, int ngroups, int length, double *scratch4[4],

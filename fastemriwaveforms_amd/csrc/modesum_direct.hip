@@ -20,12 +20,14 @@
 // Mode partial sums are flushed into the output every 5000 modes, matching
 // the reference CPU chunking (interpolate.cu:22, 554-600, 682-685).
 
+#include "batch.hpp"
 #include "few_common.hpp"
 #include "modesum_direct.hpp"
 
 namespace few {
 
-__global__ __launch_bounds__(256) void plan_segments_kernel(PlanArgs a)
+// One workgroup plans one waveform (256 threads).
+__device__ void plan_segments_block(const PlanArgs &a)
 {
     __shared__ int s_used;
     __shared__ int s_scan[256];
@@ -84,6 +86,8 @@ __global__ __launch_bounds__(256) void plan_segments_kernel(PlanArgs a)
     if (tid == 0) a.prefix[nseg] = s_carry;
 }
 
+__global__ __launch_bounds__(256) void plan_segments_kernel(PlanArgs a) { plan_segments_block(a); }
+
 // Block -> (interval, tile) by binary search over the tile prefix.
 __device__ __forceinline__ int find_segment(const int *prefix, int nseg, int b)
 {
@@ -98,17 +102,13 @@ __device__ __forceinline__ int find_segment(const int *prefix, int nseg, int b)
 #define MCH 256          // modes staged per LDS chunk
 #define CPU_CHUNK 5000   // interpolate.cu:22
 
+// One 256*SPT-sample tile (tile q of interval j) of the FP64 direct sum.
 template <int SPT>
-__global__ __launch_bounds__(256) void direct_fp64_kernel(DirectArgs a)
+__device__ void direct_fp64_tile(const DirectArgs &a, int j, int q)
 {
     __shared__ double sh[15][MCH];
     const int tid = threadIdx.x;
-    const int b = blockIdx.x;
-    const int total = a.prefix[a.nseg];
-    if (b >= total) return;
-    const int j = find_segment(a.prefix, a.nseg, b);
     const Segment sg = a.segs[j];
-    const int q = b - a.prefix[j];
 
     const long np = (long)a.nseg * 3;
     double rp[8], rt[8], rr[8];
@@ -200,6 +200,108 @@ __global__ __launch_bounds__(256) void direct_fp64_kernel(DirectArgs a)
             a.wave[idx[u]] = w;
         }
     }
+}
+
+template <int SPT>
+__global__ __launch_bounds__(256) void direct_fp64_kernel(DirectArgs a)
+{
+    const int b = blockIdx.x;
+    const int total = a.prefix[a.nseg];
+    if (b >= total) return;
+    const int j = find_segment(a.prefix, a.nseg, b);
+    direct_fp64_tile<SPT>(a, j, b - a.prefix[j]);
+}
+
+// ------------------------------------------------------------------ batch
+__global__ __launch_bounds__(256) void batch_plan_kernel(BatchDev B, BatchRunDev r, BatchPtrs p,
+                                                         int tile)
+{
+    const WaveDesc d = B.desc[blockIdx.x];
+    const double *t = p.t + d.t_off;
+    PlanArgs a{t, t, d.L, d.dt, d.N, tile, r.segs + d.seg_off, r.tpref + d.tpref_off};
+    plan_segments_block(a);
+}
+
+// tiles before each waveform (one workgroup)
+__global__ __launch_bounds__(256) void batch_tile_scan_kernel(BatchDev B, BatchRunDev r)
+{
+    __shared__ int s_scan[256];
+    __shared__ int s_carry;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    for (int base = 0; base < B.B; base += 256) {
+        const int b = base + tid;
+        int nt = 0;
+        if (b < B.B) {
+            const WaveDesc d = B.desc[b];
+            nt = r.tpref[d.tpref_off + d.L - 1];
+        }
+        s_scan[tid] = nt;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int v = (tid >= off) ? s_scan[tid - off] : 0;
+            __syncthreads();
+            s_scan[tid] += v;
+            __syncthreads();
+        }
+        const int carry = s_carry;
+        if (b < B.B) r.wt_prefix[b] = carry + s_scan[tid] - nt;
+        __syncthreads();
+        if (tid == 255) s_carry = carry + s_scan[255];
+        __syncthreads();
+    }
+    if (tid == 0) r.wt_prefix[B.B] = s_carry;
+}
+
+int launch_batch_plan(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p, int tile,
+                      hipStream_t stream)
+{
+    if (B.B <= 0) return FEW_OK;
+    hipLaunchKernelGGL(batch_plan_kernel, dim3(B.B), dim3(256), 0, stream, B, r, p, tile);
+    FEW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(batch_tile_scan_kernel, dim3(1), dim3(256), 0, stream, B, r);
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
+// FP64 control run over the batch: block -> (waveform, interval, tile).
+template <int SPT>
+__global__ __launch_bounds__(256) void batch_direct_fp64_kernel(BatchDev B, BatchRunDev r,
+                                                                BatchPtrs p)
+{
+    const int g = blockIdx.x;
+    if (g >= r.wt_prefix[B.B]) return;
+    const int b = upper_index(r.wt_prefix, B.B, g);
+    const WaveDesc d = B.desc[b];
+    const int *pref = r.tpref + d.tpref_off;
+    const int lt = g - r.wt_prefix[b];
+    const int j = upper_index(pref, d.L - 1, lt);
+    DirectArgs a;
+    a.wave = p.wave + d.wave_off / 2;
+    a.interp = p.interp + d.interp_off;
+    a.pcoef = p.pcoef + d.pc_off;
+    a.m = B.m + d.mode_off;
+    a.k = B.k + d.mode_off;
+    a.n = B.n + d.mode_off;
+    a.ylm = reinterpret_cast<const double2 *>(p.ylm + d.ylm_off);
+    a.segs = r.segs + d.seg_off;
+    a.prefix = pref;
+    a.nseg = d.L - 1;
+    a.L = d.L;
+    a.K = d.K;
+    a.dt = d.dt;
+    direct_fp64_tile<SPT>(a, j, lt - pref[j]);
+}
+
+int launch_batch_direct_fp64(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
+                             int max_blocks, hipStream_t stream)
+{
+    if (max_blocks <= 0) return FEW_OK;
+    hipLaunchKernelGGL(batch_direct_fp64_kernel<DIRECT_SPT>, dim3(max_blocks), dim3(256), 0, stream,
+                       B, r, p);
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
 }
 
 int launch_plan(const PlanArgs &p, hipStream_t stream)

@@ -19,6 +19,7 @@
 
 #include <math.h>
 
+#include "batch.hpp"
 #include "few_math.hpp"
 #include "modesum_direct.hpp"
 #include "modesum_fact.hpp"
@@ -34,19 +35,18 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 // One block per (interval j, row group r): gathers C_p[r, a] for the 4
 // powers p (FP64, fixed order per cell), scales each complex row R = 4r + p
 // by a power of two and writes the split FP16 planes.
-__global__ __launch_bounds__(256) void fact16_prep_kernel(Fact16PrepArgs a)
+__device__ void fact16_prep_block(const FactPlanDev &P, const double *interp, const double2 *ylm,
+                                  const double *h_t, int L, int j, int r, unsigned char *tile,
+                                  int RSTRH, int KB, int RB)
 {
     __shared__ double s_max[4][256];
-    const FactPlanDev P = a.plan;
-    const int j = blockIdx.x;
-    const int r = blockIdx.y;
     const int tid = threadIdx.x;
-    const int KP = 32 * a.KB;
+    const int KP = 32 * KB;
     const int ncolc = KP / 2;  // complex columns incl. padding
     double cre[2][4], cim[2][4];
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
     const long K2 = 2L * P.K;
-    const double hw = a.h_t[j + 1] - a.h_t[j];
+    const double hw = h_t[j + 1] - h_t[j];
 #pragma unroll
     for (int u = 0; u < 2; u++) {
         const int col = tid + 256 * u;
@@ -58,12 +58,12 @@ __global__ __launch_bounds__(256) void fact16_prep_kernel(Fact16PrepArgs a)
                 const int ent = P.entries[e];
                 const int qm = ent >> 1;
                 const bool cj = ent & 1;
-                const double2 Y = a.ylm[cj ? P.K + qm : qm];
+                const double2 Y = ylm[cj ? P.K + qm : qm];
 #pragma unroll
                 for (int p = 0; p < 4; p++) {
-                    const long base = ((long)p * a.L + j) * K2;
-                    const double cr = a.interp[base + qm];
-                    const double ci = cj ? -a.interp[base + P.K + qm] : a.interp[base + P.K + qm];
+                    const long base = ((long)p * L + j) * K2;
+                    const double cr = interp[base + qm];
+                    const double ci = cj ? -interp[base + P.K + qm] : interp[base + P.K + qm];
                     cre[u][p] += cr * Y.x - ci * Y.y;
                     cim[u][p] += cr * Y.y + ci * Y.x;
                 }
@@ -88,10 +88,9 @@ __global__ __launch_bounds__(256) void fact16_prep_kernel(Fact16PrepArgs a)
         }
         __syncthreads();
     }
-    unsigned char *tile = a.tiles + (long)j * a.tile_bytes;
     _Float16 *hi = reinterpret_cast<_Float16 *>(tile);
-    _Float16 *lo = hi + (long)a.RB * 16 * a.RSTRH;
-    float *inv_sigma = reinterpret_cast<float *>(lo + (long)a.RB * 16 * a.RSTRH);
+    _Float16 *lo = hi + (long)RB * 16 * RSTRH;
+    float *inv_sigma = reinterpret_cast<float *>(lo + (long)RB * 16 * RSTRH);
 #pragma unroll
     for (int p = 0; p < 4; p++) {
         const double m = s_max[p][0];
@@ -113,13 +112,20 @@ __global__ __launch_bounds__(256) void fact16_prep_kernel(Fact16PrepArgs a)
                 const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
                 const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
                 // row 2R:   (Re, -Im);  row 2R+1: (Im, Re)
-                const long o0 = (long)(2 * R) * a.RSTRH + 2 * col;
-                const long o1 = (long)(2 * R + 1) * a.RSTRH + 2 * col;
+                const long o0 = (long)(2 * R) * RSTRH + 2 * col;
+                const long o1 = (long)(2 * R + 1) * RSTRH + 2 * col;
                 hi[o0] = rh; hi[o0 + 1] = -ih; lo[o0] = rl; lo[o0 + 1] = -il;
                 hi[o1] = ih; hi[o1 + 1] = rh;  lo[o1] = il; lo[o1 + 1] = rl;
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void fact16_prep_kernel(Fact16PrepArgs a)
+{
+    const int j = blockIdx.x;
+    fact16_prep_block(a.plan, a.interp, a.ylm, a.h_t, a.L, j, blockIdx.y,
+                      a.tiles + (long)j * a.tile_bytes, a.RSTRH, a.KB, a.RB);
 }
 
 int launch_fact16_prep(const Fact16PrepArgs &a, hipStream_t stream)
@@ -158,43 +164,190 @@ constexpr int F16_NB = 2;        // 16-sample column blocks per wave
 
 // WAVES waves per workgroup share one LDS tile; 4 waves run one wave per SIMD
 // with the full register file, 8 waves two per SIMD.
+constexpr int f16_tile_samples(int waves) { return 16 * F16_NB * waves; }
+
+// One tile (tile q of the interval whose coefficient tile and phase
+// coefficients sit in LDS) of the FP16 split-operand factorised sum.
 template <int RB, int WAVES, bool ARITH>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
-void fact16_kernel(Fact16Args a)
+__device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const unsigned char *sTile,
+                                                 const double *sPh, int RSTRH, int KB,
+                                                 const Segment &sg, int q, double dt,
+                                                 double2 *wave)
 {
-    constexpr int F16_TILE = 16 * F16_NB * WAVES;
-    constexpr int NT = 64 * WAVES;
-    extern __shared__ __attribute__((aligned(16))) unsigned char sTile[];
+    constexpr int F16_TILE = f16_tile_samples(WAVES);
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wv = tid >> 6;
     const int li = lane & 15;
     const int g = lane >> 4;
-    const FactPlanDev P = a.plan;
-    const int RSTRH = a.RSTRH;
     const _Float16 *sHi = reinterpret_cast<const _Float16 *>(sTile);
     const _Float16 *sLo = sHi + (long)RB * 16 * RSTRH;
     const float *sInv = reinterpret_cast<const float *>(sLo + (long)RB * 16 * RSTRH);
-    double *sPh = reinterpret_cast<double *>(sTile + a.tile_bytes);  // 24 phase coefficients
+    const double inv_pw = 1.0 / sg.pw;
+    const double inv_hw = 1.0 / sg.hw;
 
+    // ---- per-sample setup: Phi_r (FP64) and the v_n recurrence state
+    const int base_i = sg.start + q * F16_TILE + wv * (16 * F16_NB) + li;
+    double vr[F16_NB][4], vi[F16_NB][4], zr16[F16_NB], zi16[F16_NB];
+#pragma unroll
+    for (int c = 0; c < F16_NB; c++) {
+        const double t = dt * (double)(base_i + c * 16);
+        const double s = (t - sg.pt) * inv_pw;
+        const double ur = dense_phase(sPh + 16, s, 1.0 - s) * INV_2PI16;
+        // lane g needs v_n for n = n_lo + 16 kb + 4 g + t, t = 0..3 (scaled by 256)
+        double sn, cs, zsn, zcs;
+        sincos_turns(reduce_turns((double)(P.n_lo + 4 * g) * ur), sn, cs);
+        sincos_turns(reduce_turns(ur), zsn, zcs);
+        const double zr = zcs, zi = -zsn;  // z = exp(-i Phi_r)
+        vr[c][0] = 256.0 * cs;
+        vi[c][0] = -256.0 * sn;
+#pragma unroll
+        for (int t2 = 1; t2 < 4; t2++) cmul(vr[c][t2 - 1], vi[c][t2 - 1], zr, zi, vr[c][t2], vi[c][t2]);
+        double wr = zr, wi = zi, tr, ti;
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) {  // z^16 by squaring
+            cmul(wr, wi, wr, wi, tr, ti);
+            wr = tr;
+            wi = ti;
+        }
+        zr16[c] = wr;
+        zi16[c] = wi;
+    }
+
+    v4f acc[RB][F16_NB];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+        for (int c = 0; c < F16_NB; c++) acc[rb][c] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    const _Float16 *aH = sHi + (long)li * RSTRH + 8 * g;
+    const _Float16 *aL = sLo + (long)li * RSTRH + 8 * g;
+    for (int kb = 0; kb < KB; kb++) {
+        v8h bh[F16_NB], bl[F16_NB];
+#pragma unroll
+        for (int c = 0; c < F16_NB; c++) {
+            split_pack(vr[c], vi[c], bh[c], bl[c]);
+#pragma unroll
+            for (int t2 = 0; t2 < 4; t2++) {
+                double nr, ni;
+                cmul(vr[c][t2], vi[c][t2], zr16[c], zi16[c], nr, ni);
+                vr[c][t2] = nr;
+                vi[c][t2] = ni;
+            }
+        }
+        // A fragments of the next row block are loaded while the current
+        // one feeds its MFMAs; the scheduling barrier keeps the loads
+        // from being hoisted further (register pressure).
+        v8h ahn = *reinterpret_cast<const v8h *>(aH + 32 * kb);
+        v8h aln = *reinterpret_cast<const v8h *>(aL + 32 * kb);
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const v8h ah = ahn, al = aln;
+            if (rb + 1 < RB) {
+                ahn = *reinterpret_cast<const v8h *>(aH + (long)(rb + 1) * 16 * RSTRH + 32 * kb);
+                aln = *reinterpret_cast<const v8h *>(aL + (long)(rb + 1) * 16 * RSTRH + 32 * kb);
+            }
+#pragma unroll
+            for (int c = 0; c < F16_NB; c++)
+                acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[c], acc[rb][c], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < F16_NB; c++)
+                acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[c], acc[rb][c], 0, 0, 0);
+#pragma unroll
+            for (int c = 0; c < F16_NB; c++)
+                acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[c], acc[rb][c], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+    // ---- epilogue: h = sum_{r,p} s^p u_r W / (256 sigma_R)
+    const int f = g & 1;
+    const int p0 = 2 * f;
+#pragma unroll
+    for (int c = 0; c < F16_NB; c++) {
+        // sample-level quantities recomputed here rather than held across the K loop
+        const int idx = base_i + c * 16;
+        const double t = dt * (double)idx;
+        const double s = (t - sg.pt) * inv_pw;
+        const double uph = dense_phase(sPh, s, 1.0 - s) * INV_2PI16;
+        const double uth = P.has_k ? dense_phase(sPh + 8, s, 1.0 - s) * INV_2PI16 : 0.0;
+        const float s1f = (float)((t - sg.h_t) * inv_hw);
+        const float s2f = s1f * s1f;
+        const float spa = (p0 == 0 ? 1.f : s2f) * (1.0f / 256.0f);
+        const float spb = (p0 == 0 ? s1f : s2f * s1f) * (1.0f / 256.0f);
+        double hre = 0.0, him = 0.0;
+        double u_re = 0.0, u_im = 0.0, e_re = 0.0, e_im = 0.0;
+        if constexpr (ARITH) {
+            double sn, cs;
+            const double m_first = (double)(P.m0 + (g >> 1));
+            sincos_turns(reduce_turns(m_first * uph + (double)P.k0 * uth), sn, cs);
+            u_re = cs;
+            u_im = -sn;
+            sincos_turns(reduce_turns(2.0 * uph), sn, cs);
+            e_re = cs;
+            e_im = -sn;
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const int r = 2 * rb + (g >> 1);
+            float ur, ui;
+            if constexpr (ARITH) {
+                ur = (float)u_re;
+                ui = (float)u_im;
+                double nr, ni;
+                cmul(u_re, u_im, e_re, e_im, nr, ni);
+                u_re = nr;
+                u_im = ni;
+            } else {
+                const int2 mk = r < P.Nr ? P.row_mk[r] : make_int2(0, 0);
+                double sn, cs;
+                sincos_turns(reduce_turns((double)mk.x * uph + (double)mk.y * uth), sn, cs);
+                ur = (float)cs;
+                ui = (float)(-sn);
+            }
+            if (r < P.Nr) {
+                const int R0 = 8 * rb + 2 * g;
+                const float ka = spa * sInv[R0];
+                const float kb2 = spb * sInv[R0 + 1];
+                const v4f W = acc[rb][c];
+                const float zr = ka * W[0] + kb2 * W[2];
+                const float zi = ka * W[1] + kb2 * W[3];
+                hre += (double)(ur * zr - ui * zi);
+                him += (double)(ur * zi + ui * zr);
+            }
+        }
+        hre += __shfl_xor(hre, 16);
+        him += __shfl_xor(him, 16);
+        hre += __shfl_xor(hre, 32);
+        him += __shfl_xor(him, 32);
+        if (g == 0 && idx < sg.end) {
+            double2 w = wave[idx];
+            w.x += hre;
+            w.y += him;
+            wave[idx] = w;
+        }
+    }
+}
+
+// Single waveform, persistent: block b owns a contiguous range of the
+// (interval, tile) sequence; the LDS tile is reloaded on interval change.
+template <int RB, int WAVES, bool ARITH>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
+void fact16_kernel(Fact16Args a)
+{
+    constexpr int NT = 64 * WAVES;
+    extern __shared__ __attribute__((aligned(16))) unsigned char sTile[];
+    const int tid = threadIdx.x;
+    double *sPh = reinterpret_cast<double *>(sTile + a.tile_bytes);  // 24 phase coefficients
     const int total = a.prefix[a.nseg];
     const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
     const int t0 = (int)blockIdx.x * per;
     const int t1 = min(total, t0 + per);
     if (t0 >= t1) return;
-    int j = -1, jnext_start = 0;
-    {
-        int lo = 0, hi = a.nseg;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (a.prefix[mid] <= t0) lo = mid; else hi = mid;
-        }
-        j = lo;
-        jnext_start = a.prefix[j + 1];
-    }
+    int j = upper_index(a.prefix, a.nseg, t0);
+    int jnext_start = a.prefix[j + 1];
     const long np = (long)a.nseg * 3;
     bool stale = true;
-    const int KB = a.KB;
     for (int tile = t0; tile < t1; tile++) {
         while (tile >= jnext_start) {
             j++;
@@ -211,153 +364,74 @@ void fact16_kernel(Fact16Args a)
             __syncthreads();
             stale = false;
         }
-        const Segment sg = a.segs[j];
-        const int q = tile - a.prefix[j];
-        const double inv_pw = 1.0 / sg.pw;
-        const double inv_hw = 1.0 / sg.hw;
-
-        // ---- per-sample setup: Phi_r (FP64) and the v_n recurrence state
-        const int base_i = sg.start + q * F16_TILE + wv * (16 * F16_NB) + li;
-        double vr[F16_NB][4], vi[F16_NB][4], zr16[F16_NB], zi16[F16_NB];
-#pragma unroll
-        for (int c = 0; c < F16_NB; c++) {
-            const double t = a.dt * (double)(base_i + c * 16);
-            const double s = (t - sg.pt) * inv_pw;
-            const double ur = dense_phase(sPh + 16, s, 1.0 - s) * INV_2PI16;
-            // lane g needs v_n for n = n_lo + 16 kb + 4 g + t, t = 0..3 (scaled by 256)
-            double sn, cs, zsn, zcs;
-            sincos_turns(reduce_turns((double)(P.n_lo + 4 * g) * ur), sn, cs);
-            sincos_turns(reduce_turns(ur), zsn, zcs);
-            const double zr = zcs, zi = -zsn;  // z = exp(-i Phi_r)
-            vr[c][0] = 256.0 * cs;
-            vi[c][0] = -256.0 * sn;
-#pragma unroll
-            for (int t2 = 1; t2 < 4; t2++) cmul(vr[c][t2 - 1], vi[c][t2 - 1], zr, zi, vr[c][t2], vi[c][t2]);
-            double wr = zr, wi = zi, tr, ti;
-#pragma unroll
-            for (int k2 = 0; k2 < 4; k2++) {  // z^16 by squaring
-                cmul(wr, wi, wr, wi, tr, ti);
-                wr = tr;
-                wi = ti;
-            }
-            zr16[c] = wr;
-            zi16[c] = wi;
-        }
-
-        v4f acc[RB][F16_NB];
-#pragma unroll
-        for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-            for (int c = 0; c < F16_NB; c++) acc[rb][c] = v4f{0.f, 0.f, 0.f, 0.f};
-
-        const _Float16 *aH = sHi + (long)li * RSTRH + 8 * g;
-        const _Float16 *aL = sLo + (long)li * RSTRH + 8 * g;
-        for (int kb = 0; kb < KB; kb++) {
-            v8h bh[F16_NB], bl[F16_NB];
-#pragma unroll
-            for (int c = 0; c < F16_NB; c++) {
-                split_pack(vr[c], vi[c], bh[c], bl[c]);
-#pragma unroll
-                for (int t2 = 0; t2 < 4; t2++) {
-                    double nr, ni;
-                    cmul(vr[c][t2], vi[c][t2], zr16[c], zi16[c], nr, ni);
-                    vr[c][t2] = nr;
-                    vi[c][t2] = ni;
-                }
-            }
-            // A fragments of the next row block are loaded while the current
-            // one feeds its MFMAs; the scheduling barrier keeps the loads
-            // from being hoisted further (register pressure).
-            v8h ahn = *reinterpret_cast<const v8h *>(aH + 32 * kb);
-            v8h aln = *reinterpret_cast<const v8h *>(aL + 32 * kb);
-#pragma unroll
-            for (int rb = 0; rb < RB; rb++) {
-                const v8h ah = ahn, al = aln;
-                if (rb + 1 < RB) {
-                    ahn = *reinterpret_cast<const v8h *>(aH + (long)(rb + 1) * 16 * RSTRH + 32 * kb);
-                    aln = *reinterpret_cast<const v8h *>(aL + (long)(rb + 1) * 16 * RSTRH + 32 * kb);
-                }
-#pragma unroll
-                for (int c = 0; c < F16_NB; c++)
-                    acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[c], acc[rb][c], 0, 0, 0);
-#pragma unroll
-                for (int c = 0; c < F16_NB; c++)
-                    acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[c], acc[rb][c], 0, 0, 0);
-#pragma unroll
-                for (int c = 0; c < F16_NB; c++)
-                    acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[c], acc[rb][c], 0, 0, 0);
-                __builtin_amdgcn_sched_barrier(0);
-            }
-        }
-
-        // ---- epilogue: h = sum_{r,p} s^p u_r W / (256 sigma_R)
-        const int f = g & 1;
-        const int p0 = 2 * f;
-#pragma unroll
-        for (int c = 0; c < F16_NB; c++) {
-            // sample-level quantities recomputed here rather than held across the K loop
-            const int idx = base_i + c * 16;
-            const double t = a.dt * (double)idx;
-            const double s = (t - sg.pt) * inv_pw;
-            const double uph = dense_phase(sPh, s, 1.0 - s) * INV_2PI16;
-            const double uth = P.has_k ? dense_phase(sPh + 8, s, 1.0 - s) * INV_2PI16 : 0.0;
-            const float s1f = (float)((t - sg.h_t) * inv_hw);
-            const float s2f = s1f * s1f;
-            const float spa = (p0 == 0 ? 1.f : s2f) * (1.0f / 256.0f);
-            const float spb = (p0 == 0 ? s1f : s2f * s1f) * (1.0f / 256.0f);
-            double hre = 0.0, him = 0.0;
-            double u_re = 0.0, u_im = 0.0, e_re = 0.0, e_im = 0.0;
-            if constexpr (ARITH) {
-                double sn, cs;
-                const double m_first = (double)(P.m0 + (g >> 1));
-                sincos_turns(reduce_turns(m_first * uph + (double)P.k0 * uth), sn, cs);
-                u_re = cs;
-                u_im = -sn;
-                sincos_turns(reduce_turns(2.0 * uph), sn, cs);
-                e_re = cs;
-                e_im = -sn;
-            }
-#pragma unroll
-            for (int rb = 0; rb < RB; rb++) {
-                const int r = 2 * rb + (g >> 1);
-                float ur, ui;
-                if constexpr (ARITH) {
-                    ur = (float)u_re;
-                    ui = (float)u_im;
-                    double nr, ni;
-                    cmul(u_re, u_im, e_re, e_im, nr, ni);
-                    u_re = nr;
-                    u_im = ni;
-                } else {
-                    const int2 mk = r < P.Nr ? P.row_mk[r] : make_int2(0, 0);
-                    double sn, cs;
-                    sincos_turns(reduce_turns((double)mk.x * uph + (double)mk.y * uth), sn, cs);
-                    ur = (float)cs;
-                    ui = (float)(-sn);
-                }
-                if (r < P.Nr) {
-                    const int R0 = 8 * rb + 2 * g;
-                    const float ka = spa * sInv[R0];
-                    const float kb2 = spb * sInv[R0 + 1];
-                    const v4f W = acc[rb][c];
-                    const float zr = ka * W[0] + kb2 * W[2];
-                    const float zi = ka * W[1] + kb2 * W[3];
-                    hre += (double)(ur * zr - ui * zi);
-                    him += (double)(ur * zi + ui * zr);
-                }
-            }
-            hre += __shfl_xor(hre, 16);
-            him += __shfl_xor(him, 16);
-            hre += __shfl_xor(hre, 32);
-            him += __shfl_xor(him, 32);
-            if (g == 0 && idx < sg.end) {
-                double2 w = a.wave[idx];
-                w.x += hre;
-                w.y += him;
-                a.wave[idx] = w;
-            }
-        }
+        fact16_tile_body<RB, WAVES, ARITH>(a.plan, sTile, sPh, a.RSTRH, a.KB, a.segs[j],
+                                           tile - a.prefix[j], a.dt, a.wave);
     }
+}
+
+// Batch, persistent over the (waveform, interval, tile) sequence.
+template <int RB, int WAVES, bool ARITH>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
+void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned char *tiles,
+                         long tile_bytes, int RSTRH, int KB)
+{
+    constexpr int NT = 64 * WAVES;
+    extern __shared__ __attribute__((aligned(16))) unsigned char sTile[];
+    const int tid = threadIdx.x;
+    double *sPh = reinterpret_cast<double *>(sTile + tile_bytes);
+    const int total = r.wt_prefix[B.B];
+    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * per;
+    const int t1 = min(total, t0 + per);
+    if (t0 >= t1) return;
+    int b = upper_index(r.wt_prefix, B.B, t0);
+    WaveDesc d = B.desc[b];
+    const int *pref = r.tpref + d.tpref_off;
+    int j = upper_index(pref, d.L - 1, t0 - r.wt_prefix[b]);
+    FactPlanDev P = batch_wave_plan(B, d);
+    bool stale = true;
+    for (int tile = t0; tile < t1; tile++) {
+        while (tile >= r.wt_prefix[b + 1]) {  // next waveform with tiles
+            b++;
+            d = B.desc[b];
+            pref = r.tpref + d.tpref_off;
+            P = batch_wave_plan(B, d);
+            j = 0;
+            stale = true;
+        }
+        const int lt = tile - r.wt_prefix[b];
+        while (lt >= pref[j + 1]) {
+            j++;
+            stale = true;
+        }
+        if (stale) {
+            __syncthreads();
+            const uint4 *src = reinterpret_cast<const uint4 *>(tiles + (long)(d.seg_off + j) * tile_bytes);
+            uint4 *dst = reinterpret_cast<uint4 *>(sTile);
+            const int nv = (int)(tile_bytes >> 4);
+            for (int i = tid; i < nv; i += NT) dst[i] = src[i];
+            const long np = (long)(d.L - 1) * 3;
+            if (tid < 24) sPh[tid] = p.pcoef[d.pc_off + (tid & 7) * np + (long)j * 3 + (tid >> 3)];
+            __syncthreads();
+            stale = false;
+        }
+        fact16_tile_body<RB, WAVES, ARITH>(P, sTile, sPh, RSTRH, KB, r.segs[d.seg_off + j],
+                                           lt - pref[j], d.dt, p.wave + d.wave_off / 2);
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_fact16_prep_kernel(BatchDev B, BatchPtrs p,
+                                                                 unsigned char *tiles,
+                                                                 long tile_bytes, int RSTRH, int KB,
+                                                                 int RB)
+{
+    const int gi = blockIdx.x;
+    const int b = upper_index(B.seg_prefix, B.B, gi);
+    const WaveDesc d = B.desc[b];
+    const int j = gi - B.seg_prefix[b];
+    fact16_prep_block(batch_wave_plan(B, d), p.interp + d.interp_off,
+                      reinterpret_cast<const double2 *>(p.ylm + d.ylm_off), p.t + d.t_off, d.L, j,
+                      blockIdx.y, tiles + (long)gi * tile_bytes, RSTRH, KB, RB);
 }
 
 int fact16_tile(int variant) { return 16 * F16_NB * (variant == 4 ? 4 : 8); }
@@ -416,6 +490,58 @@ int launch_fact16_sum(const Fact16Args &a, int RB, int max_blocks, int variant, 
     }
     return arith ? launch_fact16_w<8, true>(a, RB, blocks, lds, stream)
                  : launch_fact16_w<8, false>(a, RB, blocks, lds, stream);
+}
+
+template <int WAVES, bool ARITH>
+static int launch_batch_fact16_w(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
+                                 const unsigned char *tiles, long tile_bytes, int RSTRH, int KB,
+                                 int blocks, size_t lds, hipStream_t stream)
+{
+    switch (B.uni.RB) {
+#define FEW_CASEB(X)                                                                             \
+    case X:                                                                                      \
+        FEW_HIP(hipFuncSetAttribute((const void *)batch_fact16_kernel<X, WAVES, ARITH>,          \
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
+        hipLaunchKernelGGL((batch_fact16_kernel<X, WAVES, ARITH>), dim3(blocks),                \
+                           dim3(64 * WAVES), lds, stream, B, r, p, tiles, tile_bytes, RSTRH, KB); \
+        break;
+        FEW_CASEB(1) FEW_CASEB(2) FEW_CASEB(3) FEW_CASEB(4) FEW_CASEB(5) FEW_CASEB(6)
+        FEW_CASEB(7) FEW_CASEB(8) FEW_CASEB(9) FEW_CASEB(10) FEW_CASEB(11) FEW_CASEB(12)
+        FEW_CASEB(13) FEW_CASEB(14) FEW_CASEB(15) FEW_CASEB(16)
+#undef FEW_CASEB
+    default:
+        set_error("batch mixed16 sum: %d row blocks unsupported", B.uni.RB);
+        return FEW_ERR_UNSUPPORTED;
+    }
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
+int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
+                        unsigned char *tiles, long tile_bytes, int RSTRH, int KB, int variant,
+                        int stage, hipStream_t stream)
+{
+    if (stage & 1) {
+        if (B.total_intervals > 0) {
+            dim3 grid(B.total_intervals, 2 * B.uni.RB);
+            hipLaunchKernelGGL(batch_fact16_prep_kernel, grid, dim3(256), 0, stream, B, p, tiles,
+                               tile_bytes, RSTRH, KB, B.uni.RB);
+            FEW_HIP(hipGetLastError());
+        }
+    }
+    if (!(stage & 2)) return FEW_OK;
+    const size_t lds = (size_t)tile_bytes + 24 * sizeof(double);
+    if (lds > 160 * 1024) {
+        set_error("batch mixed16 sum: coefficient tile of %zu bytes exceeds LDS", lds);
+        return FEW_ERR_UNSUPPORTED;
+    }
+    const int blocks = device_cus16() * (lds <= 80 * 1024 ? 2 : 1);
+    const bool arith = B.uni.rows_arith != 0;
+    if (variant == 4)
+        return arith ? launch_batch_fact16_w<4, true>(B, r, p, tiles, tile_bytes, RSTRH, KB, blocks, lds, stream)
+                     : launch_batch_fact16_w<4, false>(B, r, p, tiles, tile_bytes, RSTRH, KB, blocks, lds, stream);
+    return arith ? launch_batch_fact16_w<8, true>(B, r, p, tiles, tile_bytes, RSTRH, KB, blocks, lds, stream)
+                 : launch_batch_fact16_w<8, false>(B, r, p, tiles, tile_bytes, RSTRH, KB, blocks, lds, stream);
 }
 
 }  // namespace few

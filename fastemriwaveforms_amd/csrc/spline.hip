@@ -20,6 +20,7 @@
 
 #pragma clang fp contract(off)
 
+#include "batch.hpp"
 #include "few_common.hpp"
 
 namespace few {
@@ -77,15 +78,12 @@ __host__ __device__ inline void prep_row(int i, int n, double x0, double x1, dou
     }
 }
 
-// The whole per-system computation; host-callable so the CPU test harness
-// can run this exact code against the oracle.
-__host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
+// The whole computation of system s of group G: n knots, scratch column col
+// of ns columns.  Host-callable so the CPU test harness runs this exact code
+// against the oracle.
+__host__ __device__ void spline_system_core(const SplineGroup &G, int s, int n, long col, long ns,
+                                            const SplineScratch &a)
 {
-    const SplineGroup &G = a.g[gi];
-    if (s >= G.nsys) return;
-    const long col = a.col_prefix[gi] + s;
-    const long ns = a.nsys_total;
-    const int n = a.length;
 
     const double *yv = G.y + (long)s * G.y_sys;  // may alias out (in-place build)
     const double *tv = G.t + (long)s * G.t_sys;
@@ -318,6 +316,14 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
 #undef YK
 }
 
+__host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
+{
+    const SplineGroup &G = a.g[gi];
+    if (s >= G.nsys) return;
+    SplineScratch sc{a.sD, a.sDU, a.sDL, a.sB, a.info};
+    spline_system_core(G, s, a.length, a.col_prefix[gi] + s, a.nsys_total, sc);
+}
+
 __global__ __launch_bounds__(64) void spline_build_kernel(SplineArgs a)
 {
     int gi = 0;
@@ -345,6 +351,46 @@ void fill_spline_args(const SplineGroup *groups, int ngroups, int length, double
     a.sDL = scratch4[2];
     a.sB = scratch4[3];
     a.info = info_out;
+}
+
+// Batched build: every system of every waveform (2K amplitude arrays, then
+// the 3 phase arrays in dense-output form), one lane per system.
+__global__ __launch_bounds__(64) void batch_spline_kernel(BatchDev B, BatchPtrs p)
+{
+    const long sg = (long)blockIdx.x * 64 + threadIdx.x;
+    if (sg >= B.total_systems) return;
+    const int b = upper_index(B.sys_prefix, B.B, sg);
+    const WaveDesc d = B.desc[b];
+    const long ls = sg - B.sys_prefix[b];
+    const long K = d.K, L = d.L;
+    const double *teuk = p.teuk + d.teuk_off;
+    const double *t = p.t + d.t_off;
+    double *interp = p.interp + d.interp_off;
+    SplineGroup G;
+    int s;
+    if (ls < K) {
+        G = SplineGroup{teuk, 2, 2 * K, t, 0, 1, interp, L * 2 * K, 1, 2 * K, (int)K, 0};
+        s = (int)ls;
+    } else if (ls < 2 * K) {
+        G = SplineGroup{teuk + 1, 2, 2 * K, t, 0, 1, interp + K, L * 2 * K, 1, 2 * K, (int)K, 0};
+        s = (int)(ls - K);
+    } else {
+        G = SplineGroup{p.pk + d.pk_off, L, 1, t, 0, 1, p.pcoef + d.pc_off, 0, 0, 0, 3, 1};
+        s = (int)(ls - 2 * K);
+    }
+    const long f = L * (2 * K + 3);
+    double *sc = p.scratch + d.sc_off;
+    SplineScratch scr{sc, sc + f, sc + 2 * f, sc + 3 * f, nullptr};
+    spline_system_core(G, s, (int)L, ls, 2 * K + 3, scr);
+}
+
+int launch_batch_splines(const BatchDev &B, const BatchPtrs &p, hipStream_t stream)
+{
+    if (B.total_systems <= 0) return FEW_OK;
+    const long blocks = (B.total_systems + 63) / 64;
+    hipLaunchKernelGGL(batch_spline_kernel, dim3((unsigned)blocks), dim3(64), 0, stream, B, p);
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
 }
 
 int launch_spline_build(const SplineGroup *groups, int ngroups, int length, double *scratch4[4],

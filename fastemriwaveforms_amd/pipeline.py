@@ -111,3 +111,97 @@ class ModeSumProblem:
         self.build_splines()
         self.mode_sum(precision)
         return self.wave
+
+
+BATCH_STAGE_SPLINES = 1
+BATCH_STAGE_PLAN = 2
+BATCH_STAGE_SUM = 4
+
+
+class BatchProblem:
+    """Many waveforms (uneven knots, mode subsets, durations) in packed HBM buffers.
+
+    One launch per stage covers the whole batch: splines of every waveform,
+    interval plans (+ coefficient tiles), and the persistent mode sum over the
+    (waveform, interval, tile) sequence.  :meth:`run` allocates nothing and
+    never synchronises, so it can be captured in a HIP graph
+    (:meth:`capture`).
+    """
+
+    def __init__(self, inputs: list, device: torch.device | str = "cuda"):
+        dev = torch.device(device)
+        self.inputs = inputs
+        B = len(inputs)
+        self.B = B
+        L = np.array([x.L for x in inputs], np.int32)
+        K = np.array([x.K for x in inputs], np.int32)
+        N = np.array([x.num_pts for x in inputs], np.int32)
+        dt = np.array([x.dt for x in inputs], np.float64)
+        m = np.ascontiguousarray(np.concatenate([x.m for x in inputs]), np.int32)
+        k = np.ascontiguousarray(np.concatenate([x.k for x in inputs]), np.int32)
+        n = np.ascontiguousarray(np.concatenate([x.n for x in inputs]), np.int32)
+        self.handle = lib().few_batch_create(B, L.ctypes.data, K.ctypes.data, N.ctypes.data,
+                                             dt.ctypes.data, m.ctypes.data, k.ctypes.data,
+                                             n.ctypes.data)
+        if not self.handle:
+            check(1, "batch_create")
+        totals = np.zeros(10, np.int64)
+        offsets = np.zeros((B, 7), np.int64)
+        check(lib().few_batch_layout(self.handle, totals.ctypes.data, offsets.ctypes.data),
+              "batch_layout")
+        self.totals, self.offsets = totals, offsets
+        self.supported16 = bool(totals[9] & 2)
+        self.mode_samples = int(np.sum(K.astype(np.int64) * N))
+        self.samples = int(np.sum(N))
+        # pack host buffers (doubles) once, then one copy each to HBM
+        host = [np.zeros(int(totals[i]), np.float64) for i in range(4)]
+        for b, x in enumerate(inputs):
+            o = offsets[b]
+            host[0][o[0]:o[0] + x.L] = x.t
+            host[1][o[1]:o[1] + 2 * x.L * x.K] = np.ascontiguousarray(x.teuk).view(np.float64).ravel()
+            host[2][o[2]:o[2] + 3 * x.L] = np.ascontiguousarray(x.phase_knots).ravel()
+            host[3][o[3]:o[3] + 4 * x.K] = np.ascontiguousarray(x.ylms).view(np.float64)
+        self.t, self.teuk, self.pk, self.ylm = (torch.from_numpy(h).to(dev) for h in host)
+        self.interp = torch.empty(int(totals[4]), dtype=torch.float64, device=dev)
+        self.pcoef = torch.empty(int(totals[5]), dtype=torch.float64, device=dev)
+        self.wave = torch.zeros(int(totals[6]) // 2, dtype=torch.complex128, device=dev)
+        nbytes = int(lib().few_batch_workspace_bytes(self.handle, PRECISION_CODES["mixed16"]))
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.variant = 0
+
+    def waveform(self, b: int) -> torch.Tensor:
+        o = int(self.offsets[b, 6]) // 2
+        return self.wave[o:o + self.inputs[b].num_pts]
+
+    def stages(self, precision: str = "mixed16", stage: int = 7) -> None:
+        rc = lib().few_batch_run_async(
+            self.handle, self.t.data_ptr(), self.teuk.data_ptr(), self.pk.data_ptr(),
+            self.ylm.data_ptr(), self.interp.data_ptr(), self.pcoef.data_ptr(),
+            self.wave.data_ptr(), PRECISION_CODES[precision], self.variant, stage,
+            self.ws.data_ptr(), self.ws.numel(), _stream())
+        check(rc, "batch_run")
+
+    def run(self, precision: str = "mixed16") -> torch.Tensor:
+        """Zero the outputs and run every stage for every waveform."""
+        self.wave.zero_()
+        self.stages(precision, 7)
+        return self.wave
+
+    def capture(self, precision: str = "mixed16") -> "torch.cuda.CUDAGraph":
+        """Capture :meth:`run` in a HIP graph (warm-up run first, on a side stream)."""
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self.run(precision)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run(precision)
+        return g
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            lib().few_batch_destroy(h)
+            self.handle = None

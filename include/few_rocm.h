@@ -151,6 +151,39 @@ int few_get_waveform_plan_async(void *d_waveform, const double *interp_array,
                                 double delta_t, const double *d_h_t, int precision, int variant,
                                 int stage, void *workspace, size_t workspace_bytes, void *stream);
 
+/*
+ * Batched mode sum: B waveforms with their own knots (L_b), modes (K_b, a
+ * subset of any mode set), output lengths (N_b) and dt_b, one launch per
+ * stage for the whole batch.  No reference counterpart (the reference sums
+ * one waveform per call; its only batching is sequential, in Python,
+ * src/few/waveform/base.py:256-268).
+ *
+ * few_batch_create takes HOST arrays: h_L, h_K, h_N, h_dt (B,) and the
+ * int32 mode indices of all waveforms concatenated (sum K_b).  Returns NULL
+ * on error.  Packed device buffers, per waveform b in order, in doubles:
+ *   t (L_b) | teuk complex (L_b, K_b) | phase knots (3, L_b) |
+ *   ylm complex (2 K_b): Y+ then Y- | interp (4, L_b, 2K_b) |
+ *   pcoef (8, L_b-1, 3) | wave complex (N_b), accumulated into.
+ * few_batch_layout fills totals10 = the 7 packed sizes (doubles), then
+ * spline systems, intervals and a supported mask (1: FP64, 2: MIXED16), and
+ * optionally offsets (B x 7).
+ * few_batch_run_async: precision FEW_PRECISION_FP64 (control run, the
+ * reference arithmetic) or FEW_PRECISION_MIXED16; stage is a mask: 1 =
+ * splines of every waveform, 2 = interval plan (+ coefficient tiles for
+ * MIXED16), 4 = the mode sum.  Stage 1 overwrites the coefficient tiles, so
+ * run 2 after it.  Workspace: few_batch_workspace_bytes(), caller-owned.
+ * No host synchronisation and no allocation: capturable in a hipGraph.
+ */
+void *few_batch_create(int B, const int *h_L, const int *h_K, const int *h_N, const double *h_dt,
+                       const int *h_m, const int *h_k, const int *h_n);
+void few_batch_destroy(void *batch);
+int few_batch_layout(const void *batch, long *totals10, long *offsets);
+size_t few_batch_workspace_bytes(const void *batch, int precision);
+int few_batch_run_async(const void *batch, const double *t, const double *teuk,
+                        const double *phase_knots, const double *ylm, double *interp, double *pcoef,
+                        void *wave, int precision, int variant, int stage, void *workspace,
+                        size_t workspace_bytes, void *stream);
+
 /* Frees the library-owned workspaces of the synchronous drop-in entries. */
 int few_release_workspace(void);
 
