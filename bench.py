@@ -84,6 +84,86 @@ def time_path(prob, precision, steps, warmup):
     return el / steps * 1e3, ms
 
 
+def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256):
+    """Uneven batch (durations U(0.1, 2) yr, knots U{50..200}, modes U{50..3000} of the
+    3843 set, dt = 10 s): the whole batch in one launch per stage, replayed as one
+    captured HIP graph, variant from the autotuner; FP64 control run beside it."""
+    import torch
+    import oracle
+    from helpers import oracle_waveform, spread_indices
+    from fastemriwaveforms_amd import autotune, synthetic
+    from fastemriwaveforms_amd.pipeline import (BATCH_STAGE_PLAN, BATCH_STAGE_SPLINES,
+                                                BATCH_STAGE_SUM, BatchProblem)
+    t0 = time.perf_counter()
+    inputs = synthetic.make_batch(count)
+    bp = BatchProblem(inputs, dev)
+    log(f"[bench] batch: {count} waveforms, {bp.samples} samples, {bp.mode_samples:.4g} "
+        f"mode-samples, {int(bp.totals[8])} intervals, ws {bp.ws.numel() / 2**30:.2f} GiB "
+        f"(setup {time.perf_counter() - t0:.1f} s)")
+    best = autotune.tune_batch(bp, reps=1)
+    log(f"[bench] batch autotune: {best}")
+    g = bp.capture("mixed16")
+    for _ in range(warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    ms_graph = a.elapsed_time(b) / steps
+    # stage breakdown (eager, same launches)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    bp.wave.zero_()
+    ev[0].record()
+    bp.stages("mixed16", BATCH_STAGE_SPLINES)
+    ev[1].record()
+    bp.stages("mixed16", BATCH_STAGE_PLAN)
+    ev[2].record()
+    bp.stages("mixed16", BATCH_STAGE_SUM)
+    ev[3].record()
+    torch.cuda.synchronize()
+    stage_ms = {"spline": ev[0].elapsed_time(ev[1]), "plan_tiles": ev[1].elapsed_time(ev[2]),
+                "sum": ev[2].elapsed_time(ev[3])}
+    h16 = bp.wave.clone()
+    # oracle check on a subset: 8 waveforms spread over the batch, spread samples
+    errs = []
+    for bi in np.linspace(0, count - 1, 8).astype(int):
+        inp = inputs[bi]
+        idx = spread_indices(inp.num_pts, count=300, seed=int(bi))
+        ref = oracle_waveform(inp, idx, nthreads=min(16, os.cpu_count() or 1))
+        errs.append(oracle.rel_l2(bp.waveform(int(bi)).cpu().numpy()[idx], ref))
+    out = {"waveforms": count, "samples": bp.samples, "mode_samples": bp.mode_samples,
+           "intervals": int(bp.totals[8]), "variant": best["variant"], "graph_ms_per_step": ms_graph,
+           "waveforms_per_s": count / (ms_graph * 1e-3),
+           "mode_samples_per_s": bp.mode_samples / (ms_graph * 1e-3), "stage_ms": stage_ms,
+           "rel_l2_vs_oracle_max_of_8": float(max(errs))}
+    log(f"[bench] batch mixed16 graph: {ms_graph:.2f} ms/step, {out['waveforms_per_s']:.1f} "
+        f"waveforms/s, {out['mode_samples_per_s']:.4g} mode-samples/s, stages {stage_ms}, "
+        f"rel-L2 (8 waveforms vs oracle) max {max(errs):.3e}")
+    if fp64:
+        reps = 2
+        bp.run("fp64")
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(reps):
+            bp.run("fp64")
+        b.record()
+        torch.cuda.synchronize()
+        ms64 = a.elapsed_time(b) / reps
+        diff = torch.linalg.vector_norm(h16 - bp.wave) / torch.linalg.vector_norm(bp.wave)
+        out["fp64_control"] = {"ms_per_step": ms64, "waveforms_per_s": count / (ms64 * 1e-3),
+                               "mode_samples_per_s": bp.mode_samples / (ms64 * 1e-3),
+                               "mixed16_rel_l2_vs_fp64_all_samples": float(diff)}
+        out["speedup_vs_fp64_control"] = ms64 / ms_graph
+        log(f"[bench] batch fp64 control: {ms64:.1f} ms/step, speed-up {ms64 / ms_graph:.2f}x, "
+            f"mixed16 rel-L2 vs fp64 (all samples) {float(diff):.3e}")
+    del bp, h16, g
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +176,9 @@ def main():
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 256-waveform uneven batch leg")
+    ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-traffic "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -182,6 +265,10 @@ def main():
                     "pipe": "FP64 VALU", "algorithmic": f"{DIRECT_FLOP} FLOP/mode-sample x {K * N} "
                                                         "mode-samples, sincos counted 0"}
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(dev, max(2, args.steps // 2), 1, not args.no_fp64, args.batch_count)
+
     cpu = None
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle, 1 thread)")
@@ -210,6 +297,7 @@ def main():
         "kernel_ms": ms,
         "fp64_plain_gpu": fp64,
         "speedup_vs_fp64_plain_gpu": (fp64["ms_per_step"] / ms_step) if fp64 else None,
+        "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -43,9 +43,10 @@ def _store(d: dict) -> None:
         pass  # a read-only home only costs re-tuning
 
 
-def _key(kind: str, rows: int, cols: int) -> str:
+def _key(kind: str, a: int, b: int) -> str:
+    """single: a, b = plan rows, columns; batch: a, b = total intervals, waveforms."""
     name = torch.cuda.get_device_name(torch.cuda.current_device())
-    return f"{name}|{kind}|rows{rows}|cols{cols}"
+    return f"{name}|{kind}|{a}|{b}"
 
 
 def _time(fn, reps: int) -> float:
@@ -93,8 +94,7 @@ def tune_problem(prob, reps: int = 3, use_cache: bool = True, candidates=None):
 def tune_batch(bp, reps: int = 2, use_cache: bool = True):
     """Pick the variant of the batched mixed16 sum; sets bp.variant."""
     from .pipeline import BATCH_STAGE_PLAN, BATCH_STAGE_SPLINES, BATCH_STAGE_SUM
-    rows = int(bp.totals[9])
-    key = _key("batch", rows, len(bp.inputs))
+    key = _key("batch", int(bp.totals[8]), len(bp.inputs)) + f"|ms{bp.mode_samples:.2e}"
     cache = _load() if use_cache else {}
     if use_cache and key in cache:
         bp.variant = cache[key]["variant"]
