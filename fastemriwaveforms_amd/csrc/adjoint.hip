@@ -1,0 +1,400 @@
+// Reverse-mode kernels of the interpolated mode sum (FP64) and of the
+// not-a-knot spline build.  The formulas are stated in adjoint.hpp.
+//
+// Mode-sum adjoint: one workgroup owns one chunk of one interval's samples
+// (at most `chunk` samples, never crossing an interval boundary) and one
+// group of ADJ_MODES modes, one mode per lane.  The chunk's sample state
+// (x, the three phases, g) is computed once per sample into LDS, then every
+// lane walks the staged samples and keeps its mode's twelve moments in
+// registers: sum x^p dL/dRe A, sum x^p dL/dIm A (p = 0..3) and
+// sum xp^p dL/dphi.  Nothing is reduced across lanes per sample; per chunk
+// the amplitude moments go to a partial buffer (chunk, 4, 2K) and the phase
+// moments (weighted by m, k, n) are reduced over the workgroup.  A second
+// kernel adds the chunks of each interval in a fixed order (deterministic).
+#include <math.h>
+
+#include "adjoint.hpp"
+#include "modesum_direct.hpp"
+
+namespace few {
+
+#define ADJ_MODES 128   // lanes (modes) per workgroup
+#define ADJ_SUB 128     // samples staged in LDS per step
+
+// LAPACK dgtsv (NRHS = 1) on vectors with stride ws: Gaussian elimination
+// with partial pivoting, rounding exactly as written (contraction off).
+__host__ __device__ static int dgtsv_strided(int n, double *dl, double *d, double *du, double *b,
+                                             long ws)
+{
+#pragma clang fp contract(off)
+#define DL(i) dl[(long)(i) * ws]
+#define DD(i) d[(long)(i) * ws]
+#define DU(i) du[(long)(i) * ws]
+#define BB(i) b[(long)(i) * ws]
+    double fact, temp;
+    int i;
+    for (i = 0; i < n - 2; i++) {
+        if (fabs(DD(i)) >= fabs(DL(i))) {
+            if (DD(i) != 0.0) {
+                fact = DL(i) / DD(i);
+                DD(i + 1) = DD(i + 1) - fact * DU(i);
+                BB(i + 1) = BB(i + 1) - fact * BB(i);
+            } else {
+                return i + 1;
+            }
+            DL(i) = 0.0;
+        } else {
+            fact = DD(i) / DL(i);
+            DD(i) = DL(i);
+            temp = DD(i + 1);
+            DD(i + 1) = DU(i) - fact * temp;
+            DL(i) = DU(i + 1);
+            DU(i + 1) = -fact * DL(i);
+            DU(i) = temp;
+            temp = BB(i);
+            BB(i) = BB(i + 1);
+            BB(i + 1) = temp - fact * BB(i + 1);
+        }
+    }
+    if (n > 1) {
+        i = n - 2;
+        if (fabs(DD(i)) >= fabs(DL(i))) {
+            if (DD(i) != 0.0) {
+                fact = DL(i) / DD(i);
+                DD(i + 1) = DD(i + 1) - fact * DU(i);
+                BB(i + 1) = BB(i + 1) - fact * BB(i);
+            } else {
+                return i + 1;
+            }
+        } else {
+            fact = DD(i) / DL(i);
+            DD(i) = DL(i);
+            temp = DD(i + 1);
+            DD(i + 1) = DU(i) - fact * temp;
+            DU(i) = temp;
+            temp = BB(i);
+            BB(i) = BB(i + 1);
+            BB(i + 1) = temp - fact * BB(i + 1);
+        }
+    }
+    if (DD(n - 1) == 0.0) return n;
+    BB(n - 1) = BB(n - 1) / DD(n - 1);
+    if (n > 1) BB(n - 2) = (BB(n - 2) - DU(n - 2) * BB(n - 1)) / DD(n - 2);
+    for (i = n - 3; i >= 0; i--) BB(i) = (BB(i) - DU(i) * BB(i + 1) - DL(i) * BB(i + 2)) / DD(i);
+    return 0;
+#undef DL
+#undef DD
+#undef DU
+#undef BB
+}
+
+__host__ __device__ int spline_adjoint_core(const double *x, int L, const double *g, long gp,
+                                            long gk, double *gy, long gyk, double *w, long ws)
+{
+#pragma clang fp contract(off)
+#define G(p, i) g[(long)(p) * gp + (long)(i) * gk]
+#define GY(i) gy[(long)(i) * gyk]
+    double *gz = w, *dl = w + (long)L * ws, *dd = w + 2L * L * ws, *du = w + 3L * L * ws;
+#define GZ(i) gz[(long)(i) * ws]
+    // final derivatives (interpolate.cu:242-280) fold into interval L-2
+    const double h = x[L - 1] - x[L - 2];
+    const double f1 = G(1, L - 1), f2 = G(2, L - 1), f3 = G(3, L - 1);
+    // fill_coefficients (interpolate.cu:28-43) transposed, interval by interval
+    for (int i = 0; i < L; i++) {
+        GY(i) = G(0, i);
+        GZ(i) = 0.0;
+    }
+    for (int i = 0; i < L - 1; i++) {
+        const double dx = x[i + 1] - x[i];
+        double g1 = G(1, i), g2 = G(2, i), g3 = G(3, i);
+        if (i == L - 2) {
+            g1 += f1;
+            g2 += 2.0 * h * f1 + f2;
+            g3 += 3.0 * h * h * f1 + 3.0 * h * f2 + f3;
+        }
+        const double gs = 3.0 * g2 / dx - 2.0 * g3 / (dx * dx);
+        GZ(i) += g1 - 2.0 * g2 / dx + g3 / (dx * dx);
+        GZ(i + 1) += -g2 / dx + g3 / (dx * dx);
+        GY(i) -= gs / dx;
+        GY(i + 1) += gs / dx;
+    }
+    // A^T (A from prep_splines, interpolate.cu:49-102): sub-diagonal
+    // A[i][i+1], super-diagonal A[i+1][i]
+    for (int i = 0; i < L; i++) {
+        double sub = 0.0, sup = 0.0, dg;
+        if (i == L - 1) {
+            dg = x[L - 2] - x[L - 3];
+        } else if (i == 0) {
+            dg = x[2] - x[1];
+            sub = x[2] - x[0];
+        } else {
+            dg = 2 * ((x[i] - x[i - 1]) + (x[i + 1] - x[i]));
+            sub = x[i] - x[i - 1];
+        }
+        if (i < L - 1) sup = (i + 1 == L - 1) ? x[L - 1] - x[L - 3] : x[i + 2] - x[i + 1];
+        dd[(long)i * ws] = dg;
+        if (i < L - 1) {
+            dl[(long)i * ws] = sub;
+            du[(long)i * ws] = sup;
+        }
+    }
+    const int info = dgtsv_strided(L, dl, dd, du, gz, ws);
+    // fill_B (interpolate.cu:107-145) transposed
+    {
+        const double dx1 = x[1] - x[0], dx2 = x[2] - x[1], d = x[2] - x[0];
+        const double al = (dx1 + 2 * d) * dx2 / (dx1 * d), be = dx1 * dx1 / (dx2 * d);
+        GY(0) -= al * GZ(0);
+        GY(1) += (al - be) * GZ(0);
+        GY(2) += be * GZ(0);
+    }
+    for (int i = 1; i < L - 1; i++) {
+        const double dx1 = x[i] - x[i - 1], dx2 = x[i + 1] - x[i];
+        const double al = 3.0 * dx2 / dx1, be = 3.0 * dx1 / dx2;
+        GY(i - 1) -= al * GZ(i);
+        GY(i) += (al - be) * GZ(i);
+        GY(i + 1) += be * GZ(i);
+    }
+    {
+        const double dx1 = x[L - 2] - x[L - 3], dx2 = x[L - 1] - x[L - 2];
+        const double d = x[L - 1] - x[L - 3];
+        const double al = dx2 * dx2 / (dx1 * d), be = (2 * d + dx2) * dx1 / (dx2 * d);
+        GY(L - 3) -= al * GZ(L - 1);
+        GY(L - 2) += (al - be) * GZ(L - 1);
+        GY(L - 1) += be * GZ(L - 1);
+    }
+    return info;
+#undef G
+#undef GY
+#undef GZ
+}
+
+__global__ __launch_bounds__(64) void spline_adjoint_kernel(SplineAdjArgs a)
+{
+    long s = (long)blockIdx.x * 64 + threadIdx.x;
+    for (int gi = 0; gi < a.ngroups; gi++) {
+        const SplineAdjGroup &G = a.grp[gi];
+        if (s < G.nsys) {
+            const long col = (long)blockIdx.x * 64 + threadIdx.x;
+            spline_adjoint_core(a.t, a.L, G.g + s * G.g_sys, G.g_p, G.g_knot, G.gy + s * G.gy_sys,
+                                G.gy_knot, a.scratch + col, a.ns);
+            return;
+        }
+        s -= G.nsys;
+    }
+}
+
+// ------------------------------------------------------------ mode-sum adjoint
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(ADJ_MODES) void adjoint_fp64_kernel(AdjArgs a)
+{
+    __shared__ double s_x[ADJ_SUB], s_pp[ADJ_SUB], s_pt[ADJ_SUB], s_pr[ADJ_SUB];
+    __shared__ double s_gr[ADJ_SUB], s_gi[ADJ_SUB], s_xp[ADJ_SUB];
+    __shared__ double s_red[ADJ_MODES / 64][12];
+
+    const int c = blockIdx.x;
+    const int total = a.prefix[a.nseg];
+    if (c >= total) return;  // uniform over the workgroup
+    int lo_s = 0, hi_s = a.nseg;
+    while (hi_s - lo_s > 1) {
+        const int mid = (lo_s + hi_s) >> 1;
+        if (a.prefix[mid] <= c) lo_s = mid; else hi_s = mid;
+    }
+    const int j = lo_s;
+    const Segment sg = a.segs[j];
+    const int lo = sg.start + (c - a.prefix[j]) * a.chunk;
+    const int hi = min(lo + a.chunk, sg.end);
+    const int tid = threadIdx.x;
+    const int K = a.K;
+    const int mode = blockIdx.y * ADJ_MODES + tid;
+    const bool act = mode < K;
+
+    double cr0 = 0, cr1 = 0, cr2 = 0, cr3 = 0, ci0 = 0, ci1 = 0, ci2 = 0, ci3 = 0;
+    double ypr = 0, ypi = 0, ymr = 0, ymi = 0, mm = 0, kk = 0, nn = 0;
+    if (act) {
+        const long nb = (long)a.L * 2 * K;
+        const long br = (long)j * 2 * K + mode, bi = br + K;
+        cr0 = a.interp[br]; cr1 = a.interp[nb + br]; cr2 = a.interp[2 * nb + br]; cr3 = a.interp[3 * nb + br];
+        ci0 = a.interp[bi]; ci1 = a.interp[nb + bi]; ci2 = a.interp[2 * nb + bi]; ci3 = a.interp[3 * nb + bi];
+        const double2 yp = a.ylm[mode];
+        ypr = yp.x;
+        ypi = yp.y;
+        const int m = a.m[mode];
+        if (m != 0) {
+            const double2 ym = a.ylm[K + mode];
+            ymr = ym.x;
+            ymi = ym.y;
+        }
+        mm = (double)m;
+        kk = (double)a.k[mode];
+        nn = (double)a.n[mode];
+    }
+    const long np = (long)a.nseg * 3;
+    double rp[8], rt[8], rr[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        rp[q] = a.pcoef[q * np + (long)j * 3 + 0];
+        rt[q] = a.pcoef[q * np + (long)j * 3 + 1];
+        rr[q] = a.pcoef[q * np + (long)j * 3 + 2];
+    }
+
+    double dr0 = 0, dr1 = 0, dr2 = 0, dr3 = 0, di0 = 0, di1 = 0, di2 = 0, di3 = 0;
+    double R0 = 0, R1 = 0, R2 = 0, R3 = 0;
+    for (int base = lo; base < hi; base += ADJ_SUB) {
+        __syncthreads();
+        {
+            const int i = base + tid;
+            SampleState st;
+            sample_state(a.dt, i, sg.h_t, sg.pt, sg.pw, rp, rt, rr, st);
+            double gr = 0.0, gi = 0.0;
+            if (i < hi) {
+                const double2 g = a.grad[i];
+                gr = g.x;
+                gi = g.y;
+            }
+            s_x[tid] = st.x;
+            s_pp[tid] = st.Pp;
+            s_pt[tid] = st.Pt;
+            s_pr[tid] = st.Pr;
+            s_gr[tid] = gr;
+            s_gi[tid] = gi;
+            s_xp[tid] = a.dt * (double)i - sg.pt;
+        }
+        __syncthreads();
+        const int cnt = min(ADJ_SUB, hi - base);
+        for (int q = 0; q < cnt; q++) {
+            const double x = s_x[q];
+            const double phase = mode_phase(mm, s_pp[q], kk, s_pt[q], nn, s_pr[q]);
+            double e_im, e_re;
+            sincos(-phase, &e_im, &e_re);
+            const double gr = s_gr[q], gi = s_gi[q];
+            // q+ = conj(g) e, q- = conj(g) conj(e)
+            const double A1 = gr * e_re, A2 = gi * e_im, A3 = gr * e_im, A4 = gi * e_re;
+            const double qpr = A1 + A2, qpi = A3 - A4, qmr = A1 - A2, qmi = -(A3 + A4);
+            const double Wr = qpr * ypr - qpi * ypi, Wi = qpr * ypi + qpi * ypr;
+            const double Vr = qmr * ymr - qmi * ymi, Vi = qmr * ymi + qmi * ymr;
+            const double d_re = Wr + Vr, d_im = Vi - Wi;
+            const double a_re = cr0 + x * (cr1 + x * (cr2 + x * cr3));
+            const double a_im = ci0 + x * (ci1 + x * (ci2 + x * ci3));
+            const double rho = a_im * d_re - a_re * d_im;
+            const double x2 = x * x, x3 = x2 * x;
+            dr0 += d_re; dr1 += x * d_re; dr2 += x2 * d_re; dr3 += x3 * d_re;
+            di0 += d_im; di1 += x * d_im; di2 += x2 * d_im; di3 += x3 * d_im;
+            const double xp = s_xp[q], xp2 = xp * xp, xp3 = xp2 * xp;
+            R0 += rho; R1 += xp * rho; R2 += xp2 * rho; R3 += xp3 * rho;
+        }
+    }
+    if (act) {
+        const long K2 = 2L * K;
+        double *pa = a.part_amp + (long)c * 4 * K2;
+        pa[mode] = dr0;          pa[K + mode] = di0;
+        pa[K2 + mode] = dr1;     pa[K2 + K + mode] = di1;
+        pa[2 * K2 + mode] = dr2; pa[2 * K2 + K + mode] = di2;
+        pa[3 * K2 + mode] = dr3; pa[3 * K2 + K + mode] = di3;
+    }
+    // phase moments: sum over the group's modes of (m, k, n) * R_p, [p][q] order
+    const double Rs[4] = {R0, R1, R2, R3};
+    const int wv = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const double vm = wave_sum(mm * Rs[p]);
+        const double vk = wave_sum(kk * Rs[p]);
+        const double vn = wave_sum(nn * Rs[p]);
+        if (lane == 0) {
+            s_red[wv][p * 3 + 0] = vm;
+            s_red[wv][p * 3 + 1] = vk;
+            s_red[wv][p * 3 + 2] = vn;
+        }
+    }
+    __syncthreads();
+    if (tid < 12) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < ADJ_MODES / 64; w++) v += s_red[w][tid];
+        a.part_phase[((long)c * a.groups + blockIdx.y) * 12 + tid] = v;
+    }
+}
+
+// Interval j (blockIdx.y; j = L-1 and empty intervals give zeros) adds its
+// chunks in order.
+__global__ __launch_bounds__(256) void adjoint_reduce_kernel(AdjReduceArgs r)
+{
+    const int j = blockIdx.y;
+    const long K2 = 2L * r.K;
+    int c0 = 0, c1 = 0;
+    if (j < r.nseg) {
+        c0 = r.prefix[j];
+        c1 = r.prefix[j + 1];
+    }
+    const long s = (long)blockIdx.x * 256 + threadIdx.x;
+    if (s < K2) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int c = c0; c < c1; c++) {
+            const double *pa = r.part_amp + (long)c * 4 * K2 + s;
+#pragma unroll
+            for (int p = 0; p < 4; p++) acc[p] += pa[p * K2];
+        }
+#pragma unroll
+        for (int p = 0; p < 4; p++) r.g_interp[((long)p * r.L + j) * K2 + s] = acc[p];
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 12) {
+        const int v = threadIdx.x;
+        double acc = 0.0;
+        for (int c = c0; c < c1; c++)
+            for (int g = 0; g < r.groups; g++) acc += r.part_phase[((long)c * r.groups + g) * 12 + v];
+        const int p = v / 3, q = v % 3;
+        r.g_phase[((long)p * r.L + j) * 3 + q] = acc;
+    }
+}
+
+size_t adjoint_workspace_bytes(int L, int out_len, int K, int chunk, int groups)
+{
+    const long max_chunks = (long)(out_len + chunk - 1) / chunk + L;
+    size_t b = 0;
+    b += ((sizeof(Segment) * (size_t)L + 255) / 256) * 256;
+    b += ((sizeof(int) * (size_t)(L + 1) + 255) / 256) * 256;
+    b += ((sizeof(double) * (size_t)max_chunks * 8 * K + 255) / 256) * 256;
+    b += ((sizeof(double) * (size_t)max_chunks * groups * 12 + 255) / 256) * 256;
+    return b;
+}
+
+int launch_adjoint_fp64(const AdjArgs &a, int max_chunks, hipStream_t stream)
+{
+    if (max_chunks <= 0 || a.K <= 0) return FEW_OK;
+    hipLaunchKernelGGL(adjoint_fp64_kernel, dim3(max_chunks, a.groups), dim3(ADJ_MODES), 0, stream,
+                       a);
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
+int launch_adjoint_reduce(const AdjReduceArgs &r, hipStream_t stream)
+{
+    const long K2 = 2L * r.K;
+    const int bx = (int)((K2 + 255) / 256);
+    hipLaunchKernelGGL(adjoint_reduce_kernel, dim3(bx > 0 ? bx : 1, r.L), dim3(256), 0, stream, r);
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
+int launch_spline_adjoint(const SplineAdjArgs &a, hipStream_t stream)
+{
+    long total = 0;
+    for (int g = 0; g < a.ngroups; g++) total += a.grp[g].nsys;
+    if (total <= 0) return FEW_OK;
+    if (total > a.ns) {
+        set_error("spline_adjoint: %ld systems exceed %ld scratch columns", total, a.ns);
+        return FEW_ERR_ARG;
+    }
+    hipLaunchKernelGGL(spline_adjoint_kernel, dim3((unsigned)((total + 63) / 64)), dim3(64), 0,
+                       stream, a);
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
+}  // namespace few

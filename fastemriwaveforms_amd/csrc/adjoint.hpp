@@ -1,0 +1,79 @@
+// Adjoint (reverse mode) of the interpolated mode sum and of the spline
+// build, for a real loss L with upstream gradient g = dL/dh, i.e.
+// L = sum_i (g_re h_re + g_im h_im) to first order.
+//
+// Mode sum (make_waveform, interpolate.cu:618-686).  For a sample of
+// interval j with x = t - h_t[j], e = exp(-i phi), A the amplitude cubic:
+//   W = conj(g) e Y+,  V = [m != 0] conj(g) conj(e) Y-
+//   dL/dRe A = Re(W + V),  dL/dIm A = Im(V - W)
+//   dL/dphi  = Im(A) dL/dRe A - Re(A) dL/dIm A
+// and phi = m Phi_phi + k Phi_theta + n Phi_r.  The coefficients of
+// interval j collect x^p times these over the interval's samples
+// (segmented per-mode reductions); the phase cubics of interval j collect
+// xp^p dL/dPhi with xp = t - phase_t[j].
+//
+// Spline build (interpolate.cu:284-325): linear in the knot values for
+// fixed knots, so its adjoint is the transpose: final derivatives folded
+// back, fill_coefficients transposed, one transposed tridiagonal solve
+// A^T lambda = dL/dz, then fill_B transposed.
+#pragma once
+
+#include "few_common.hpp"
+
+namespace few {
+
+struct AdjArgs {
+    const double2 *grad;      // (out_len,) dL/dh
+    const double *interp;     // (4, L, 2K) amplitude coefficients
+    const double *pcoef;      // (8, L-1, 3) dense-output phase coefficients
+    const int *m, *k, *n;     // (K,)
+    const double2 *ylm;       // (2K,) Y+ then Y-
+    const Segment *segs;      // (L-1,) from the planner, chunk = tile
+    const int *prefix;        // (L,) exclusive chunk prefix
+    int nseg, L, K;
+    double dt;
+    int chunk;                // samples per chunk
+    int groups;               // mode groups (blockIdx.y)
+    double *part_amp;         // (max_chunks, 4, 2K) per-chunk partial dL/dc
+    double *part_phase;       // (max_chunks, groups, 12) per-chunk partial phase moments
+};
+
+struct AdjReduceArgs {
+    const int *prefix;
+    const double *part_amp;
+    const double *part_phase;
+    int nseg, L, K, groups;
+    double *g_interp;         // (4, L, 2K) out
+    double *g_phase;          // (4, L, 3) out
+};
+
+// One transposed spline system: gradient G(p, i) = g[p*gp + i*gi] of the
+// (4, length) coefficients of one array -> dL/dy(i) = gy[i*gyi].
+struct SplineAdjGroup {
+    const double *g;   // coefficient gradient of system 0
+    long g_sys, g_p, g_knot;
+    double *gy;        // knot-value gradient of system 0
+    long gy_sys, gy_knot;
+    int nsys;
+};
+
+struct SplineAdjArgs {
+    const double *t;          // (L,) knots shared by every system
+    SplineAdjGroup grp[3];
+    int ngroups;
+    int L;
+    long ns;                  // scratch columns (total systems)
+    double *scratch;          // 4 * L * ns doubles
+};
+
+// The transpose of one system's build; w holds 4 scratch vectors of L
+// entries with stride ws between knots.  Returns the solve's INFO.
+__host__ __device__ int spline_adjoint_core(const double *t, int L, const double *g, long gp,
+                                            long gk, double *gy, long gyk, double *w, long ws);
+
+size_t adjoint_workspace_bytes(int L, int out_len, int K, int chunk, int groups);
+int launch_adjoint_fp64(const AdjArgs &a, int max_chunks, hipStream_t stream);
+int launch_adjoint_reduce(const AdjReduceArgs &r, hipStream_t stream);
+int launch_spline_adjoint(const SplineAdjArgs &a, hipStream_t stream);
+
+}  // namespace few

@@ -15,6 +15,7 @@
 #include <mutex>
 #include <vector>
 
+#include "adjoint.hpp"
 #include "few_common.hpp"
 #include "few_rocm.h"
 #include "modesum_direct.hpp"
@@ -252,6 +253,30 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
         return FEW_ERR_LIMIT;
     }
     return launch_direct_fp64(a, (int)max_blocks, stream);
+}
+
+// ------------------------------------------------------------------ adjoint
+
+#define ADJ_CHUNK 2048   // samples per chunk of the mode-sum adjoint
+#define ADJ_GROUP 128    // modes per workgroup of the mode-sum adjoint
+
+struct AdjLayout {
+    size_t segs, prefix, amp, phase, total;
+    long max_chunks;
+    int groups;
+};
+static AdjLayout adj_layout(int L, int out_len, int K)
+{
+    AdjLayout a;
+    a.max_chunks = ((long)out_len + ADJ_CHUNK - 1) / ADJ_CHUNK + (L > 1 ? L - 1 : 1);
+    a.groups = (K + ADJ_GROUP - 1) / ADJ_GROUP;
+    size_t off = 0;
+    a.segs = off; off = align256(off + sizeof(Segment) * (size_t)(L > 1 ? L - 1 : 1));
+    a.prefix = off; off = align256(off + sizeof(int) * (size_t)L);
+    a.amp = off; off = align256(off + sizeof(double) * (size_t)a.max_chunks * 8 * (size_t)K);
+    a.phase = off; off = align256(off + sizeof(double) * (size_t)a.max_chunks * a.groups * 12);
+    a.total = off;
+    return a;
 }
 
 }  // namespace few
@@ -515,6 +540,134 @@ int few_get_waveform_plan_async(void *d_waveform, const double *interp_array,
                         static_cast<char *>(workspace), workspace_bytes,
                         reinterpret_cast<hipStream_t>(stream), static_cast<const FactPlan *>(plan),
                         variant, stage);
+}
+
+size_t few_adjoint_workspace_bytes(int init_len, int out_len, int num_modes, int precision)
+{
+    (void)precision;
+    if (init_len < 2 || out_len < 0 || num_modes < 0) return 0;
+    return adj_layout(init_len, out_len, num_modes).total;
+}
+
+int few_waveform_adjoint_async(const void *grad_wave, const double *interp_array,
+                               const double *phase_interp_t, const double *phase_interp_coeffs,
+                               const int *d_m, const int *d_k, const int *d_n, int init_len,
+                               int out_len, int num_teuk_modes, const void *d_Ylms, double delta_t,
+                               const double *h_t, double *g_interp, double *g_phase, int precision,
+                               void *workspace, size_t workspace_bytes, void *stream)
+{
+    int rc = check_mode_args(grad_wave, interp_array, phase_interp_coeffs, d_m, d_k, d_n, init_len,
+                             out_len, num_teuk_modes, d_Ylms, delta_t);
+    if (rc) return rc;
+    if (!phase_interp_t || !h_t || !g_interp || !g_phase || !workspace) {
+        set_error("waveform_adjoint: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (precision != FEW_PRECISION_FP64) {
+        set_error("waveform_adjoint: precision %d is not available", precision);
+        return FEW_ERR_UNSUPPORTED;
+    }
+    const int L = init_len, K = num_teuk_modes;
+    const AdjLayout al = adj_layout(L, out_len, K);
+    if (workspace_bytes < al.total) {
+        set_error("waveform_adjoint: workspace of %zu bytes, need %zu", workspace_bytes, al.total);
+        return FEW_ERR_ARG;
+    }
+    if (al.max_chunks > 0x7fffffffL || al.groups > 65535) {
+        set_error("waveform_adjoint: problem too large (%d samples, %d modes)", out_len, K);
+        return FEW_ERR_LIMIT;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    char *ws = static_cast<char *>(workspace);
+    Segment *segs = reinterpret_cast<Segment *>(ws + al.segs);
+    int *prefix = reinterpret_cast<int *>(ws + al.prefix);
+    PlanArgs p{h_t, phase_interp_t, L, delta_t, out_len, ADJ_CHUNK, segs, prefix};
+    rc = launch_plan(p, st);
+    if (rc) return rc;
+    AdjArgs a;
+    a.grad = static_cast<const double2 *>(grad_wave);
+    a.interp = interp_array;
+    a.pcoef = phase_interp_coeffs;
+    a.m = d_m;
+    a.k = d_k;
+    a.n = d_n;
+    a.ylm = static_cast<const double2 *>(d_Ylms);
+    a.segs = segs;
+    a.prefix = prefix;
+    a.nseg = L - 1;
+    a.L = L;
+    a.K = K;
+    a.dt = delta_t;
+    a.chunk = ADJ_CHUNK;
+    a.groups = al.groups;
+    a.part_amp = reinterpret_cast<double *>(ws + al.amp);
+    a.part_phase = reinterpret_cast<double *>(ws + al.phase);
+    if (K > 0 && out_len > 0) {
+        rc = launch_adjoint_fp64(a, (int)al.max_chunks, st);
+        if (rc) return rc;
+    }
+    AdjReduceArgs r;
+    r.prefix = prefix;
+    r.part_amp = a.part_amp;
+    r.part_phase = a.part_phase;
+    r.nseg = L - 1;
+    r.L = L;
+    r.K = K;
+    r.groups = (K > 0 && out_len > 0) ? al.groups : 0;
+    r.g_interp = g_interp;
+    r.g_phase = g_phase;
+    // with no samples the planner leaves every chunk count at zero and the
+    // reduction writes zeros
+    return launch_adjoint_reduce(r, st);
+}
+
+int few_spline_adjoint_async(const double *t, const double *g_coef, long g_sys, long g_p,
+                             long g_knot, int nsys, int length, double *g_y, long gy_sys,
+                             long gy_knot, double *scratch, void *stream)
+{
+    if (!t || !g_coef || !g_y || !scratch) {
+        set_error("spline_adjoint: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (nsys < 0 || length < 4) {
+        set_error("spline_adjoint: need length >= 4 (the 3-knot not-a-knot system is singular) "
+                  "and nsys >= 0 (got %d, %d)", length, nsys);
+        return FEW_ERR_ARG;
+    }
+    SplineAdjArgs a;
+    a.t = t;
+    a.grp[0] = SplineAdjGroup{g_coef, g_sys, g_p, g_knot, g_y, gy_sys, gy_knot, nsys};
+    a.ngroups = 1;
+    a.L = length;
+    a.ns = nsys;
+    a.scratch = scratch;
+    return launch_spline_adjoint(a, reinterpret_cast<hipStream_t>(stream));
+}
+
+int few_modes_adjoint_async(const double *g_interp, const double *g_phase, const double *t,
+                            int length, int num_modes, double *grad_teuk, double *grad_phase_knots,
+                            double *scratch, void *stream)
+{
+    if (!g_interp || !t || !grad_teuk || !scratch || (g_phase && !grad_phase_knots)) {
+        set_error("modes_adjoint: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (num_modes <= 0 || length < 4) {
+        set_error("modes_adjoint: need modes > 0 and length >= 4 (got %d, %d)", num_modes, length);
+        return FEW_ERR_ARG;
+    }
+    const long K = num_modes, L = length;
+    SplineAdjArgs a;
+    a.t = t;
+    // Re parts: g_interp (4, L, 2K) columns [0, K) -> grad_teuk (L, K) complex, real slots
+    a.grp[0] = SplineAdjGroup{g_interp, 1, L * 2 * K, 2 * K, grad_teuk, 2, 2 * K, num_modes};
+    a.grp[1] = SplineAdjGroup{g_interp + K, 1, L * 2 * K, 2 * K, grad_teuk + 1, 2, 2 * K, num_modes};
+    a.grp[2] = SplineAdjGroup{g_phase, 1, 3 * L, 3, grad_phase_knots, L, 1, g_phase ? 3 : 0};
+    a.ngroups = g_phase ? 3 : 2;
+    a.L = length;
+    a.ns = 2 * K + (g_phase ? 3 : 0);
+    a.scratch = scratch;
+    return launch_spline_adjoint(a, reinterpret_cast<hipStream_t>(stream));
 }
 
 int few_release_workspace(void)

@@ -184,6 +184,54 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
                         void *wave, int precision, int variant, int stage, void *workspace,
                         size_t workspace_bytes, void *stream);
 
+/*
+ * Reverse mode (no reference counterpart: the reference has no backward
+ * pass).  For a real loss L and the upstream gradient g = dL/dh (complex,
+ * out_len), these compute the exact derivatives of get_waveform
+ * (src/few/cutils/interpolate.cu:726-797 with make_waveform :444-689) and
+ * of interpolate_arrays (:284-325) for fixed knot times.
+ *
+ * few_waveform_adjoint_async: same mode-sum arguments as get_waveform
+ * (device pointers, phase_interp_t and h_t included) plus grad_wave;
+ * writes g_interp (4, init_len, 2K) = dL/d(interp_array) in its layout
+ * (row init_len-1, which the sum never reads, is zero) and g_phase
+ * (4, init_len, 3) = dL/dc_p of the cubic of phase q on interval j at
+ * [p][j][q] (the dense-output coefficients equal that cubic; see
+ * few_prepare_modes_async).  Segmented per-mode reductions over each
+ * interval's samples, in a fixed order (deterministic).  precision:
+ * FEW_PRECISION_FP64.  Workspace: few_adjoint_workspace_bytes().
+ */
+size_t few_adjoint_workspace_bytes(int init_len, int out_len, int num_modes, int precision);
+int few_waveform_adjoint_async(const void *grad_wave, const double *interp_array,
+                               const double *phase_interp_t, const double *phase_interp_coeffs,
+                               const int *d_m, const int *d_k, const int *d_n, int init_len,
+                               int out_len, int num_teuk_modes, const void *d_Ylms, double delta_t,
+                               const double *h_t, double *g_interp, double *g_phase, int precision,
+                               void *workspace, size_t workspace_bytes, void *stream);
+
+/*
+ * Transpose of the spline build (interpolate_arrays) for one group of
+ * systems of `length` >= 4 knots sharing t: g_coef[s*g_sys + p*g_p +
+ * i*g_knot] = dL/dc_p(i) of system s -> g_y[s*gy_sys + i*gy_knot] = dL/dy_i.
+ * The final-derivative row, the coefficient formulas, the tridiagonal solve
+ * (transposed, LAPACK dgtsv's pivoting) and the right-hand side are
+ * reversed in turn.  scratch: 4 * length * nsys doubles.
+ */
+int few_spline_adjoint_async(const double *t, const double *g_coef, long g_sys, long g_p,
+                             long g_knot, int nsys, int length, double *g_y, long gy_sys,
+                             long gy_knot, double *scratch, void *stream);
+
+/*
+ * Adjoint of few_prepare_modes_async: g_interp (4, length, 2K) and g_phase
+ * (4, length, 3) (as written by few_waveform_adjoint_async) -> grad_teuk
+ * (length, K) complex (dL/dRe + i dL/dIm) and, if g_phase is not NULL,
+ * grad_phase_knots (3, length).  One launch; scratch as in
+ * few_prepare_modes_async (4 * length * (2K + 3) doubles).
+ */
+int few_modes_adjoint_async(const double *g_interp, const double *g_phase, const double *t,
+                            int length, int num_modes, double *grad_teuk, double *grad_phase_knots,
+                            double *scratch, void *stream);
+
 /* Frees the library-owned workspaces of the synchronous drop-in entries. */
 int few_release_workspace(void);
 

@@ -66,6 +66,10 @@ def lib() -> ctypes.CDLL:
         _lib.orc_waveform_samples.restype = ctypes.c_int
         _lib.orc_cubic_to_dense.argtypes = [d, d, ctypes.c_int, ctypes.c_int, d]
         _lib.orc_cubic_to_dense.restype = None
+        _lib.orc_waveform_adjoint.argtypes = [d] + wave[:-1] + [d, d, ctypes.c_int]
+        _lib.orc_waveform_adjoint.restype = ctypes.c_int
+        _lib.orc_spline_adjoint.argtypes = [d, d, ctypes.c_int, ctypes.c_int, d]
+        _lib.orc_spline_adjoint.restype = ctypes.c_int
         _lib.orc_max_threads.argtypes = []
         _lib.orc_max_threads.restype = ctypes.c_int
     return _lib
@@ -178,6 +182,44 @@ def waveform_samples(idx, interp_array_tl, phase_t, phase_coeffs_flat, m, k, n, 
     if rc != 0:
         raise RuntimeError(f"oracle waveform_samples failed: {rc}")
     return out
+
+
+def waveform_adjoint(grad_wave, interp_array_tl, phase_t, phase_coeffs_flat, m, k, n, init_len,
+                     out_len, num_modes, ylms, dt, h_t, nthreads=0):
+    """Adjoint of get_waveform for L = sum Re(conj(g) h), g = grad_wave (complex, out_len).
+
+    Returns (g_interp (4, init_len, 2K) in the interp_array layout,
+    g_phase (4, init_len, 3): dL/dc_p of phase q (cubic coefficients) on interval j).
+    """
+    gw = np.ascontiguousarray(grad_wave, dtype=np.complex128)
+    assert len(gw) >= out_len
+    keep = [_f64(interp_array_tl), _f64(phase_t), _f64(phase_coeffs_flat), _i32(m), _i32(k),
+            _i32(n), _f64(h_t)]
+    if nthreads <= 0:
+        nthreads = min(8, os.cpu_count() or 1)
+    args, yl = _wave_args(keep[0], keep[1], keep[2], keep[3], keep[4], keep[5], init_len,
+                          out_len, num_modes, ylms, dt, keep[6], nthreads)
+    g_interp = np.zeros((4, init_len, 2 * num_modes))
+    g_phase = np.zeros((4, init_len, 3))
+    rc = lib().orc_waveform_adjoint(_dp(gw.view(np.float64)), *args[:-1], _dp(g_interp),
+                                    _dp(g_phase), int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"oracle waveform_adjoint failed: {rc}")
+    return g_interp, g_phase
+
+
+def spline_adjoint(t, g_coef):
+    """Transpose of interpolate_arrays' map y -> interp_array for fixed knots t.
+
+    g_coef: (4, ninterps, length).  Returns (g_y (ninterps, length), info)."""
+    g_coef = _f64(g_coef)
+    _, ninterps, length = g_coef.shape
+    t = _f64(t)
+    if t.ndim == 1:
+        t = _f64(np.tile(t, (ninterps, 1)))
+    g_y = np.zeros((ninterps, length))
+    info = lib().orc_spline_adjoint(_dp(t), _dp(g_coef), ninterps, length, _dp(g_y))
+    return g_y, info
 
 
 def max_threads() -> int:

@@ -453,6 +453,233 @@ void orc_cubic_to_dense(const double *coeffs, const double *t, int nphase, int l
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Adjoint of the mode sum and of the spline build.                          */
+/* The reference has no backward pass; these restate the exact derivatives  */
+/* of orc_get_waveform and orc_interpolate_arrays above, for the loss        */
+/*   L = sum_i Re(conj(g_i) h_i) = sum_i (g_re h_re + g_im h_im),            */
+/* g being the upstream gradient dL/dh (interleaved complex).                */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Gradient of L with respect to the coefficients the sum reads
+ * (make_waveform, src/few/cutils/interpolate.cu:618-686, restated in
+ * orc_eval_sample).  For sample i of interval j with x = t - h_t[j],
+ * e = exp(-i phi), P = e A, T = P Y+ + [m != 0] conj(P) Y-:
+ *   dL/dRe A = Re(q+ Y+) + [m != 0] Re(q- Y-)
+ *   dL/dIm A = -Im(q+ Y+) + [m != 0] Im(q- Y-)
+ *   dL/dphi  = Im(q+ A Y+) - [m != 0] Im(q- conj(A) Y-)
+ * with q+ = conj(g) e and q- = conj(g) conj(e); phi = m Phi_phi + k Phi_theta
+ * + n Phi_r, so dL/dPhi_phi = sum_modes m dL/dphi (k, n likewise).  The
+ * phase of interval j is the cubic sum_p c_p xp^p, xp = t - phase_t[j]
+ * (the dense-output form evaluated at :642 equals it identically, see
+ * orc_cubic_to_dense), so dL/dc_p = sum_i xp^p dL/dPhi.
+ *   g_interp : (4, init_len, 2K), the interp_array layout (slot p, knot j,
+ *              Re part at mode, Im part at K + mode); row init_len-1 and
+ *              intervals without samples stay zero
+ *   g_phase  : (4, init_len, 3): dL/dc_p of phase q on interval j at
+ *              [p][j][q] (cubic coefficients of the phase splines)
+ * Both outputs are overwritten.  Intervals are independent, so OpenMP
+ * splits them; each interval sums its samples in order.
+ */
+int orc_waveform_adjoint(const double *grad_wave, const double *interp_array,
+                         const double *phase_t, const double *phase_coeffs, const int *m,
+                         const int *k, const int *n, int init_len, int out_len, int num_modes,
+                         const double *ylms, double delta_t, const double *h_t,
+                         double *g_interp, double *g_phase, int nthreads)
+{
+    if (init_len < 2 || out_len < 0 || num_modes < 0) return -1;
+    const int K = num_modes;
+    const long num_base = (long)init_len * (2 * K);
+    const long num_phase = (long)(init_len - 1) * 3;
+    memset(g_interp, 0, sizeof(double) * 4 * (size_t)num_base);
+    memset(g_phase, 0, sizeof(double) * 4 * (size_t)init_len * 3);
+    int *start_inds = (int *)malloc(sizeof(int) * (size_t)(init_len + 1));
+    if (!start_inds) return -2;
+    int used = orc_find_start_inds(start_inds, h_t, delta_t, init_len, out_len);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
+#endif
+    for (int j = 0; j < used - 1; j++) {
+        double r_phi[8], r_theta[8], r_r[8];
+        for (int c = 0; c < 8; c++) {
+            r_phi[c] = phase_coeffs[c * num_phase + j * 3 + 0];
+            r_theta[c] = phase_coeffs[c * num_phase + j * 3 + 1];
+            r_r[c] = phase_coeffs[c * num_phase + j * 3 + 2];
+        }
+        const double start_t = h_t[j];
+        const double pt_here = phase_t[j];
+        const double seg = phase_t[j + 1] - phase_t[j];
+        double gp[4][3] = {{0}};
+        for (long i = start_inds[j]; i < start_inds[j + 1]; i++) {
+            const double t = delta_t * i;
+            const double x = t - start_t;
+            const double xpow[4] = {1.0, x, x * x, x * x * x};
+            const double s = (t - pt_here) / seg;
+            const double s1 = 1.0 - s;
+            const double Phi_phi = orc_dense_phase(r_phi, s, s1);
+            const double Phi_theta = orc_dense_phase(r_theta, s, s1);
+            const double Phi_r = orc_dense_phase(r_r, s, s1);
+            const double gr = grad_wave[2 * i], gi = grad_wave[2 * i + 1];
+            double dphi = 0.0, dtheta = 0.0, dr = 0.0;
+            for (int mode = 0; mode < K; mode++) {
+                const long ind_re = (long)j * (2 * K) + mode;
+                const long ind_im = ind_re + K;
+                double a_re = interp_array[ind_re] + interp_array[num_base + ind_re] * x +
+                              interp_array[2 * num_base + ind_re] * xpow[2] +
+                              interp_array[3 * num_base + ind_re] * xpow[3];
+                double a_im = interp_array[ind_im] + interp_array[num_base + ind_im] * x +
+                              interp_array[2 * num_base + ind_im] * xpow[2] +
+                              interp_array[3 * num_base + ind_im] * xpow[3];
+                const int mm = m[mode], kk = k[mode], nn = n[mode];
+                double phase = (double)mm * Phi_phi + (double)kk * Phi_theta + (double)nn * Phi_r;
+                double e_re = cos(-phase), e_im = sin(-phase);
+                double qp_re, qp_im, qm_re, qm_im;
+                cmul(gr, -gi, e_re, e_im, &qp_re, &qp_im);
+                cmul(gr, -gi, e_re, -e_im, &qm_re, &qm_im);
+                const double yp_re = ylms[2 * mode], yp_im = ylms[2 * mode + 1];
+                double u_re, u_im;
+                cmul(qp_re, qp_im, yp_re, yp_im, &u_re, &u_im);       /* q+ Y+ */
+                double d_re = u_re, d_im = -u_im;
+                double ay_re, ay_im, w_re, w_im;
+                cmul(a_re, a_im, yp_re, yp_im, &ay_re, &ay_im);       /* A Y+ */
+                cmul(qp_re, qp_im, ay_re, ay_im, &w_re, &w_im);
+                double rho = w_im;
+                if (mm != 0) {
+                    const double ym_re = ylms[2 * (K + mode)], ym_im = ylms[2 * (K + mode) + 1];
+                    double v_re, v_im;
+                    cmul(qm_re, qm_im, ym_re, ym_im, &v_re, &v_im);   /* q- Y- */
+                    d_re += v_re;
+                    d_im += v_im;
+                    cmul(a_re, -a_im, ym_re, ym_im, &ay_re, &ay_im);  /* conj(A) Y- */
+                    cmul(qm_re, qm_im, ay_re, ay_im, &w_re, &w_im);
+                    rho -= w_im;
+                }
+                for (int p = 0; p < 4; p++) {
+                    g_interp[p * num_base + ind_re] += xpow[p] * d_re;
+                    g_interp[p * num_base + ind_im] += xpow[p] * d_im;
+                }
+                dphi += (double)mm * rho;
+                dtheta += (double)kk * rho;
+                dr += (double)nn * rho;
+            }
+            const double xp = t - pt_here;
+            double xpp = 1.0;
+            for (int p = 0; p < 4; p++) {
+                gp[p][0] += xpp * dphi;
+                gp[p][1] += xpp * dtheta;
+                gp[p][2] += xpp * dr;
+                xpp *= xp;
+            }
+        }
+        for (int p = 0; p < 4; p++)
+            for (int q = 0; q < 3; q++) g_phase[((long)p * init_len + j) * 3 + q] = gp[p][q];
+    }
+    free(start_inds);
+    return 0;
+}
+
+/*
+ * Adjoint (transpose) of the linear map y -> interp_array performed by
+ * orc_interpolate_arrays (src/few/cutils/interpolate.cu:284-325) for fixed
+ * knots: given G = dL/d(interp_array) in its (4, ninterps, length) layout,
+ * writes dL/dy (ninterps, length).  Steps in reverse order:
+ *   final derivatives (:242-280) fold into interval length-2;
+ *   fill_coefficients (:28-43): c2 = 3 slope/dx - 2 z_i/dx - z_{i+1}/dx,
+ *     c3 = (z_i + z_{i+1} - 2 slope)/dx^2, c1 = z_i, slope = dy/dx;
+ *   the not-a-knot solve A z = b (:201-239) -> A^T lambda = dL/dz, solved
+ *     with the same restated dgtsv on the transposed tridiagonal;
+ *   fill_B (:49-145): b is linear in y with the coefficients of
+ *     orc_prep_row, applied transposed.
+ * Returns the first non-zero dgtsv INFO (a singular A, e.g. length 3).
+ */
+int orc_spline_adjoint(const double *t_arr, const double *g_coef, int ninterps, int length,
+                       double *g_y)
+{
+    if (ninterps <= 0 || length < 3) return -1;
+    const int L = length;
+    const size_t nb = (size_t)ninterps * L;
+    double *G = (double *)malloc(sizeof(double) * 4 * (size_t)L);
+    double *gz = (double *)malloc(sizeof(double) * (size_t)L);
+    double *dl = (double *)malloc(sizeof(double) * (size_t)L);
+    double *dd = (double *)malloc(sizeof(double) * (size_t)L);
+    double *du = (double *)malloc(sizeof(double) * (size_t)L);
+    double *b = (double *)malloc(sizeof(double) * (size_t)L);
+    double *ud = (double *)malloc(sizeof(double) * (size_t)L);
+    double *diag = (double *)malloc(sizeof(double) * (size_t)L);
+    double *ld = (double *)malloc(sizeof(double) * (size_t)L);
+    double *zero = (double *)calloc((size_t)L, sizeof(double));
+    int first_info = 0;
+    if (!G || !gz || !dl || !dd || !du || !b || !ud || !diag || !ld || !zero) {
+        first_info = -2;
+        goto done;
+    }
+    for (int s = 0; s < ninterps; s++) {
+        const double *x = &t_arr[(size_t)s * L];
+        double *gy = &g_y[(size_t)s * L];
+        for (int p = 0; p < 4; p++)
+            for (int i = 0; i < L; i++) G[p * L + i] = g_coef[p * nb + (size_t)s * L + i];
+        /* final derivatives at L-1 */
+        {
+            const double h = x[L - 1] - x[L - 2];
+            const double g1 = G[1 * L + L - 1], g2 = G[2 * L + L - 1], g3 = G[3 * L + L - 1];
+            G[1 * L + L - 2] += g1;
+            G[2 * L + L - 2] += 2.0 * h * g1 + g2;
+            G[3 * L + L - 2] += 3.0 * h * h * g1 + 3.0 * h * g2 + g3;
+        }
+        for (int i = 0; i < L; i++) {
+            gy[i] = G[i];  /* slot 0 is y itself */
+            gz[i] = 0.0;
+        }
+        for (int i = 0; i < L - 1; i++) {
+            const double dx = x[i + 1] - x[i];
+            const double g1 = G[1 * L + i], g2 = G[2 * L + i], g3 = G[3 * L + i];
+            const double gs = 3.0 * g2 / dx - 2.0 * g3 / (dx * dx);
+            gz[i] += g1 - 2.0 * g2 / dx + g3 / (dx * dx);
+            gz[i + 1] += -g2 / dx + g3 / (dx * dx);
+            gy[i] -= gs / dx;
+            gy[i + 1] += gs / dx;
+        }
+        /* the system matrix (orc_prep_row with any y: A depends on x only) */
+        for (int i = 0; i < L; i++) orc_prep_row(i, L, b, ud, diag, ld, x, zero);
+        /* A^T: sub-diagonal A[i][i+1] = ud[i], super-diagonal A[i+1][i] = ld[i+1] */
+        for (int i = 0; i < L - 1; i++) {
+            dl[i] = ud[i];
+            du[i] = ld[i + 1];
+        }
+        for (int i = 0; i < L; i++) dd[i] = diag[i];
+        int info = orc_dgtsv(L, dl, dd, du, gz);
+        if (info != 0 && first_info == 0) first_info = info;
+        /* b = B(y): transpose of the rows of orc_prep_row */
+        {
+            const double dx1 = x[1] - x[0], dx2 = x[2] - x[1], d = x[2] - x[0];
+            const double al = (dx1 + 2 * d) * dx2 / (dx1 * d), be = dx1 * dx1 / (dx2 * d);
+            gy[0] -= al * gz[0];
+            gy[1] += (al - be) * gz[0];
+            gy[2] += be * gz[0];
+        }
+        for (int i = 1; i < L - 1; i++) {
+            const double dx1 = x[i] - x[i - 1], dx2 = x[i + 1] - x[i];
+            const double al = 3.0 * dx2 / dx1, be = 3.0 * dx1 / dx2;
+            gy[i - 1] -= al * gz[i];
+            gy[i] += (al - be) * gz[i];
+            gy[i + 1] += be * gz[i];
+        }
+        {
+            const double dx1 = x[L - 2] - x[L - 3], dx2 = x[L - 1] - x[L - 2];
+            const double d = x[L - 1] - x[L - 3];
+            const double al = dx2 * dx2 / (dx1 * d), be = (2 * d + dx2) * dx1 / (dx2 * d);
+            gy[L - 3] -= al * gz[L - 1];
+            gy[L - 2] += (al - be) * gz[L - 1];
+            gy[L - 1] += be * gz[L - 1];
+        }
+    }
+done:
+    free(G); free(gz); free(dl); free(dd); free(du); free(b); free(ud); free(diag); free(ld);
+    free(zero);
+    return first_info;
+}
+
 int orc_max_threads(void)
 {
 #ifdef _OPENMP

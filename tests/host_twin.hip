@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "spline.hip"
+#include "adjoint.hip"
 #include "modesum_direct.hpp"
 #include "few_math.hpp"
 
@@ -144,6 +145,20 @@ int twin_direct_samples(double *out, const long long *idx, long nidx, const doub
         out[2 * q + 1] = w_im;
     }
     return 0;
+}
+
+// The transposed spline build, system by system, on the reference layout
+// g_coef (4, nsys, L) -> g_y (nsys, L).  Returns the first non-zero INFO.
+int twin_spline_adjoint(const double *t, const double *g_coef, int nsys, int L, double *g_y)
+{
+    std::vector<double> w(4 * (size_t)L);
+    int first = 0;
+    for (int s = 0; s < nsys; s++) {
+        int info = few::spline_adjoint_core(t, L, g_coef + (long)s * L, (long)nsys * L, 1,
+                                            g_y + (long)s * L, 1, w.data(), 1);
+        if (info && !first) first = info;
+    }
+    return first;
 }
 
 void twin_sincos_turns(const double *f, double *s, double *c, long n)

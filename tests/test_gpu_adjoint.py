@@ -1,0 +1,229 @@
+"""GPU tests of the reverse mode: the adjoint kernels against the oracle's
+adjoint, the transposed spline build bit for bit, the autograd Function
+against central differences, and a full-size linearity identity."""
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import oracle_coefficients
+
+pytestmark = pytest.mark.gpu
+
+
+def _with_theta(inp, seed):
+    rng = np.random.default_rng(seed)
+    inp.k = rng.integers(-2, 3, inp.K).astype(np.int32)
+    inp.phase_knots = inp.phase_knots.copy()
+    inp.phase_knots[1] = 0.37 * inp.phase_knots[0] + rng.normal(0, 0.1, inp.L)
+    return inp
+
+
+def _cases():
+    from fastemriwaveforms_amd import synthetic
+    out = {
+        "small_theta": _with_theta(synthetic.small_input(seed=11, L=12, K=24), 11),
+        "many_modes": synthetic.make_input(0.003, 10.0, 9, "m_nonneg_30", 700, 41, name="many"),
+        "one_mode": synthetic.make_input(0.002, 10.0, 10, "m_nonneg_30", 1, 77, name="one"),
+    }
+    dense = synthetic.make_input(0.001, 10.0, 40, "m_nonneg_30", 30, 13, name="dense_knots")
+    # knots closer than dt: empty intervals
+    dense.t = np.sort(np.concatenate([dense.t[:30], dense.t[5:15] + 3.0]))[:40]
+    out["dense_knots"] = dense
+    cut = synthetic.make_input(0.003, 10.0, 14, "m_nonneg_30", 40, 78, name="cut")
+    cut.num_pts -= 3000   # the last knots lie past the last sample
+    out["cut"] = cut
+    return out
+
+
+CASES = _cases()
+
+
+def _engine(inp, precision="fp64"):
+    from fastemriwaveforms_amd.autograd import DifferentiableModeSum
+    return DifferentiableModeSum(inp.t, inp.m, inp.k, inp.n, inp.ylms, inp.dt, inp.num_pts,
+                                 precision=precision)
+
+
+def _dev(inp):
+    import torch
+    teuk = torch.from_numpy(np.ascontiguousarray(inp.teuk)).cuda()
+    pk = torch.from_numpy(np.ascontiguousarray(inp.phase_knots)).cuda()
+    return teuk, pk
+
+
+def _grad(inp, seed=3):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=inp.num_pts) + 1j * rng.normal(size=inp.num_pts)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_waveform_adjoint_vs_oracle(gpu, name):
+    import torch
+    inp = CASES[name]
+    print(f"[adjoint] {name}: L={inp.L} K={inp.K} N={inp.num_pts}", flush=True)
+    eng = _engine(inp)
+    teuk, pk = _dev(inp)
+    g = _grad(inp)
+    eng.prepare(torch.view_as_real(teuk), pk)
+    gd = torch.from_numpy(g).cuda()
+    gt = torch.empty((inp.L, inp.K, 2), dtype=torch.float64, device="cuda")
+    gp = torch.empty((3, inp.L), dtype=torch.float64, device="cuda")
+    eng.adjoint(gd, gt, gp)
+    torch.cuda.synchronize()
+    interp_tl, pcoef = oracle_coefficients(inp)
+    gi_ref, gph_ref = oracle.waveform_adjoint(g, interp_tl, inp.t, pcoef, inp.m, inp.k, inp.n,
+                                              inp.L, inp.num_pts, inp.K, inp.ylms, inp.dt, inp.t)
+    gi = eng.g_interp.cpu().numpy()
+    gph = eng.g_phase.cpu().numpy()
+    for p in range(4):  # powers of x span many decades: compare per slot
+        e = oracle.rel_l2(gi[p], gi_ref[p]) if np.any(gi_ref[p]) else float(np.abs(gi[p]).max())
+        assert e < 1e-12, (p, e)
+        e = oracle.rel_l2(gph[p], gph_ref[p]) if np.any(gph_ref[p]) else float(np.abs(gph[p]).max())
+        assert e < 1e-11, (p, e)
+    # knot gradients through the transposed spline builds
+    gy_ref, info = oracle.spline_adjoint(inp.t, np.transpose(gi_ref, (0, 2, 1)))
+    assert info == 0
+    gpk_ref, info = oracle.spline_adjoint(inp.t, np.transpose(gph_ref, (0, 2, 1)))
+    assert info == 0
+    gt_ref = np.stack([gy_ref[:inp.K].T, gy_ref[inp.K:].T], axis=-1)
+    e_t = oracle.rel_l2(gt.cpu().numpy(), gt_ref)
+    e_p = oracle.rel_l2(gp.cpu().numpy(), gpk_ref)
+    print(f"   knot grads rel-L2: amplitudes {e_t:.2e} phases {e_p:.2e}", flush=True)
+    assert e_t < 1e-12 and e_p < 1e-11
+
+
+@pytest.mark.parametrize("L", [4, 5, 13, 120, 200])
+def test_spline_adjoint_bitwise(gpu, L):
+    """few_spline_adjoint_async equals the oracle's transpose bit for bit."""
+    import torch
+    from fastemriwaveforms_amd._lib import check, lib
+    rng = np.random.default_rng(L)
+    nsys = 37
+    t = np.concatenate([[0.0], np.sort(rng.uniform(0, 1, L - 2)), [1.0]]) * 3e7
+    G = np.ascontiguousarray(rng.normal(size=(4, nsys, L)))
+    ref, info = oracle.spline_adjoint(t, G)
+    assert info == 0
+    td, Gd = torch.from_numpy(t).cuda(), torch.from_numpy(G).cuda()
+    gy = torch.empty((nsys, L), dtype=torch.float64, device="cuda")
+    sc = torch.empty(4 * L * nsys, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    check(lib().few_spline_adjoint_async(td.data_ptr(), Gd.data_ptr(), L, nsys * L, 1, nsys, L,
+                                         gy.data_ptr(), L, 1, sc.data_ptr(), stream),
+          "spline_adjoint")
+    torch.cuda.synchronize()
+    assert np.array_equal(gy.cpu().numpy(), ref)
+
+
+def test_autograd_vs_central_differences(gpu):
+    """Gradients through torch.autograd against FP64 central differences of the
+    oracle's forward (knots -> splines -> sum), loss Re<d, h>."""
+    import torch
+    inp = CASES["small_theta"]
+    eng = _engine(inp)
+    teuk, pk = _dev(inp)
+    teuk.requires_grad_(True)
+    pk.requires_grad_(True)
+    h = eng(teuk, pk)
+    rng = np.random.default_rng(8)
+    d_np = _grad(inp, 8)
+    d = torch.from_numpy(d_np).cuda()
+    loss = torch.sum(d.real * h.real + d.imag * h.imag)
+    loss.backward()
+    gt = torch.view_as_real(teuk.grad).cpu().numpy()
+    gp = pk.grad.cpu().numpy()
+    from fastemriwaveforms_amd import synthetic
+
+    def oracle_loss(teuk_np, pk_np):
+        x = synthetic.ModeSumInput(**{**inp.__dict__, "teuk": teuk_np, "phase_knots": pk_np})
+        ic, pcf = oracle_coefficients(x)
+        hh = oracle.get_waveform(ic, x.t, pcf, x.m, x.k, x.n, x.L, x.num_pts, x.K, x.ylms, x.dt,
+                                 x.t)
+        return float(np.sum(d_np.real * hh.real + d_np.imag * hh.imag))
+
+    worst = 0.0
+    amp_scale = float(np.abs(inp.teuk).max())
+    for _ in range(8):
+        # the loss is linear in the amplitudes (no truncation error): the step is
+        # relative to the largest knot amplitude, so rounding stays negligible
+        l, c, part = int(rng.integers(inp.L)), int(rng.integers(inp.K)), int(rng.integers(2))
+        step = 1e-6 * amp_scale
+        tp, tm = inp.teuk.copy(), inp.teuk.copy()
+        tp[l, c] += step * (1j if part else 1.0)
+        tm[l, c] -= step * (1j if part else 1.0)
+        fd = (oracle_loss(tp, inp.phase_knots) - oracle_loss(tm, inp.phase_knots)) / (2 * step)
+        worst = max(worst, abs(gt[l, c, part] - fd) / max(abs(fd), 1e-3 * np.abs(gt).max()))
+    for q in range(3):
+        for l in (0, 5, inp.L - 1):
+            step = 1e-6
+            pp, pm = inp.phase_knots.copy(), inp.phase_knots.copy()
+            pp[q, l] += step
+            pm[q, l] -= step
+            fd = (oracle_loss(inp.teuk, pp) - oracle_loss(inp.teuk, pm)) / (2 * step)
+            worst = max(worst, abs(gp[q, l] - fd) / max(abs(fd), 1e-3 * np.abs(gp[q]).max()))
+    print(f"[autograd] worst relative FD disagreement {worst:.2e}", flush=True)
+    assert worst < 1e-6
+
+
+def test_autograd_mixed_forward_fp64_backward(gpu):
+    """A mixed16 forward keeps the FP64 adjoint: identical gradients to the FP64 engine."""
+    import torch
+    from fastemriwaveforms_amd import synthetic
+    inp = synthetic.make_input(0.003, 10.0, 16, "m_nonneg_30", 200, 5, name="mix")
+    grads = {}
+    for prec in ("fp64", "mixed16"):
+        eng = _engine(inp, prec)
+        teuk, pk = _dev(inp)
+        teuk.requires_grad_(True)
+        pk.requires_grad_(True)
+        h = eng(teuk, pk)
+        (h.real.sum() + 0.5 * h.imag.sum()).backward()
+        grads[prec] = (teuk.grad.clone(), pk.grad.clone(), h.detach().clone())
+    assert torch.equal(grads["fp64"][0], grads["mixed16"][0])
+    assert torch.equal(grads["fp64"][1], grads["mixed16"][1])
+    e = oracle.rel_l2(grads["mixed16"][2].cpu().numpy(), grads["fp64"][2].cpu().numpy())
+    assert e < 1e-6
+
+
+def test_full_size_linearity(gpu):
+    """Full-size (T = 1 yr, 3843 modes, 120 knots): the mode sum is linear in the
+    amplitude coefficients, so Re<g, h> = <c, dL/dc> over every slot; and a
+    phase-knot derivative agrees with central differences of the FP64 GPU
+    forward (itself equal to the oracle to ~1e-16)."""
+    import torch
+    from fastemriwaveforms_amd import synthetic
+    inp = synthetic.make_workload("full1")
+    eng = _engine(inp)
+    teuk, pk = _dev(inp)
+    rng = np.random.default_rng(0)
+    g = torch.from_numpy(_grad(inp, 1)).cuda()
+    wave = torch.zeros(inp.num_pts, dtype=torch.complex128, device="cuda")
+    eng.prepare(torch.view_as_real(teuk), pk)
+    eng.sum_into(wave, "fp64")
+    gt = torch.empty((inp.L, inp.K, 2), dtype=torch.float64, device="cuda")
+    gp = torch.empty((3, inp.L), dtype=torch.float64, device="cuda")
+    eng.adjoint(g, gt, gp)
+    lhs = float(torch.sum(g.real * wave.real + g.imag * wave.imag))
+    terms = eng.interp * eng.g_interp
+    rhs = float(terms.sum())
+    scale = float(terms.abs().sum())
+    print(f"[full] <g,h> {lhs:.12e} <c,dL/dc> {rhs:.12e} rel {abs(lhs - rhs) / scale:.2e}",
+          flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * scale
+
+    def loss(pk_pert):
+        w = torch.zeros_like(wave)
+        eng.prepare(torch.view_as_real(teuk), pk_pert)
+        eng.sum_into(w, "fp64")
+        return float(torch.sum(g.real * w.real + g.imag * w.imag))
+
+    for q, l in ((0, 60), (2, 7), (0, inp.L - 1)):
+        step = 1e-6
+        pp, pm = pk.clone(), pk.clone()
+        pp[q, l] += step
+        pm[q, l] -= step
+        fd = (loss(pp) - loss(pm)) / (2 * step)
+        ad = float(gp[q, l])
+        rel = abs(ad - fd) / max(abs(fd), 1e-3 * float(gp[q].abs().max()))
+        print(f"[full] phase knot ({q},{l}): adjoint {ad:.10e} FD {fd:.10e} rel {rel:.2e}", flush=True)
+        assert rel < 1e-5

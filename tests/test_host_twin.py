@@ -29,7 +29,22 @@ def tw(twin):
     twin.twin_plan.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_int, P, P]
     twin.twin_direct_samples.argtypes = [P, P, ctypes.c_long, P, P, P, P, P, P, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, P, ctypes.c_double, P]
+    twin.twin_spline_adjoint.argtypes = [P, P, ctypes.c_int, ctypes.c_int, P]
     return twin
+
+
+@pytest.mark.parametrize("nsys,length", [(3, 4), (2, 5), (4, 9), (5, 13), (7, 120), (3, 200)])
+def test_twin_spline_adjoint_bitwise(tw, nsys, length):
+    """The kernels' transposed spline build equals the oracle's bit for bit
+    (same operation order, contraction off), including pivoting rows."""
+    rng = np.random.default_rng(100 + length)
+    t = np.concatenate([[0.0], np.sort(rng.uniform(0, 1, length - 2)), [1.0]]) * 3e7
+    G = np.ascontiguousarray(rng.normal(size=(4, nsys, length)))
+    gy_ref, info = oracle.spline_adjoint(t, G)
+    assert info == 0
+    gy = np.zeros((nsys, length))
+    assert tw.twin_spline_adjoint(_p(t), _p(G), nsys, length, _p(gy)) == 0
+    assert np.array_equal(gy, gy_ref)
 
 
 @pytest.mark.parametrize("ninterps,length", [(1, 3), (3, 4), (2, 9), (2, 10), (2, 11), (2, 12), (5, 13), (33, 120), (7, 200)])
