@@ -58,6 +58,9 @@ def _declare(lib: ctypes.CDLL) -> None:
         "few_spline_adjoint_async": ([_d, _d, c_long, c_long, c_long, c_int, c_int, _d, c_long,
                                       c_long, _d, _d], c_int),
         "few_modes_adjoint_async": ([_d, _d, _d, c_int, c_int, _d, _d, _d, _d], c_int),
+        "few_plan_adjoint_workspace_bytes": ([_d, c_int, c_int, c_int], c_size),
+        "few_waveform_adjoint_plan_async": ([_d, _d, _d, _d, _d, _d, _d, _d, c_int, c_int, c_int, _d,
+                                             c_double, _d, _d, _d, c_int, _d, c_size, _d], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -97,4 +100,5 @@ def exported_symbols() -> list[str]:
         "few_get_waveform_plan_async", "few_batch_create", "few_batch_destroy", "few_batch_layout",
         "few_batch_workspace_bytes", "few_batch_run_async", "few_adjoint_workspace_bytes",
         "few_waveform_adjoint_async", "few_spline_adjoint_async", "few_modes_adjoint_async",
+        "few_plan_adjoint_workspace_bytes", "few_waveform_adjoint_plan_async",
     ]

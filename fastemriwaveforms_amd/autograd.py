@@ -14,8 +14,10 @@ the forward's evaluation is stored), then
 2. ``few_modes_adjoint_async`` -- the transposed spline build (transposed
    tridiagonal solve) back to dL/d(knot values), amplitudes and phases.
 
-Both run in FP64 whatever the forward precision, so the gradient is that of
-the FP64 mode sum.  Inputs are real views: amplitudes as (L, K, 2) float64
+Step 1 runs in FP64 (``backward="fp64"``, the exact derivative of the FP64
+sum) or in mixed precision (``backward="mixed16"``: the per-interval moments
+sum_i sigma^s conj(g_i) u_r v_n on split-FP16 MFMA, finished per mode in
+FP64); step 2 is always FP64.  Inputs are real views: amplitudes as (L, K, 2) float64
 (``torch.view_as_real`` of the complex (L, K) knot values), phases as (3, L)
 float64 (Phi_phi, Phi_theta, Phi_r).  The waveform comes back as (N, 2)
 float64; :meth:`DifferentiableModeSum.__call__` wraps it as complex.
@@ -40,8 +42,10 @@ class DifferentiableModeSum:
     """
 
     def __init__(self, t, m, k, n, ylms, dt: float, out_len: int, precision: str = "fp64",
-                 device: torch.device | str = "cuda"):
+                 device: torch.device | str = "cuda", backward: str = "fp64"):
         dev = torch.device(device)
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
         self.device = dev
         f64 = torch.float64
         self.t = torch.as_tensor(np.ascontiguousarray(t, np.float64)).to(dev)
@@ -54,6 +58,9 @@ class DifferentiableModeSum:
         if precision not in PRECISION_CODES:
             raise ValueError(f"precision must be one of {sorted(PRECISION_CODES)}")
         self.precision = precision
+        if backward not in ("fp64", "mixed16"):
+            raise ValueError("backward must be 'fp64' or 'mixed16'")
+        self.backward = backward
         self.variant = 0
         L, K = self.L, self.K
         if L < 4:
@@ -71,7 +78,11 @@ class DifferentiableModeSum:
         self.g_phase = torch.empty((4, L, 3), dtype=f64, device=dev)
         fwd = self.plan.workspace_bytes(L, self.N, precision)
         adj = int(lib().few_adjoint_workspace_bytes(L, self.N, K, _lib.FEW_PRECISION_FP64))
-        self.ws = torch.empty(max(fwd, adj), dtype=torch.uint8, device=dev)
+        adj16 = 0
+        if backward == "mixed16":
+            adj16 = int(lib().few_plan_adjoint_workspace_bytes(self.plan.handle, L, self.N,
+                                                               _lib.FEW_PRECISION_MIXED16))
+        self.ws = torch.empty(max(fwd, adj, adj16), dtype=torch.uint8, device=dev)
 
     # -- device stages -------------------------------------------------------
     def prepare(self, teuk_ri: torch.Tensor, phase_knots: torch.Tensor) -> None:
@@ -94,11 +105,12 @@ class DifferentiableModeSum:
                 grad_phase: torch.Tensor | None) -> None:
         """grad_wave (N,) complex128 -> grad_teuk (L, K, 2), grad_phase (3, L) (or None).
         The splines in self.interp / self.pcoef must be those of the knot values."""
-        rc = lib().few_waveform_adjoint_async(
+        prec = PRECISION_CODES[self.backward]
+        rc = lib().few_waveform_adjoint_plan_async(
             grad_wave.data_ptr(), self.interp.data_ptr(), self.t.data_ptr(),
-            self.pcoef.data_ptr(), self.m.data_ptr(), self.k.data_ptr(), self.n.data_ptr(),
-            self.L, self.N, self.K, self.ylm.data_ptr(), self.dt, self.t.data_ptr(),
-            self.g_interp.data_ptr(), self.g_phase.data_ptr(), _lib.FEW_PRECISION_FP64,
+            self.pcoef.data_ptr(), self.plan.handle, self.m.data_ptr(), self.k.data_ptr(),
+            self.n.data_ptr(), self.L, self.N, self.K, self.ylm.data_ptr(), self.dt,
+            self.t.data_ptr(), self.g_interp.data_ptr(), self.g_phase.data_ptr(), prec,
             self.ws.data_ptr(), self.ws.numel(), _stream())
         check(rc, "waveform_adjoint")
         rc = lib().few_modes_adjoint_async(

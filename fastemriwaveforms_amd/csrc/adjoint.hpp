@@ -19,6 +19,7 @@
 #pragma once
 
 #include "few_common.hpp"
+#include "modesum_fact.hpp"
 
 namespace few {
 
@@ -70,6 +71,43 @@ struct SplineAdjArgs {
 // entries with stride ws between knots.  Returns the solve's INFO.
 __host__ __device__ int spline_adjoint_core(const double *t, int L, const double *g, long gp,
                                             long gk, double *gy, long gyk, double *w, long ws);
+
+// ---- mixed precision (adjoint16.hip): moments on split-FP16 MFMA
+#define A16_MOM 7              // moments sigma^0 .. sigma^6
+#define A16_CHUNK 1024         // samples per chunk (one workgroup)
+
+struct Adj16Args {
+    const double2 *grad;       // (out_len,) dL/dh
+    const double *pcoef;       // (8, L-1, 3)
+    const Segment *segs;       // planner output, tile = chunk
+    const int *prefix;
+    int nseg;
+    double dt;
+    int chunk;
+    FactPlanDev plan;
+    int RBM;                   // 16-row blocks of the moment tile: ceil(2 * 7 * Nr / 16)
+    int NBC;                   // 16-column blocks (waves): ceil(Nc / 16)
+    float *part;               // (max_chunks, RBM*16, NBC*16) chunk moments
+};
+
+struct Adj16FinishArgs {
+    FactPlanDev plan;
+    double *M;                 // (L-1, RBM*16, NBC*16) interval moments (FP64)
+    long tile;                 // RBM*16 * NBC*16
+    int ncol;                  // NBC*16
+    const double *h_t;         // (L,)
+    int L;
+    const double *interp;      // (4, L, 2K)
+    const double2 *ylm;
+    const int *m, *k, *n;
+    double *g_interp;          // (4, L, 2K) out
+    double *g_phase;           // (4, L, 3) out
+    double *part_phase;        // (L, groups, 12)
+};
+
+size_t adjoint16_lds_bytes(int RBM, int NBC, int Nr);
+int launch_adjoint16(const Adj16Args &a, int max_chunks, hipStream_t stream);
+int launch_adjoint16_finish(const Adj16Args &a, const Adj16FinishArgs &f, hipStream_t stream);
 
 size_t adjoint_workspace_bytes(int L, int out_len, int K, int chunk, int groups);
 int launch_adjoint_fp64(const AdjArgs &a, int max_chunks, hipStream_t stream);

@@ -214,6 +214,7 @@ void *few_batch_create(int B, const int *h_L, const int *h_K, const int *h_N, co
     U.row_mk = reinterpret_cast<const int2 *>(base + o_row);
     U.cell_start = nullptr;
     U.entries = nullptr;
+    U.mode_cell = nullptr;
     return bp;
 }
 

@@ -279,6 +279,33 @@ static AdjLayout adj_layout(int L, int out_len, int K)
     return a;
 }
 
+// Mixed-precision adjoint: shapes and workspace.
+struct Adj16Layout {
+    int RBM = 0, NBC = 0, groups = 0;
+    long max_chunks = 0, tile = 0;
+    size_t segs = 0, prefix = 0, part = 0, M = 0, phase = 0, total = 0;
+    bool ok = false;
+};
+static Adj16Layout adj16_layout(const FactPlanHost &h, int L, int out_len)
+{
+    Adj16Layout a;
+    a.RBM = (2 * A16_MOM * h.Nr + 15) / 16;
+    a.NBC = (h.Nc + 15) / 16;
+    a.groups = (h.K + 255) / 256;
+    a.tile = (long)a.RBM * 16 * a.NBC * 16;
+    a.max_chunks = ((long)out_len + A16_CHUNK - 1) / A16_CHUNK + (L > 1 ? L - 1 : 1);
+    size_t off = 0;
+    a.segs = off; off = align256(off + sizeof(Segment) * (size_t)(L > 1 ? L - 1 : 1));
+    a.prefix = off; off = align256(off + sizeof(int) * (size_t)L);
+    a.part = off; off = align256(off + sizeof(float) * (size_t)a.max_chunks * a.tile);
+    a.M = off; off = align256(off + sizeof(double) * (size_t)(L > 1 ? L - 1 : 1) * a.tile);
+    a.phase = off; off = align256(off + sizeof(double) * (size_t)L * a.groups * 12);
+    a.total = off;
+    a.ok = h.ok && a.RBM >= 1 && a.RBM <= 24 && a.NBC >= 1 && a.NBC <= 8 &&
+           adjoint16_lds_bytes(a.RBM, a.NBC, h.Nr) <= 160 * 1024;
+    return a;
+}
+
 }  // namespace few
 
 using namespace few;
@@ -462,6 +489,8 @@ void *few_mode_plan_create(const int *h_m, const int *h_k, const int *h_n, int n
     buf.insert(buf.end(), h.cell_start.begin(), h.cell_start.end());
     const size_t o_ent = buf.size();
     buf.insert(buf.end(), h.entries.begin(), h.entries.end());
+    const size_t o_mc = buf.size();
+    buf.insert(buf.end(), h.mode_cell.begin(), h.mode_cell.end());
     if (buf.empty()) buf.push_back(0);
     pl->dev_bytes = buf.size() * sizeof(int);
     if (hipGetDevice(&pl->device) != hipSuccess ||
@@ -476,6 +505,7 @@ void *few_mode_plan_create(const int *h_m, const int *h_k, const int *h_n, int n
     pl->dev.row_mk = reinterpret_cast<const int2 *>(d);
     pl->dev.cell_start = d + o_cell;
     pl->dev.entries = d + o_ent;
+    pl->dev.mode_cell = d + o_mc;
     return pl;
 }
 
@@ -619,6 +649,106 @@ int few_waveform_adjoint_async(const void *grad_wave, const double *interp_array
     // with no samples the planner leaves every chunk count at zero and the
     // reduction writes zeros
     return launch_adjoint_reduce(r, st);
+}
+
+size_t few_plan_adjoint_workspace_bytes(const void *plan, int init_len, int out_len, int precision)
+{
+    if (init_len < 2 || out_len < 0) return 0;
+    if (precision != FEW_PRECISION_MIXED16 || !plan)
+        return adj_layout(init_len, out_len, plan ? static_cast<const FactPlan *>(plan)->host.K : 0)
+            .total;
+    return adj16_layout(static_cast<const FactPlan *>(plan)->host, init_len, out_len).total;
+}
+
+int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_array,
+                                    const double *phase_interp_t, const double *phase_interp_coeffs,
+                                    const void *plan, const int *d_m, const int *d_k, const int *d_n,
+                                    int init_len, int out_len, int num_teuk_modes, const void *d_Ylms,
+                                    double delta_t, const double *h_t, double *g_interp,
+                                    double *g_phase, int precision, void *workspace,
+                                    size_t workspace_bytes, void *stream)
+{
+    if (precision == FEW_PRECISION_FP64)
+        return few_waveform_adjoint_async(grad_wave, interp_array, phase_interp_t,
+                                          phase_interp_coeffs, d_m, d_k, d_n, init_len, out_len,
+                                          num_teuk_modes, d_Ylms, delta_t, h_t, g_interp, g_phase,
+                                          precision, workspace, workspace_bytes, stream);
+    int rc = check_mode_args(grad_wave, interp_array, phase_interp_coeffs, d_m, d_k, d_n, init_len,
+                             out_len, num_teuk_modes, d_Ylms, delta_t);
+    if (rc) return rc;
+    if (precision != FEW_PRECISION_MIXED16) {
+        set_error("waveform_adjoint: precision %d is not available", precision);
+        return FEW_ERR_UNSUPPORTED;
+    }
+    if (!plan || !phase_interp_t || !h_t || !g_interp || !g_phase || !workspace) {
+        set_error("waveform_adjoint: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (phase_interp_t != h_t) {
+        set_error("waveform_adjoint: the mixed adjoint needs the phase and amplitude knots to be "
+                  "the same array (as InterpolatedModeSum passes them)");
+        return FEW_ERR_UNSUPPORTED;
+    }
+    const FactPlan *pl = static_cast<const FactPlan *>(plan);
+    const FactPlanHost &ph = pl->host;
+    if (ph.K != num_teuk_modes) {
+        set_error("waveform_adjoint: plan built for %d modes, called with %d", ph.K, num_teuk_modes);
+        return FEW_ERR_ARG;
+    }
+    const int L = init_len;
+    const Adj16Layout al = adj16_layout(ph, L, out_len);
+    if (!al.ok) {
+        set_error("waveform_adjoint: mode set too large for the mixed adjoint (%d rows, %d columns)",
+                  ph.Nr, ph.Nc);
+        return FEW_ERR_UNSUPPORTED;
+    }
+    if (workspace_bytes < al.total) {
+        set_error("waveform_adjoint: workspace of %zu bytes, need %zu", workspace_bytes, al.total);
+        return FEW_ERR_ARG;
+    }
+    if (al.max_chunks > 0x7fffffffL) {
+        set_error("waveform_adjoint: output too long (%d samples)", out_len);
+        return FEW_ERR_LIMIT;
+    }
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    char *ws = static_cast<char *>(workspace);
+    Segment *segs = reinterpret_cast<Segment *>(ws + al.segs);
+    int *prefix = reinterpret_cast<int *>(ws + al.prefix);
+    PlanArgs p{h_t, phase_interp_t, L, delta_t, out_len, A16_CHUNK, segs, prefix};
+    rc = launch_plan(p, st);
+    if (rc) return rc;
+    Adj16Args a;
+    a.grad = static_cast<const double2 *>(grad_wave);
+    a.pcoef = phase_interp_coeffs;
+    a.segs = segs;
+    a.prefix = prefix;
+    a.nseg = L - 1;
+    a.dt = delta_t;
+    a.chunk = A16_CHUNK;
+    a.plan = pl->dev;
+    a.RBM = al.RBM;
+    a.NBC = al.NBC;
+    a.part = reinterpret_cast<float *>(ws + al.part);
+    if (out_len > 0) {
+        rc = launch_adjoint16(a, (int)al.max_chunks, st);
+        if (rc) return rc;
+    }
+    Adj16FinishArgs f;
+    f.plan = pl->dev;
+    f.M = reinterpret_cast<double *>(ws + al.M);
+    f.tile = al.tile;
+    f.ncol = al.NBC * 16;
+    f.h_t = h_t;
+    f.L = L;
+    f.interp = interp_array;
+    f.ylm = static_cast<const double2 *>(d_Ylms);
+    f.m = d_m;
+    f.k = d_k;
+    f.n = d_n;
+    f.g_interp = g_interp;
+    f.g_phase = g_phase;
+    f.part_phase = reinterpret_cast<double *>(ws + al.phase);
+    return launch_adjoint16_finish(a, f, st);
 }
 
 int few_spline_adjoint_async(const double *t, const double *g_coef, long g_sys, long g_p,

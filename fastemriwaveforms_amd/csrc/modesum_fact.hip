@@ -77,9 +77,14 @@ bool build_fact_plan(const int *m, const int *k, const int *n, int K, FactPlanHo
     for (int c = 0; c < ncell; c++) p.cell_start[c + 1] = p.cell_start[c] + count[c + 1];
     p.entries.assign(p.cell_start[ncell], 0);
     std::vector<int> fill(p.cell_start.begin(), p.cell_start.end() - 1);
+    p.mode_cell.assign(2 * (size_t)K, -1);
     for (int q = 0; q < K; q++) {
-        p.entries[fill[cell_of(m[q], k[q], n[q])]++] = 2 * q;
-        if (m[q] != 0) p.entries[fill[cell_of(-m[q], -k[q], -n[q])]++] = 2 * q + 1;
+        p.mode_cell[2 * q] = cell_of(m[q], k[q], n[q]);
+        p.entries[fill[p.mode_cell[2 * q]]++] = 2 * q;
+        if (m[q] != 0) {
+            p.mode_cell[2 * q + 1] = cell_of(-m[q], -k[q], -n[q]);
+            p.entries[fill[p.mode_cell[2 * q + 1]]++] = 2 * q + 1;
+        }
     }
     p.rows_arith = 1;
     for (int rr = 0; rr < p.Nr; rr++)

@@ -37,6 +37,7 @@ struct FactPlanHost {
     std::vector<int> row_m, row_k;        // (Nr,)
     std::vector<int> cell_start;          // (Nr*Nc + 1,)
     std::vector<int> entries;             // q*2 + conj, grouped by cell, q ascending
+    std::vector<int> mode_cell;           // (2K,): cell of mode q, cell of its conjugate term (-1: none)
     bool ok = false;
 };
 
@@ -47,6 +48,7 @@ struct FactPlanDev {
     const int2 *row_mk;      // (Nr,)
     const int *cell_start;   // (Nr*Nc + 1,)
     const int *entries;
+    const int *mode_cell;    // (2K,)
 };
 
 struct FactPlan {

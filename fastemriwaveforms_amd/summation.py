@@ -22,7 +22,8 @@ from .backend import ParallelModuleBase
 
 YRSID_SI = 31558149.763545595  # cutils/global.h:23
 
-PRECISIONS = {"fp64": _lib.FEW_PRECISION_FP64}
+PRECISIONS = {"fp64": _lib.FEW_PRECISION_FP64, "mixed": _lib.FEW_PRECISION_MIXED,
+              "mixed16": _lib.FEW_PRECISION_MIXED16}
 
 
 def _device() -> torch.device:
@@ -210,7 +211,11 @@ class InterpolatedModeSum(SummationBase):
     """Interpolated mode summation (summation/interpolatedmodesum.py:260-375).
 
     ``precision`` (constructor or ``sum`` keyword) selects the arithmetic:
-    ``"fp64"`` is the reference's plain FP64 sum.
+    ``"fp64"`` is the reference's plain FP64 sum; ``"mixed"`` (FP64 phases,
+    FP32 MFMA contraction) and ``"mixed16"`` (FP64 phases, split-FP16 MFMA with
+    FP32 accumulation) are the factorised sums (rel-L2 ~1e-7).  The mixed paths
+    need a mode plan, built once per mode set and cached; a mode set whose tile
+    does not fit raises :class:`FewRocmError`.
     """
 
     def __init__(self, force_backend=None, precision: str = "fp64", **kwargs):
@@ -219,6 +224,7 @@ class InterpolatedModeSum(SummationBase):
             raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
         self.precision = precision
         self._ws = None
+        self._plans = {}
 
     @classmethod
     def supported_backends(cls):
@@ -229,6 +235,17 @@ class InterpolatedModeSum(SummationBase):
         if self._ws is None or self._ws.numel() < nbytes:
             self._ws = torch.empty(nbytes, dtype=torch.uint8, device=_device())
         return self._ws
+
+    def _plan(self, m_arr, k_arr, n_arr):
+        from .pipeline import ModePlan
+        host = [np.ascontiguousarray(x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else x,
+                                     np.int32) for x in (m_arr, k_arr, n_arr)]
+        key = b"".join(h.tobytes() for h in host)
+        plan = self._plans.get(key)
+        if plan is None:
+            plan = ModePlan(*host)
+            self._plans = {key: plan}  # keep the most recent mode set only
+        return plan
 
     def sum(self, t, teuk_modes, ylms, phase_interp_t, phase_interp_coeffs, l_arr, m_arr, k_arr,
             n_arr, *args, dt: float = 10.0, integrate_backwards: bool = False,
@@ -259,10 +276,22 @@ class InterpolatedModeSum(SummationBase):
         m = as_device(m_arr, torch.int32)
         k = as_device(k_arr, torch.int32)
         n = as_device(n_arr, torch.int32)
-        ws = self._workspace(L, self.num_pts, K, prec)
-        rc = lib().few_get_waveform_async(
-            self.waveform.data_ptr(), interp.data_ptr(), pt.data_ptr(), pcoef.data_ptr(),
-            m.data_ptr(), k.data_ptr(), n.data_ptr(), L, self.num_pts, K, yl.data_ptr(), float(dt),
-            h_t.data_ptr(), prec, ws.data_ptr(), ws.numel(), _stream())
+        if prec == _lib.FEW_PRECISION_FP64:
+            ws = self._workspace(L, self.num_pts, K, prec)
+            rc = lib().few_get_waveform_async(
+                self.waveform.data_ptr(), interp.data_ptr(), pt.data_ptr(), pcoef.data_ptr(),
+                m.data_ptr(), k.data_ptr(), n.data_ptr(), L, self.num_pts, K, yl.data_ptr(),
+                float(dt), h_t.data_ptr(), prec, ws.data_ptr(), ws.numel(), _stream())
+        else:
+            plan = self._plan(m_arr, k_arr, n_arr)
+            nbytes = int(lib().few_plan_workspace_bytes(plan.handle, L, self.num_pts, prec))
+            if self._ws is None or self._ws.numel() < nbytes:
+                self._ws = torch.empty(nbytes, dtype=torch.uint8, device=_device())
+            ws = self._ws
+            rc = lib().few_get_waveform_plan_async(
+                self.waveform.data_ptr(), interp.data_ptr(), pt.data_ptr(), pcoef.data_ptr(),
+                plan.handle, m.data_ptr(), k.data_ptr(), n.data_ptr(), L, self.num_pts, K,
+                yl.data_ptr(), float(dt), h_t.data_ptr(), prec, 0, 3, ws.data_ptr(), ws.numel(),
+                _stream())
         check(rc, "get_waveform")
         self._keep = (interp, scratch, pcoef, yl, m, k, n, pt, h_t)

@@ -210,6 +210,25 @@ int few_waveform_adjoint_async(const void *grad_wave, const double *interp_array
                                void *workspace, size_t workspace_bytes, void *stream);
 
 /*
+ * The same adjoint with a mode plan (few_mode_plan_create), so it can run in
+ * mixed precision.  precision FEW_PRECISION_FP64 is few_waveform_adjoint_async;
+ * FEW_PRECISION_MIXED16 computes the per-interval moments
+ * sum_i sigma^s conj(g_i) u_r v_n (s = 0..6, u_r v_n the factorised phasors)
+ * on split-FP16 MFMA with FP32 accumulation per 1024-sample chunk, adds the
+ * chunks in FP64 and finishes per mode in FP64.  The mixed path needs
+ * phase_interp_t == h_t (one knot array, as InterpolatedModeSum passes it).
+ * Workspace: few_plan_adjoint_workspace_bytes().
+ */
+size_t few_plan_adjoint_workspace_bytes(const void *plan, int init_len, int out_len, int precision);
+int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_array,
+                                    const double *phase_interp_t, const double *phase_interp_coeffs,
+                                    const void *plan, const int *d_m, const int *d_k, const int *d_n,
+                                    int init_len, int out_len, int num_teuk_modes, const void *d_Ylms,
+                                    double delta_t, const double *h_t, double *g_interp,
+                                    double *g_phase, int precision, void *workspace,
+                                    size_t workspace_bytes, void *stream);
+
+/*
  * Transpose of the spline build (interpolate_arrays) for one group of
  * systems of `length` >= 4 knots sharing t: g_coef[s*g_sys + p*g_p +
  * i*g_knot] = dL/dc_p(i) of system s -> g_y[s*gy_sys + i*gy_knot] = dL/dy_i.

@@ -130,6 +130,26 @@ def test_native_interpolated_mode_sum(gpu):
     assert oracle.rel_l2(h, ref) <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision", ["mixed", "mixed16"])
+def test_native_interpolated_mode_sum_mixed(gpu, precision):
+    """The native class in the factorised precisions: same call, rel-L2 < 1e-6,
+    and the cached mode plan is reused on a second call."""
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.summation import InterpolatedModeSum
+    inp = synthetic.small_input(seed=6, L=20, K=300, T_years=0.004)
+    _, pcoef = oracle_coefficients(inp)
+    dopr = np.transpose(pcoef, (1, 2, 0))
+    ylms = inp.ylms.copy()
+    ylms[inp.K:] *= (-1.0) ** inp.l
+    summ = InterpolatedModeSum(precision=precision)
+    ref = oracle_waveform(inp)
+    for _ in range(2):
+        h = summ(inp.t, inp.teuk, ylms, inp.t, dopr, inp.l, inp.m, inp.k, inp.n,
+                 T=inp.T_years, dt=inp.dt).cpu().numpy()
+        assert oracle.rel_l2(h, ref) < 1e-6
+    assert len(summ._plans) == 1
+
+
 def test_brute_force_small(gpu):
     """Direct numpy sum over modes (directmodesum.py:91-126 pattern) on dense
     spline evaluations: pins the whole FP64 pipeline on a tiny case."""
