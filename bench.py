@@ -53,7 +53,14 @@ def cpu_baseline(inp, seconds_target: float = 12.0):
     t0 = time.perf_counter()
     oracle.waveform_samples(idx, *args, nthreads=1)
     el = time.perf_counter() - t0
+    # the same restatement with OpenMP over samples, all host threads available (<= 16)
+    nth = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+                     else (os.cpu_count() or 1)))
+    t0 = time.perf_counter()
+    oracle.waveform_samples(idx, *args, nthreads=nth)
+    el_omp = time.perf_counter() - t0
     return {"value": nsamp * inp.K / el, "unit": "mode-samples/s", "cores": 1, "kind": "port",
+            "omp": {"value": nsamp * inp.K / el_omp, "threads": nth},
             "sample": f"{nsamp} evenly spread output samples x {inp.K} modes of the same workload "
                       f"({nsamp * inp.K:.3g} mode-samples, {el:.1f} s); oracle/few_oracle.c "
                       "restating interpolate.cu's CPU branch (-O2 -ffp-contract=off), 1 thread"}

@@ -192,8 +192,8 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
                     const int col = 4 * cg4 + t2;
                     const bool in = col < P.Nc;
                     _Float16 hr, lr, hi_, li_;
-                    split16(in ? (float)vr : 0.f, hr, lr);
-                    split16(in ? (float)vi : 0.f, hi_, li_);
+                    split16(in ? f32_opaque((float)vr) : 0.f, hr, lr);
+                    split16(in ? f32_opaque((float)vi) : 0.f, hi_, li_);
                     const int o = col * 32 + 2 * ii;
                     sVh[o] = hr;
                     sVh[o + 1] = hi_;

@@ -8,6 +8,7 @@
 // (pi/4)^19/19! ~ 1e-19), then rotated back by the quadrant.
 #pragma once
 
+#include <hip/hip_runtime.h>
 #include <math.h>
 
 namespace few {
@@ -51,5 +52,16 @@ __host__ __device__ inline void sincos_turns(double f, double &s, double &c)
 
 // Angle in turns reduced to [-1/2, 1/2]: a - rint(a).
 __host__ __device__ inline double reduce_turns(double a) { return a - rint(a); }
+
+// A float the optimiser cannot trace back to the double it was rounded from.
+// Without it, (_Float16)(float)d is folded into one double -> half rounding,
+// which gfx950 has no instruction for (~30 integer instructions each); with
+// it, the conversions are the hardware float -> half ones (round to nearest).
+// The asm statement is empty: it emits no instruction.
+__device__ __forceinline__ float f32_opaque(float x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
 
 }  // namespace few

@@ -107,8 +107,8 @@ __device__ void fact16_prep_block(const FactPlanDev &P, const double *interp, co
         for (int u = 0; u < 2; u++) {
             const int col = tid + 256 * u;
             if (col < ncolc) {
-                const float xr = (float)(cre[u][p] * sigma);
-                const float xi = (float)(cim[u][p] * sigma);
+                const float xr = f32_opaque((float)(cre[u][p] * sigma));
+                const float xi = f32_opaque((float)(cim[u][p] * sigma));
                 const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
                 const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
                 // row 2R:   (Re, -Im);  row 2R+1: (Im, Re)
@@ -143,8 +143,8 @@ __device__ __forceinline__ void split_pack(const double (&vr)[4], const double (
 {
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        const float xr = (float)vr[t];
-        const float xi = (float)vi[t];
+        const float xr = f32_opaque((float)vr[t]);
+        const float xi = f32_opaque((float)vi[t]);
         const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
         bh[2 * t] = hr;
         bh[2 * t + 1] = hi_;

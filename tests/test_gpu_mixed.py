@@ -67,21 +67,19 @@ def test_mixed_full1_vs_fp64(gpu, prec):
     assert err < MIXED_TOL and err_o < MIXED_TOL
 
 
-def test_mixed_explicit_pm_modes(gpu):
+@pytest.mark.parametrize("prec", PRECS)
+def test_mixed_explicit_pm_modes(gpu, prec):
     """Workload mixed2 shape (|m| <= l, n in [-55, 55], Y- = 0) on a short duration.
-    Its FP16 tile (21 rows x 111 columns) exceeds LDS: mixed16 reports unsupported."""
+    Its FP16 tile (21 rows x 111 columns) fits LDS only without row padding."""
     from fastemriwaveforms_amd import synthetic
-    from fastemriwaveforms_amd.exceptions import FewRocmError
     inp = synthetic.make_input(0.05, 5.0, 60, "m_all_55", 2000, 1002, minus_m_ylm=False)
-    h, p = _run(inp, "mixed")
-    assert not p.plan.supported16
-    with pytest.raises(FewRocmError):
-        p.mode_sum("mixed16")
+    h, p = _run(inp, prec)
+    assert p.plan.supported16
     assert p.plan.cols == 111
     idx = spread_indices(inp.num_pts, count=1500)
     ref = oracle_waveform(inp, idx)
     err = oracle.rel_l2(h[idx], ref)
-    print(f"mixed2-shape rel-L2 {err:.3e}")
+    print(f"mixed2-shape {prec} rel-L2 {err:.3e}")
     assert err < MIXED_TOL
 
 
