@@ -14,6 +14,7 @@
 #include <math.h>
 
 #include "adjoint.hpp"
+#include "few_math.hpp"
 #include "modesum_direct.hpp"
 
 namespace few {
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(ADJ_MODES) void adjoint_fp64_kernel(AdjArgs a)
             const double x = s_x[q];
             const double phase = mode_phase(mm, s_pp[q], kk, s_pt[q], nn, s_pr[q]);
             double e_im, e_re;
-            sincos(-phase, &e_im, &e_re);
+            if (!sincos_cw(-phase, e_im, e_re)) sincos(-phase, &e_im, &e_re);
             const double gr = s_gr[q], gi = s_gi[q];
             // q+ = conj(g) e, q- = conj(g) conj(e)
             const double A1 = gr * e_re, A2 = gi * e_im, A3 = gr * e_im, A4 = gi * e_re;

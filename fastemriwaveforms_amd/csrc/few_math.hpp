@@ -1,5 +1,7 @@
-// Small host+device math helpers of the factorised sum.
+// Small host+device math helpers of the mode sums.
 //
+// sincos_cw(x): sin and cos of x in radians with a Cody-Waite reduction
+// (|x| up to ~1e8; the mode phases reach 1e5-1e7 rad).
 // sincos_turns(f): sin and cos of 2*pi*f for f in [-1/2, 1/2] (an angle in
 // turns already reduced in FP64).  The quadrant q = rint(4f) is removed
 // exactly (f - q/4 is exact in binary floating point), leaving
@@ -13,11 +15,11 @@
 
 namespace few {
 
-__host__ __device__ inline void sincos_turns(double f, double &s, double &c)
+// sin and cos of x in [-pi/4, pi/4] (radians), rotated by q quarter turns:
+// Taylor series summed by Horner's rule up to x^17 and x^18 (truncation
+// below (pi/4)^19/19! ~ 1e-19).
+__host__ __device__ inline void sincos_poly(double x, int q, double &s, double &c)
 {
-    const double q = rint(4.0 * f);
-    const double r = f - 0.25 * q;
-    const double x = r * 6.283185307179586476925286766559;  // 2*pi (double)
     const double x2 = x * x;
     // sin x = x (1 - x^2/3! + x^4/5! - ... + x^16/17!)
     double ps = 2.8114572543455207632e-15;            // 1/17!
@@ -42,12 +44,35 @@ __host__ __device__ inline void sincos_turns(double f, double &s, double &c)
     const double cs = 1.0 - x2 * pc;
     // rotate by q quarter turns (branch-free): odd q swaps sin and cos,
     // q = 2, 3 negate sin, q = 1, 2 negate cos
-    const int qi = ((int)q) & 3;
+    const int qi = q & 3;
     const bool swap = qi & 1;
     const double s0 = swap ? cs : sn;
     const double c0 = swap ? sn : cs;
     s = (qi & 2) ? -s0 : s0;
     c = ((qi + 1) & 2) ? -c0 : c0;
+}
+
+__host__ __device__ inline void sincos_turns(double f, double &s, double &c)
+{
+    const double q = rint(4.0 * f);
+    const double r = f - 0.25 * q;
+    sincos_poly(r * 6.283185307179586476925286766559, (int)q, s, c);  // 2*pi (double)
+}
+
+// sin and cos of x (radians) for |x| <= 2^26 pi/2 (~1.05e8): Cody-Waite
+// reduction x = k pi/2 + r with pi/2 = P1 + P2 + P3 (26, 26 and 53
+// significant bits, from pi to 80 digits; |pi/2 - P1 - P2 - P3| < 2e-33), so
+// k P1 and k P2 are exact for |k| <= 2^26 and the fused steps lose nothing;
+// then sincos_poly.  Returns false (outputs untouched) beyond that range.
+__host__ __device__ inline bool sincos_cw(double x, double &s, double &c)
+{
+    const double k = rint(x * 0.63661977236758134308);  // 2/pi
+    if (!(fabs(k) <= 67108864.0)) return false;          // 2^26 (also rejects NaN)
+    double r = fma(-k, 1.5707963109016418457, x);        // P1 = 0x1.921fb5p+0
+    r = fma(-k, 1.5893254712295857e-08, r);              // P2 = 0x1.110b46p-26
+    r = fma(-k, 6.123233995736766e-17, r);               // P3 = 0x1.1a62633145c07p-54
+    sincos_poly(r, (int)(long long)k, s, c);
+    return true;
 }
 
 // Angle in turns reduced to [-1/2, 1/2]: a - rint(a).

@@ -5,6 +5,7 @@
 #include <math.h>
 
 #include "few_common.hpp"
+#include "few_math.hpp"
 
 #ifndef DIRECT_SPT
 #define DIRECT_SPT 2  // output samples per lane in the FP64 direct kernel
@@ -100,7 +101,7 @@ __host__ __device__ inline void mode_term_fp64(const ModeCoef &c, double x, doub
 {
     const double phase = mode_phase(c.m, Pp, c.k, Pt, c.n, Pr);
     double e_im, e_re;
-    sincos(-phase, &e_im, &e_re);
+    if (!sincos_cw(-phase, e_im, e_re)) sincos(-phase, &e_im, &e_re);
     const double a_re = c.ry + c.rc1 * x + c.rc2 * x2 + c.rc3 * x3;
     const double a_im = c.iy + c.ic1 * x + c.ic2 * x2 + c.ic3 * x3;
     const double p_re = e_re * a_re - e_im * a_im;

@@ -161,6 +161,11 @@ int twin_spline_adjoint(const double *t, const double *g_coef, int nsys, int L, 
     return first;
 }
 
+void twin_sincos_cw(const double *x, double *s, double *c, int *ok, long n)
+{
+    for (long i = 0; i < n; i++) ok[i] = sincos_cw(x[i], s[i], c[i]) ? 1 : 0;
+}
+
 void twin_sincos_turns(const double *f, double *s, double *c, long n)
 {
     for (long i = 0; i < n; i++) sincos_turns(reduce_turns(f[i]), s[i], c[i]);
