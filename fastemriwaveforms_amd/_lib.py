@@ -59,6 +59,10 @@ def _declare(lib: ctypes.CDLL) -> None:
                                       c_long, _d, _d], c_int),
         "few_modes_adjoint_async": ([_d, _d, _d, c_int, c_int, _d, _d, _d, _d], c_int),
         "few_plan_adjoint_workspace_bytes": ([_d, c_int, c_int, c_int], c_size),
+        "few_spline_eval_async": ([_d, c_long, c_long, c_int, c_int, _d, c_long, c_long, c_long, _d,
+                                   c_long, c_int, c_int, _d, _d, _d], c_int),
+        "few_direct_mode_sum_async": ([_d, _d, _d, _d, _d, _d, _d, _d, _d, c_int, c_int, _d, _d],
+                                      c_int),
         "few_waveform_adjoint_plan_async": ([_d, _d, _d, _d, _d, _d, _d, _d, c_int, c_int, c_int, _d,
                                              c_double, _d, _d, _d, c_int, _d, c_size, _d], c_int),
     }
@@ -101,4 +105,5 @@ def exported_symbols() -> list[str]:
         "few_batch_workspace_bytes", "few_batch_run_async", "few_adjoint_workspace_bytes",
         "few_waveform_adjoint_async", "few_spline_adjoint_async", "few_modes_adjoint_async",
         "few_plan_adjoint_workspace_bytes", "few_waveform_adjoint_plan_async",
+        "few_spline_eval_async", "few_direct_mode_sum_async",
     ]

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "adjoint.hpp"
+#include "eval.hpp"
 #include "few_common.hpp"
 #include "few_rocm.h"
 #include "modesum_direct.hpp"
@@ -305,6 +306,7 @@ static Adj16Layout adj16_layout(const FactPlanHost &h, int L, int out_len)
            adjoint16_lds_bytes(a.RBM, a.NBC, h.Nr) <= 160 * 1024;
     return a;
 }
+
 
 }  // namespace few
 
@@ -798,6 +800,47 @@ int few_modes_adjoint_async(const double *g_interp, const double *g_phase, const
     a.ns = 2 * K + (g_phase ? 3 : 0);
     a.scratch = scratch;
     return launch_spline_adjoint(a, reinterpret_cast<hipStream_t>(stream));
+}
+
+int few_spline_eval_async(const double *t, long t_sys, long t_knot, int length, int ninterps,
+                          const double *coef, long c_coef, long c_sys, long c_knot,
+                          const double *tnew, long tn_sys, int npts, int deriv_order, double *out,
+                          int *flags, void *stream)
+{
+    if (!t || !coef || !tnew || !out || !flags) {
+        set_error("spline_eval: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (length < 2 || ninterps < 0 || npts < 0) {
+        set_error("spline_eval: invalid sizes (length=%d ninterps=%d npts=%d)", length, ninterps, npts);
+        return FEW_ERR_ARG;
+    }
+    if (deriv_order < 0 || deriv_order > 3) {
+        set_error("deriv_order must be within 0 <= deriv_order <= 3.");
+        return FEW_ERR_ARG;
+    }
+    SplineEvalArgs a{t, t_sys, t_knot, length, ninterps, coef, c_coef, c_sys, c_knot,
+                     tnew, tn_sys, npts, deriv_order, out, flags};
+    return launch_spline_eval(a, reinterpret_cast<hipStream_t>(stream));
+}
+
+int few_direct_mode_sum_async(const void *teuk, const void *ylms, const double *phi_phi,
+                              const double *phi_theta, const double *phi_r, const int *l,
+                              const int *m, const int *k, const int *n, int num_samples,
+                              int num_modes, void *out, void *stream)
+{
+    if (!teuk || !ylms || !phi_phi || !phi_theta || !phi_r || !l || !m || !k || !n || !out) {
+        set_error("direct_mode_sum: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (num_samples < 0 || num_modes < 0) {
+        set_error("direct_mode_sum: invalid sizes (%d samples, %d modes)", num_samples, num_modes);
+        return FEW_ERR_ARG;
+    }
+    DenseSumArgs a{static_cast<const double2 *>(teuk), static_cast<const double2 *>(ylms), phi_phi,
+                   phi_theta, phi_r, l, m, k, n, num_samples, num_modes,
+                   static_cast<double2 *>(out)};
+    return launch_dense_sum(a, reinterpret_cast<hipStream_t>(stream));
 }
 
 int few_release_workspace(void)

@@ -13,6 +13,8 @@ constructors raise.
 
 from __future__ import annotations
 
+import logging
+
 import numpy as np
 import torch
 
@@ -117,48 +119,45 @@ class CubicSplineInterpolant(ParallelModuleBase):
     def c3(self):
         return self._coef(3)
 
-    def _get_inds(self, tnew: torch.Tensor):
-        t = self.t.contiguous()
-        inds = torch.searchsorted(t, tnew.contiguous(), right=True) - 1
-        inds = torch.where(tnew == t[:, -1:], torch.full_like(inds, self.length - 2), inds)
-        bad_left = tnew < t[:, :1]
-        bad_right = tnew > t[:, -1:]
-        return inds, bad_left, bad_right
-
     def __call__(self, tnew, deriv_order: int = 0):
-        """Evaluate the splines (or derivatives 1-3) at tnew (interpolatedmodesum.py:161-257)."""
+        """Evaluate the splines (or derivatives 1-3) at tnew (interpolatedmodesum.py:161-257).
+
+        One HIP launch (binary search + the reference's expressions, bitwise
+        equal to numpy on the same coefficients).  Points outside the knot range
+        get the edge values with a warning for order 0 and raise ValueError for
+        derivatives, as in the reference."""
+        if deriv_order not in (0, 1, 2, 3):
+            raise ValueError("deriv_order must be within 0 <= deriv_order <= 3.")
         tnew = as_device(tnew, torch.float64)
         if tnew.ndim == 2:
             if tnew.shape[0] != self.ninterps:
                 raise ValueError(
                     "If providing a 2D tnew array, must have same number of interpolants as was "
                     "entered during initialization.")
+            tn_sys = tnew.shape[1]
         elif tnew.ndim <= 1:
-            tnew = torch.atleast_1d(tnew).expand(self.ninterps, -1)
-        inds, bad_left, bad_right = self._get_inds(tnew)
-        ic = inds.clamp(0, self.length - 1)
-        t_here = torch.gather(self.t.contiguous(), 1, ic)
-        x = tnew - t_here
-        x2 = x * x
-        x3 = x2 * x
-        y, c1, c2, c3 = (torch.gather(self._coef(p).contiguous(), 1, ic) for p in range(4))
-        if deriv_order == 0:
-            out = y + c1 * x + c2 * x2 + c3 * x3
-            out = torch.where(bad_left, self.y[:, :1].expand_as(out), out)
-            out = torch.where(bad_right, self.y[:, -1:].expand_as(out), out)
+            tnew = torch.atleast_1d(tnew)
+            tn_sys = 0
         else:
-            if bool(bad_left.any()) or bool(bad_right.any()):
+            raise ValueError("tnew must be 1 or 2 dimensions.")
+        npts = tnew.shape[-1]
+        out = torch.empty((self.ninterps, npts), dtype=torch.float64, device=tnew.device)
+        flags = torch.zeros(2, dtype=torch.int32, device=tnew.device)
+        t_sys = 0 if self._t_src.ndim == 1 else self.length
+        rc = lib().few_spline_eval_async(
+            self._t_src.data_ptr(), t_sys, 1, self.length, self.ninterps,
+            self.interp_array.data_ptr(), self.length * self.ninterps, 1, self.ninterps,
+            tnew.data_ptr(), tn_sys, npts, deriv_order, out.data_ptr(), flags.data_ptr(),
+            _stream())
+        check(rc, "spline evaluation")
+        if bool(flags.any()):
+            if deriv_order != 0:
                 raise ValueError(
                     "x points outside of the domain of the spline are not supported when taking "
                     "derivatives.")
-            if deriv_order == 1:
-                out = c1 + 2 * c2 * x + 3 * c3 * x2
-            elif deriv_order == 2:
-                out = 2 * c2 + 6 * c3 * x
-            elif deriv_order == 3:
-                out = 6 * c3
-            else:
-                raise ValueError("deriv_order must be within 0 <= deriv_order <= 3.")
+            logging.getLogger("fastemriwaveforms_amd").warning(
+                "(CubicSplineInterpolant) Warning: New t array outside bounds of input t array. "
+                "These points are filled with edge values.")
         return out.squeeze()
 
 
@@ -295,3 +294,38 @@ class InterpolatedModeSum(SummationBase):
                 _stream())
         check(rc, "get_waveform")
         self._keep = (interp, scratch, pcoef, yl, m, k, n, pt, h_t)
+
+
+class DirectModeSum(SummationBase):
+    """Direct summation over dense per-sample amplitudes and phases
+    (summation/directmodesum.py:6-126), one HIP launch, FP64.
+
+    ``sum(t, teuk_modes, ylms, phase_interp_t, phases_in, l, m, k, n)`` with
+    teuk_modes (len(t), K) complex, ylms (2K,) complex (Y+ then Y-, the
+    (-1)^l of the m < 0 term applied inside, as in the reference) and
+    phases_in = (Phi_phi, Phi_theta, Phi_r) at the samples.
+    """
+
+    @classmethod
+    def supported_backends(cls):
+        return cls.GPU_RECOMMENDED()
+
+    def sum(self, t, teuk_modes, ylms, phase_interp_t, phases_in, l_arr, m_arr, k_arr, n_arr,
+            *args, dt: float = 10.0, integrate_backwards: bool = False, **kwargs):
+        if integrate_backwards:
+            raise NotImplementedError  # as in the reference (directmodesum.py:87-92)
+        Phi_phi, Phi_theta, Phi_r = phases_in
+        teuk = as_device(teuk_modes, torch.complex128)
+        N, K = teuk.shape
+        yl = as_device(ylms, torch.complex128)
+        pp, pt, pr = (as_device(x, torch.float64) for x in (Phi_phi, Phi_theta, Phi_r))
+        ints = [as_device(x, torch.int32) for x in (l_arr, m_arr, k_arr, n_arr)]
+        if yl.numel() < 2 * K or min(x.numel() for x in (pp, pt, pr)) < N:
+            raise ValueError("DirectModeSum: arrays smaller than teuk_modes implies")
+        out = torch.empty(N, dtype=torch.complex128, device=teuk.device)
+        rc = lib().few_direct_mode_sum_async(
+            teuk.data_ptr(), yl.data_ptr(), pp.data_ptr(), pt.data_ptr(), pr.data_ptr(),
+            ints[0].data_ptr(), ints[1].data_ptr(), ints[2].data_ptr(), ints[3].data_ptr(), N, K,
+            out.data_ptr(), _stream())
+        check(rc, "direct_mode_sum")
+        self.waveform = out

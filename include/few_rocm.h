@@ -251,6 +251,38 @@ int few_modes_adjoint_async(const double *g_interp, const double *g_phase, const
                             int length, int num_modes, double *grad_teuk, double *grad_phase_knots,
                             double *scratch, void *stream);
 
+/*
+ * Replaces CubicSplineInterpolant.__call__ (src/few/summation/
+ * interpolatedmodesum.py:139-257; the reference evaluates with numpy/cupy
+ * gathers after a per-array searchsorted loop, :145-149).  All device
+ * pointers: knots t[s*t_sys + j*t_knot] (length >= 2), coefficient p of array
+ * s at knot j coef[p*c_coef + s*c_sys + j*c_knot], points tnew[s*tn_sys + q]
+ * (tn_sys = 0 shares one row), out[s*npts + q].  deriv_order 0..3 with the
+ * reference's expressions (bitwise equal to numpy on the same coefficients);
+ * order 0 fills points outside [t0, t_last] with the edge values.  flags
+ * (device int[2], cleared by the caller) receive 1 when some point lies left
+ * / right of the range: the caller warns (order 0) or raises (derivatives),
+ * as the reference does.
+ */
+int few_spline_eval_async(const double *t, long t_sys, long t_knot, int length, int ninterps,
+                          const double *coef, long c_coef, long c_sys, long c_knot,
+                          const double *tnew, long tn_sys, int npts, int deriv_order, double *out,
+                          int *flags, void *stream);
+
+/*
+ * Replaces DirectModeSum.sum (src/few/summation/directmodesum.py:24-126):
+ * h[i] = sum_q Y+_q A[i,q] e^{-i phi_q(i)} + sum_{q: m_q > 0} (-1)^l_q Y-_q
+ * conj(A[i,q]) e^{+i phi_q(i)}, phi_q = m Phi_phi + k Phi_theta + n Phi_r, on
+ * dense per-sample inputs: teuk (num_samples, num_modes) complex128, ylms
+ * (2 num_modes) complex128 (Y+ then Y-, as passed to the reference), the three
+ * phases (num_samples,), l/m/k/n int32.  Writes out (num_samples) complex128.
+ * FP64 throughout.
+ */
+int few_direct_mode_sum_async(const void *teuk, const void *ylms, const double *phi_phi,
+                              const double *phi_theta, const double *phi_r, const int *l,
+                              const int *m, const int *k, const int *n, int num_samples,
+                              int num_modes, void *out, void *stream);
+
 /* Frees the library-owned workspaces of the synchronous drop-in entries. */
 int few_release_workspace(void);
 
