@@ -138,7 +138,6 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
 #pragma unroll
     for (int rb = 0; rb < RBM; rb++) acc[rb] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    const int nA = A16_MOM * Nr * 16;
     for (int sbase = lo; sbase < hi; sbase += A16_SUPER) {
         __syncthreads();
         for (int q = tid; q < A16_SUPER; q += NT) {
@@ -195,10 +194,9 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
                     split16(in ? f32_opaque((float)vr) : 0.f, hr, lr);
                     split16(in ? f32_opaque((float)vi) : 0.f, hi_, li_);
                     const int o = col * 32 + 2 * ii;
-                    sVh[o] = hr;
-                    sVh[o + 1] = hi_;
-                    sVl[o] = lr;
-                    sVl[o + 1] = li_;
+                    typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+                    *reinterpret_cast<h2v *>(sVh + o) = h2v{hr, hi_};
+                    *reinterpret_cast<h2v *>(sVl + o) = h2v{lr, li_};
                     const double nr = vr * zr - vi * zi;
                     const double ni = vr * zi + vi * zr;
                     vr = nr;
@@ -206,21 +204,28 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
                 }
             }
             __syncthreads();
-            // A tile: complex row R = 7 r + s, sample ii: a = w_s u_r
-            for (int e = tid; e < nA; e += NT) {
-                const int R = e >> 4, ii = e & 15;
-                const int r = R / A16_MOM, s = R - A16_MOM * r;
-                const float2 w = sW[s * 16 + ii];
-                const float2 u = sU[r * 16 + ii];
-                const float ar = w.x * u.x - w.y * u.y;
-                const float ai = w.x * u.y + w.y * u.x;
-                _Float16 hr, lr, hi_, li_;
-                split16(ar, hr, lr);
-                split16(ai, hi_, li_);
-                const int o0 = (2 * R) * A16_AST + 2 * ii;
-                const int o1 = o0 + A16_AST;
-                sAh[o0] = hr;  sAh[o0 + 1] = -hi_; sAh[o1] = hi_; sAh[o1 + 1] = hr;
-                sAl[o0] = lr;  sAl[o0 + 1] = -li_; sAl[o1] = li_; sAl[o1 + 1] = lr;
+            // A tile: complex row R = 7 r + s, sample ii: a = w_s u_r.  A thread
+            // keeps its sample ii and walks the rows; each (re, im) pair of a
+            // row is one 32-bit LDS store.
+            {
+                const int ii = tid & 15;
+                for (int R = tid >> 4; R < A16_MOM * Nr; R += NT >> 4) {
+                    const int r = R / A16_MOM, s = R - A16_MOM * r;
+                    const float2 w = sW[s * 16 + ii];
+                    const float2 u = sU[r * 16 + ii];
+                    const float ar = w.x * u.x - w.y * u.y;
+                    const float ai = w.x * u.y + w.y * u.x;
+                    _Float16 hr, lr, hi_, li_;
+                    split16(ar, hr, lr);
+                    split16(ai, hi_, li_);
+                    const int o0 = (2 * R) * A16_AST + 2 * ii;
+                    const int o1 = o0 + A16_AST;
+                    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+                    *reinterpret_cast<h2 *>(sAh + o0) = h2{hr, -hi_};
+                    *reinterpret_cast<h2 *>(sAh + o1) = h2{hi_, hr};
+                    *reinterpret_cast<h2 *>(sAl + o0) = h2{lr, -li_};
+                    *reinterpret_cast<h2 *>(sAl + o1) = h2{li_, lr};
+                }
             }
             __syncthreads();
             const v8h bh = *reinterpret_cast<const v8h *>(sVh + (16 * wv + li) * 32 + 8 * g);
