@@ -254,12 +254,18 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
     ad, fd = np.array(ad), np.array(fd)
     per = np.abs(ad - fd) / np.maximum(np.abs(fd), 1e-300)
     n_amp = fd_coords * 3 // 4
+    # per coordinate, relative to the scale of its gradient array (a single knot can
+    # have a derivative far below that scale, where cancellation leaves |FD| tiny)
+    scale = np.concatenate([np.full(n_amp, float(g_t.abs().max())),
+                            np.full(fd_coords - n_amp, float(g_p.abs().max()))])
+    per_scale = np.abs(ad - fd) / scale
     out["fd_check"] = {"coords": fd_coords, "amplitude_coords": fd_coords * 3 // 4,
                        "phase_coords": fd_coords - fd_coords * 3 // 4,
                        "rel_l2": float(np.linalg.norm(ad - fd) / np.linalg.norm(fd)),
                        "max_rel": float(per.max()), "median_rel": float(np.median(per)),
                        "max_rel_amplitude": float(per[:n_amp].max()),
                        "max_rel_phase": float(per[n_amp:].max()),
+                       "max_rel_to_gradient_scale": float(per_scale.max()),
                        "steps": "amplitudes: central difference, step 1e-6 max|A| (the loss is "
                                 "linear in them); phases: 5-point stencil, step 1e-2 rad (FP64 "
                                 "rounding of 1e5-rad phases floors a 1e-6 rad difference at ~1e-3)",

@@ -236,9 +236,14 @@ def test_full_size_linearity(gpu):
 
         fd = (-at(2 * step) + 8 * at(step) - 8 * at(-step) + at(-2 * step)) / (12 * step)
         ad = float(gp[q, l])
-        rel = abs(ad - fd) / max(abs(fd), 1e-3 * float(gp[q].abs().max()))
-        print(f"[full] phase knot ({q},{l}): adjoint {ad:.10e} FD {fd:.10e} rel {rel:.2e}", flush=True)
-        assert rel < 2e-5
+        # measured against the scale of the phase-knot gradient: single knots can
+        # have derivatives far below it (contributions of opposite sign cancel),
+        # where the stencil's truncation on the large terms dominates
+        scale = float(gp[q].abs().max())
+        rel = abs(ad - fd) / scale
+        print(f"[full] phase knot ({q},{l}): adjoint {ad:.10e} FD {fd:.10e} "
+              f"|diff|/max|grad| {rel:.2e} (|diff|/|FD| {abs(ad - fd) / abs(fd):.2e})", flush=True)
+        assert rel < 1e-6
 
 
 def _adjoint_grads(inp, backward, g):
