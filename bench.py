@@ -28,6 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6, "fp32_mfma": 157.3, "fp16_mfma": 2500.0}  # MI355X TFLOP/s (dense)
+# HBM bytes per launch of fact16_kernel on full1, from two rocprofv3 PMC runs
+# (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a 1 GiB copy in the same
+# process): profiles/fact16_full1_hbm_traffic.md
+FACT16_FULL1_TRAFFIC = 2.00 * 44899.2 * 1024 + 1.00 * 56937.4 * 1024
 DIRECT_FLOP = 37  # algorithmic FLOP per mode-sample of the direct sum (sincos counted 0)
 
 
@@ -401,8 +405,11 @@ def main():
                         "kernel": "fact_f32_kernel", "algorithmic": alg}
         else:
             peak = PEAK["fp16_mfma"] / 3.0  # three FP16 products per FP32-accurate product
+            traffic = FACT16_FULL1_TRAFFIC if args.workload == "full1" else None
             roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                        "frac": achieved / peak, "traffic": None, "kernel": "fact16_kernel",
+                        "frac": achieved / peak, "traffic": traffic, "kernel": "fact16_kernel",
+                        "traffic_source": "profiles/fact16_full1_hbm_traffic.md (rocprofv3 PMC)"
+                        if traffic else None,
                         "algorithmic": alg,
                         "peak_basis": "FP16 dense MFMA 2500 TFLOP/s / 3 split products",
                         "frac_of_fp32_mfma_peak": achieved / PEAK["fp32_mfma"]}
