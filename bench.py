@@ -232,6 +232,12 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
         return float(torch.sum(d.real * wave.real + d.imag * wave.imag))
 
     g_t, g_p = grads["fp64"]
+    ga = float(torch.linalg.vector_norm(grads["mixed16"][0] - g_t) / torch.linalg.vector_norm(g_t))
+    gph = float(torch.linalg.vector_norm(grads["mixed16"][1] - g_p) / torch.linalg.vector_norm(g_p))
+    out["mixed16"]["grad_rel_l2_vs_fp64"] = {"amplitude_knots": ga, "phase_knots": gph}
+    log(f"[bench] adjoint mixed16 gradient rel-L2 vs fp64: amplitudes {ga:.2e}, phases {gph:.2e}")
+    if fd_coords <= 0:
+        return out
     base_t = torch.view_as_real(teuk).contiguous()
     amp_scale = float(torch.abs(teuk).max())
     ad, fd = [], []
@@ -277,13 +283,11 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
                                 "stencils at 1e-3 and 5e-4 rad (FP64 rounding of 1e5-1e7 rad "
                                 "phases floors a 1e-6 rad central difference at ~1e-3)",
                        "function": "FP64 GPU forward (rel-L2 ~4e-16 vs the oracle)"}
-    ga = float(torch.linalg.vector_norm(grads["mixed16"][0] - g_t) / torch.linalg.vector_norm(g_t))
-    gph = float(torch.linalg.vector_norm(grads["mixed16"][1] - g_p) / torch.linalg.vector_norm(g_p))
-    out["mixed16"]["grad_rel_l2_vs_fp64"] = {"amplitude_knots": ga, "phase_knots": gph}
-    log(f"[bench] adjoint mixed16 gradient rel-L2 vs fp64: amplitudes {ga:.2e}, phases {gph:.2e}")
     log(f"[bench] adjoint FD check ({fd_coords} coords): rel-L2 {out['fd_check']['rel_l2']:.2e}, "
         f"max per-coordinate {per.max():.2e}, median {np.median(per):.2e}")
     # gradient throughput: 16 full-size waveforms (own knots and amplitudes), fwd + bwd each
+    if batch <= 0:
+        return out
     batch_inputs = [synthetic.make_workload("full1", seed=5000 + b) for b in range(batch)]
     for prec in ("fp64", "mixed16"):
         eng = engines[prec]
@@ -332,6 +336,10 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiation leg (forward + backward, FD check)")
+    ap.add_argument("--fd-coords", type=int, default=64,
+                    help="coordinates of the finite-difference check (0 skips it)")
+    ap.add_argument("--grad-batch", type=int, default=16,
+                    help="waveforms of the gradient-throughput measurement (0 skips it)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-traffic "
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -427,7 +435,7 @@ def main():
 
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_leg(dev)
+        adjoint = adjoint_leg(dev, batch=args.grad_batch, fd_coords=args.fd_coords)
 
     cpu = None
     if not args.no_cpu:
