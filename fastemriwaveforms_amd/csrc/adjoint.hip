@@ -22,104 +22,24 @@ namespace few {
 #define ADJ_MODES 128   // lanes (modes) per workgroup
 #define ADJ_SUB 128     // samples staged in LDS per step
 
-// LAPACK dgtsv (NRHS = 1) on vectors with stride ws: Gaussian elimination
-// with partial pivoting, rounding exactly as written (contraction off).
-__host__ __device__ static int dgtsv_strided(int n, double *dl, double *d, double *du, double *b,
-                                             long ws)
+// Transposed spline build, in two parts.  Every system of one call shares
+// the knots, hence the matrix A^T of the not-a-knot system; LAPACK dgtsv's
+// pivot choices and multipliers depend on the matrix only, so the
+// elimination is done once (tri_factor_transposed) and each system then
+// sweeps only its right-hand side with the recorded multipliers
+// (spline_adjoint_rhs).  Every operation is the one dgtsv / the transposed
+// build performs, in the same order, with contraction off: the result equals
+// the one-system-at-a-time transpose bit for bit.
+//
+// F (7 x L doubles): [0] pivot flag of elimination step i, [1] its
+// multiplier, [2] d, [3] du, [4] dl (second super-diagonal fill-in) after
+// elimination, [5] al and [6] be, the weights of row i of fill_B transposed.
+__host__ __device__ int tri_factor_transposed(const double *x, int L, double *F)
 {
 #pragma clang fp contract(off)
-#define DL(i) dl[(long)(i) * ws]
-#define DD(i) d[(long)(i) * ws]
-#define DU(i) du[(long)(i) * ws]
-#define BB(i) b[(long)(i) * ws]
-    double fact, temp;
-    int i;
-    for (i = 0; i < n - 2; i++) {
-        if (fabs(DD(i)) >= fabs(DL(i))) {
-            if (DD(i) != 0.0) {
-                fact = DL(i) / DD(i);
-                DD(i + 1) = DD(i + 1) - fact * DU(i);
-                BB(i + 1) = BB(i + 1) - fact * BB(i);
-            } else {
-                return i + 1;
-            }
-            DL(i) = 0.0;
-        } else {
-            fact = DD(i) / DL(i);
-            DD(i) = DL(i);
-            temp = DD(i + 1);
-            DD(i + 1) = DU(i) - fact * temp;
-            DL(i) = DU(i + 1);
-            DU(i + 1) = -fact * DL(i);
-            DU(i) = temp;
-            temp = BB(i);
-            BB(i) = BB(i + 1);
-            BB(i + 1) = temp - fact * BB(i + 1);
-        }
-    }
-    if (n > 1) {
-        i = n - 2;
-        if (fabs(DD(i)) >= fabs(DL(i))) {
-            if (DD(i) != 0.0) {
-                fact = DL(i) / DD(i);
-                DD(i + 1) = DD(i + 1) - fact * DU(i);
-                BB(i + 1) = BB(i + 1) - fact * BB(i);
-            } else {
-                return i + 1;
-            }
-        } else {
-            fact = DD(i) / DL(i);
-            DD(i) = DL(i);
-            temp = DD(i + 1);
-            DD(i + 1) = DU(i) - fact * temp;
-            DU(i) = temp;
-            temp = BB(i);
-            BB(i) = BB(i + 1);
-            BB(i + 1) = temp - fact * BB(i + 1);
-        }
-    }
-    if (DD(n - 1) == 0.0) return n;
-    BB(n - 1) = BB(n - 1) / DD(n - 1);
-    if (n > 1) BB(n - 2) = (BB(n - 2) - DU(n - 2) * BB(n - 1)) / DD(n - 2);
-    for (i = n - 3; i >= 0; i--) BB(i) = (BB(i) - DU(i) * BB(i + 1) - DL(i) * BB(i + 2)) / DD(i);
-    return 0;
-#undef DL
-#undef DD
-#undef DU
-#undef BB
-}
-
-__host__ __device__ int spline_adjoint_core(const double *x, int L, const double *g, long gp,
-                                            long gk, double *gy, long gyk, double *w, long ws)
-{
-#pragma clang fp contract(off)
-#define G(p, i) g[(long)(p) * gp + (long)(i) * gk]
-#define GY(i) gy[(long)(i) * gyk]
-    double *gz = w, *dl = w + (long)L * ws, *dd = w + 2L * L * ws, *du = w + 3L * L * ws;
-#define GZ(i) gz[(long)(i) * ws]
-    // final derivatives (interpolate.cu:242-280) fold into interval L-2
-    const double h = x[L - 1] - x[L - 2];
-    const double f1 = G(1, L - 1), f2 = G(2, L - 1), f3 = G(3, L - 1);
-    // fill_coefficients (interpolate.cu:28-43) transposed, interval by interval
-    for (int i = 0; i < L; i++) {
-        GY(i) = G(0, i);
-        GZ(i) = 0.0;
-    }
-    for (int i = 0; i < L - 1; i++) {
-        const double dx = x[i + 1] - x[i];
-        double g1 = G(1, i), g2 = G(2, i), g3 = G(3, i);
-        if (i == L - 2) {
-            g1 += f1;
-            g2 += 2.0 * h * f1 + f2;
-            g3 += 3.0 * h * h * f1 + 3.0 * h * f2 + f3;
-        }
-        const double gs = 3.0 * g2 / dx - 2.0 * g3 / (dx * dx);
-        GZ(i) += g1 - 2.0 * g2 / dx + g3 / (dx * dx);
-        GZ(i + 1) += -g2 / dx + g3 / (dx * dx);
-        GY(i) -= gs / dx;
-        GY(i + 1) += gs / dx;
-    }
-    // A^T (A from prep_splines, interpolate.cu:49-102): sub-diagonal
+    double *piv = F, *fact = F + L, *d = F + 2L * L, *du = F + 3L * L, *dl = F + 4L * L;
+    double *al = F + 5L * L, *be = F + 6L * L;
+    // A^T of prep_splines' matrix (interpolate.cu:49-102): sub-diagonal
     // A[i][i+1], super-diagonal A[i+1][i]
     for (int i = 0; i < L; i++) {
         double sub = 0.0, sup = 0.0, dg;
@@ -133,51 +53,175 @@ __host__ __device__ int spline_adjoint_core(const double *x, int L, const double
             sub = x[i] - x[i - 1];
         }
         if (i < L - 1) sup = (i + 1 == L - 1) ? x[L - 1] - x[L - 3] : x[i + 2] - x[i + 1];
-        dd[(long)i * ws] = dg;
-        if (i < L - 1) {
-            dl[(long)i * ws] = sub;
-            du[(long)i * ws] = sup;
+        d[i] = dg;
+        dl[i] = sub;
+        du[i] = sup;
+    }
+    const int n = L;
+    double f, temp;
+    int i;
+    for (i = 0; i < n - 2; i++) {
+        if (fabs(d[i]) >= fabs(dl[i])) {
+            if (d[i] == 0.0) return i + 1;
+            f = dl[i] / d[i];
+            d[i + 1] = d[i + 1] - f * du[i];
+            dl[i] = 0.0;
+            piv[i] = 0.0;
+        } else {
+            f = d[i] / dl[i];
+            d[i] = dl[i];
+            temp = d[i + 1];
+            d[i + 1] = du[i] - f * temp;
+            dl[i] = du[i + 1];
+            du[i + 1] = -f * dl[i];
+            du[i] = temp;
+            piv[i] = 1.0;
         }
+        fact[i] = f;
     }
-    const int info = dgtsv_strided(L, dl, dd, du, gz, ws);
-    // fill_B (interpolate.cu:107-145) transposed
+    i = n - 2;
+    if (fabs(d[i]) >= fabs(dl[i])) {
+        if (d[i] == 0.0) return i + 1;
+        f = dl[i] / d[i];
+        d[i + 1] = d[i + 1] - f * du[i];
+        piv[i] = 0.0;
+    } else {
+        f = d[i] / dl[i];
+        d[i] = dl[i];
+        temp = d[i + 1];
+        d[i + 1] = du[i] - f * temp;
+        du[i] = temp;
+        piv[i] = 1.0;
+    }
+    fact[i] = f;
+    if (d[n - 1] == 0.0) return n;
+    // fill_B (interpolate.cu:107-145): weights of row i on y[i-1] (-al), y[i] (al - be), y[i+1] (be)
     {
-        const double dx1 = x[1] - x[0], dx2 = x[2] - x[1], d = x[2] - x[0];
-        const double al = (dx1 + 2 * d) * dx2 / (dx1 * d), be = dx1 * dx1 / (dx2 * d);
-        GY(0) -= al * GZ(0);
-        GY(1) += (al - be) * GZ(0);
-        GY(2) += be * GZ(0);
+        const double dx1 = x[1] - x[0], dx2 = x[2] - x[1], dd = x[2] - x[0];
+        al[0] = (dx1 + 2 * dd) * dx2 / (dx1 * dd);
+        be[0] = dx1 * dx1 / (dx2 * dd);
     }
-    for (int i = 1; i < L - 1; i++) {
-        const double dx1 = x[i] - x[i - 1], dx2 = x[i + 1] - x[i];
-        const double al = 3.0 * dx2 / dx1, be = 3.0 * dx1 / dx2;
-        GY(i - 1) -= al * GZ(i);
-        GY(i) += (al - be) * GZ(i);
-        GY(i + 1) += be * GZ(i);
+    for (int r = 1; r < L - 1; r++) {
+        const double dx1 = x[r] - x[r - 1], dx2 = x[r + 1] - x[r];
+        al[r] = 3.0 * dx2 / dx1;
+        be[r] = 3.0 * dx1 / dx2;
     }
     {
         const double dx1 = x[L - 2] - x[L - 3], dx2 = x[L - 1] - x[L - 2];
-        const double d = x[L - 1] - x[L - 3];
-        const double al = dx2 * dx2 / (dx1 * d), be = (2 * d + dx2) * dx1 / (dx2 * d);
-        GY(L - 3) -= al * GZ(L - 1);
-        GY(L - 2) += (al - be) * GZ(L - 1);
-        GY(L - 1) += be * GZ(L - 1);
+        const double dd = x[L - 1] - x[L - 3];
+        al[L - 1] = dx2 * dx2 / (dx1 * dd);
+        be[L - 1] = (2 * dd + dx2) * dx1 / (dx2 * dd);
     }
-    return info;
+    return 0;
+}
+
+// One system: gradient G(p, i) = g[p*gp + i*gk] of its (4, L) coefficients ->
+// dL/dy(i) = gy[i*gyk].  s: L scratch doubles with stride ws.
+__host__ __device__ void spline_adjoint_rhs(const double *x, int L, const double *F,
+                                            const double *g, long gp, long gk, double *gy,
+                                            long gyk, double *s, long ws)
+{
+#pragma clang fp contract(off)
+#define G(p, i) g[(long)(p) * gp + (long)(i) * gk]
+#define GY(i) gy[(long)(i) * gyk]
+#define S(i) s[(long)(i) * ws]
+    const double *piv = F, *fact = F + L, *d = F + 2L * L, *du = F + 3L * L, *dl = F + 4L * L;
+    const double *al = F + 5L * L, *be = F + 6L * L;
+    // final derivatives (interpolate.cu:242-280) fold into interval L-2;
+    // fill_coefficients (interpolate.cu:28-43) transposed, interval by
+    // interval: z = dL/d(slopes) into s, the direct y terms into gy
+    const double h = x[L - 1] - x[L - 2];
+    const double f1 = G(1, L - 1), f2 = G(2, L - 1), f3 = G(3, L - 1);
+    double pend_z = 0.0, q_prev = 0.0;
+    for (int i = 0; i < L - 1; i++) {
+        const double dx = x[i + 1] - x[i];
+        double g1 = G(1, i), g2 = G(2, i), g3 = G(3, i);
+        if (i == L - 2) {
+            g1 += f1;
+            g2 += 2.0 * h * f1 + f2;
+            g3 += 3.0 * h * h * f1 + 3.0 * h * f2 + f3;
+        }
+        const double gs = 3.0 * g2 / dx - 2.0 * g3 / (dx * dx);
+        S(i) = pend_z + (g1 - 2.0 * g2 / dx + g3 / (dx * dx));
+        pend_z = 0.0 + (-g2 / dx + g3 / (dx * dx));
+        const double q = gs / dx;
+        GY(i) = (i > 0 ? G(0, i) + q_prev : G(0, i)) - q;
+        q_prev = q;
+    }
+    S(L - 1) = pend_z;
+    GY(L - 1) = G(0, L - 1) + q_prev;
+    // the solve A^T lambda = z: forward sweep with the recorded multipliers
+    double cur = S(0);
+    for (int i = 0; i < L - 1; i++) {
+        const double nxt = S(i + 1);
+        if (piv[i] == 0.0) {
+            S(i) = cur;
+            cur = nxt - fact[i] * cur;
+        } else {
+            S(i) = nxt;
+            cur = cur - fact[i] * nxt;
+        }
+    }
+    // back substitution
+    double b1 = cur / d[L - 1];
+    S(L - 1) = b1;
+    double b0 = (S(L - 2) - du[L - 2] * b1) / d[L - 2];
+    S(L - 2) = b0;
+    for (int i = L - 3; i >= 0; i--) {
+        const double bi = (S(i) - du[i] * b0 - dl[i] * b1) / d[i];
+        S(i) = bi;
+        b1 = b0;
+        b0 = bi;
+    }
+    // fill_B (interpolate.cu:107-145) transposed, rows in ascending order:
+    // row r adds to gy[r-1], gy[r], gy[r+1]; a three-value window
+    double y0 = GY(0), y1 = GY(1), y2 = GY(2);
+    double lam = S(0);
+    y0 -= al[0] * lam;
+    y1 += (al[0] - be[0]) * lam;
+    y2 += be[0] * lam;
+    for (int r = 1; r <= L - 2; r++) {
+        lam = S(r);
+        y0 -= al[r] * lam;
+        y1 += (al[r] - be[r]) * lam;
+        y2 += be[r] * lam;
+        if (r < L - 2) {
+            GY(r - 1) = y0;
+            y0 = y1;
+            y1 = y2;
+            y2 = GY(r + 2);
+        }
+    }
+    lam = S(L - 1);
+    y0 -= al[L - 1] * lam;
+    y1 += (al[L - 1] - be[L - 1]) * lam;
+    y2 += be[L - 1] * lam;
+    GY(L - 3) = y0;
+    GY(L - 2) = y1;
+    GY(L - 1) = y2;
 #undef G
 #undef GY
-#undef GZ
+#undef S
+}
+
+__global__ __launch_bounds__(64) void tri_factor_kernel(const double *t, int L, double *F,
+                                                        int *info)
+{
+    if (threadIdx.x == 0) {
+        const int r = tri_factor_transposed(t, L, F);
+        if (info) *info = r;
+    }
 }
 
 __global__ __launch_bounds__(64) void spline_adjoint_kernel(SplineAdjArgs a)
 {
     long s = (long)blockIdx.x * 64 + threadIdx.x;
+    const long col = s;
     for (int gi = 0; gi < a.ngroups; gi++) {
         const SplineAdjGroup &G = a.grp[gi];
         if (s < G.nsys) {
-            const long col = (long)blockIdx.x * 64 + threadIdx.x;
-            spline_adjoint_core(a.t, a.L, G.g + s * G.g_sys, G.g_p, G.g_knot, G.gy + s * G.gy_sys,
-                                G.gy_knot, a.scratch + col, a.ns);
+            spline_adjoint_rhs(a.t, a.L, a.factor, G.g + s * G.g_sys, G.g_p, G.g_knot,
+                               G.gy + s * G.gy_sys, G.gy_knot, a.scratch + col, a.ns);
             return;
         }
         s -= G.nsys;
@@ -392,6 +436,9 @@ int launch_spline_adjoint(const SplineAdjArgs &a, hipStream_t stream)
         set_error("spline_adjoint: %ld systems exceed %ld scratch columns", total, a.ns);
         return FEW_ERR_ARG;
     }
+    hipLaunchKernelGGL(tri_factor_kernel, dim3(1), dim3(64), 0, stream, a.t, a.L, a.factor,
+                       nullptr);
+    FEW_HIP(hipGetLastError());
     hipLaunchKernelGGL(spline_adjoint_kernel, dim3((unsigned)((total + 63) / 64)), dim3(64), 0,
                        stream, a);
     FEW_HIP(hipGetLastError());

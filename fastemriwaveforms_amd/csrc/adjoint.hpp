@@ -64,13 +64,18 @@ struct SplineAdjArgs {
     int ngroups;
     int L;
     long ns;                  // scratch columns (total systems)
-    double *scratch;          // 4 * L * ns doubles
+    double *scratch;          // L * ns doubles: one right-hand side per system
+    double *factor;           // 7 * L doubles: the shared elimination (tri_factor_transposed)
 };
 
-// The transpose of one system's build; w holds 4 scratch vectors of L
-// entries with stride ws between knots.  Returns the solve's INFO.
-__host__ __device__ int spline_adjoint_core(const double *t, int L, const double *g, long gp,
-                                            long gk, double *gy, long gyk, double *w, long ws);
+// Elimination of the transposed not-a-knot matrix of knots x (shared by all
+// systems on those knots) into F (7 x L doubles); returns dgtsv's INFO.
+__host__ __device__ int tri_factor_transposed(const double *x, int L, double *F);
+// The transpose of one system's build with the shared elimination F;
+// s holds L scratch doubles with stride ws.
+__host__ __device__ void spline_adjoint_rhs(const double *x, int L, const double *F,
+                                            const double *g, long gp, long gk, double *gy,
+                                            long gyk, double *s, long ws);
 
 // ---- mixed precision (adjoint16.hip): moments on split-FP16 MFMA
 #define A16_MOM 7              // moments sigma^0 .. sigma^6

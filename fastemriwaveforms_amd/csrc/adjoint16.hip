@@ -38,6 +38,8 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 #define A16_SUPER 256          // samples whose phases are staged at a time
 #define A16_AST 40             // A-tile row stride in halfs (32 + 8 padding)
+#define A16_VST 40             // v-table column stride in halfs (32 + 8 padding: the 16 lanes of a
+                               // fragment read start on distinct 4-bank groups)
 #define INV_2PI_A 0.15915494309189533577
 
 struct A16Layout {
@@ -52,8 +54,8 @@ __host__ __device__ inline A16Layout a16_layout(int RBM, int NBC, int Nr)
     size_t o = 0;
     L.a_hi = o; o = a16_align(o + (size_t)RBM * 16 * A16_AST * 2);
     L.a_lo = o; o = a16_align(o + (size_t)RBM * 16 * A16_AST * 2);
-    L.v_hi = o; o = a16_align(o + (size_t)NBC * 16 * 32 * 2);
-    L.v_lo = o; o = a16_align(o + (size_t)NBC * 16 * 32 * 2);
+    L.v_hi = o; o = a16_align(o + (size_t)NBC * 16 * A16_VST * 2);
+    L.v_lo = o; o = a16_align(o + (size_t)NBC * 16 * A16_VST * 2);
     L.u = o;    o = a16_align(o + (size_t)Nr * 16 * 8);
     L.w = o;    o = a16_align(o + (size_t)A16_MOM * 16 * 8);
     L.pp = o;   o = a16_align(o + (size_t)A16_SUPER * 8);
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
                     _Float16 hr, lr, hi_, li_;
                     split16(in ? f32_opaque((float)vr) : 0.f, hr, lr);
                     split16(in ? f32_opaque((float)vi) : 0.f, hi_, li_);
-                    const int o = col * 32 + 2 * ii;
+                    const int o = col * A16_VST + 2 * ii;
                     typedef _Float16 h2v __attribute__((ext_vector_type(2)));
                     *reinterpret_cast<h2v *>(sVh + o) = h2v{hr, hi_};
                     *reinterpret_cast<h2v *>(sVl + o) = h2v{lr, li_};
@@ -228,8 +230,8 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
                 }
             }
             __syncthreads();
-            const v8h bh = *reinterpret_cast<const v8h *>(sVh + (16 * wv + li) * 32 + 8 * g);
-            const v8h bl = *reinterpret_cast<const v8h *>(sVl + (16 * wv + li) * 32 + 8 * g);
+            const v8h bh = *reinterpret_cast<const v8h *>(sVh + (16 * wv + li) * A16_VST + 8 * g);
+            const v8h bl = *reinterpret_cast<const v8h *>(sVl + (16 * wv + li) * A16_VST + 8 * g);
             const _Float16 *aH = sAh + li * A16_AST + 8 * g;
             const _Float16 *aL = sAl + li * A16_AST + 8 * g;
 #pragma unroll

@@ -772,7 +772,8 @@ int few_spline_adjoint_async(const double *t, const double *g_coef, long g_sys, 
     a.ngroups = 1;
     a.L = length;
     a.ns = nsys;
-    a.scratch = scratch;
+    a.factor = scratch;
+    a.scratch = scratch + 7L * length;
     return launch_spline_adjoint(a, reinterpret_cast<hipStream_t>(stream));
 }
 
@@ -798,7 +799,8 @@ int few_modes_adjoint_async(const double *g_interp, const double *g_phase, const
     a.ngroups = g_phase ? 3 : 2;
     a.L = length;
     a.ns = 2 * K + (g_phase ? 3 : 0);
-    a.scratch = scratch;
+    a.factor = scratch;
+    a.scratch = scratch + 7L * length;
     return launch_spline_adjoint(a, reinterpret_cast<hipStream_t>(stream));
 }
 

@@ -233,8 +233,9 @@ int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_
  * systems of `length` >= 4 knots sharing t: g_coef[s*g_sys + p*g_p +
  * i*g_knot] = dL/dc_p(i) of system s -> g_y[s*gy_sys + i*gy_knot] = dL/dy_i.
  * The final-derivative row, the coefficient formulas, the tridiagonal solve
- * (transposed, LAPACK dgtsv's pivoting) and the right-hand side are
- * reversed in turn.  scratch: 4 * length * nsys doubles.
+ * (transposed, LAPACK dgtsv's pivoting; the elimination of the shared
+ * matrix is done once for all systems) and the right-hand side are reversed
+ * in turn.  scratch: length * (nsys + 7) doubles.
  */
 int few_spline_adjoint_async(const double *t, const double *g_coef, long g_sys, long g_p,
                              long g_knot, int nsys, int length, double *g_y, long gy_sys,

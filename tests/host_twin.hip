@@ -151,14 +151,13 @@ int twin_direct_samples(double *out, const long long *idx, long nidx, const doub
 // g_coef (4, nsys, L) -> g_y (nsys, L).  Returns the first non-zero INFO.
 int twin_spline_adjoint(const double *t, const double *g_coef, int nsys, int L, double *g_y)
 {
-    std::vector<double> w(4 * (size_t)L);
-    int first = 0;
-    for (int s = 0; s < nsys; s++) {
-        int info = few::spline_adjoint_core(t, L, g_coef + (long)s * L, (long)nsys * L, 1,
-                                            g_y + (long)s * L, 1, w.data(), 1);
-        if (info && !first) first = info;
-    }
-    return first;
+    std::vector<double> F(7 * (size_t)L), w((size_t)L);
+    const int info = few::tri_factor_transposed(t, L, F.data());
+    if (info) return info;
+    for (int s = 0; s < nsys; s++)
+        few::spline_adjoint_rhs(t, L, F.data(), g_coef + (long)s * L, (long)nsys * L, 1,
+                                g_y + (long)s * L, 1, w.data(), 1);
+    return 0;
 }
 
 void twin_sincos_cw(const double *x, double *s, double *c, int *ok, long n)
