@@ -22,11 +22,13 @@
 
 #include "batch.hpp"
 #include "few_common.hpp"
+#include "tridiag.hpp"
 
 namespace few {
 
 #define SPLINE_MAX_GROUPS 4
 #define SPLINE_PF 8  // prefetch distance (steps) of the sequential sweeps
+#define SPLINE_SHARED_MAX_L 150  // LDS path: (128 + 6) * L doubles per 64-system block
 
 struct SplineArgs {
     SplineGroup g[SPLINE_MAX_GROUPS];
@@ -37,6 +39,7 @@ struct SplineArgs {
     int nsys_total;
     double *sD, *sDU, *sDL, *sB;
     int *info;
+    int shared_lds;      // groups with shared knots take the LDS path (spline_block_shared)
 };
 
 // Row i of the not-a-knot system (interpolate.cu:49-102).  xa/ya are the
@@ -316,6 +319,125 @@ __host__ __device__ void spline_system_core(const SplineGroup &G, int s, int n, 
 #undef YK
 }
 
+// Coefficients of interval i of system s from slope(i) = xi and slope(i+1) =
+// x_next (set_spline_constants / fill_coefficients, interpolate.cu:28-43),
+// written in the group's layout; the last interval's c1..c3 are returned for
+// the final derivatives.
+__host__ __device__ inline void spline_emit(const SplineGroup &G, int s, int n, int i, double xi,
+                                            double x_next, double t0, double t1, double y0,
+                                            double y1, double &c1L, double &c2L, double &c3L)
+{
+#pragma clang fp contract(off)
+    double *out = G.out;
+    const double dt = t1 - t0;
+    const double slope = (y1 - y0) / dt;
+    const double tt = (xi + x_next - 2 * slope) / dt;
+    const double c1 = xi;
+    const double c2 = (slope - xi) / dt - tt;
+    const double c3 = tt / dt;
+    if (G.dense) {
+        const double h = dt;
+        const double h2 = h * h;
+        const double h3 = h2 * h;
+        const double q1 = c1 * h;
+        const double q2 = c2 * h2;
+        const double q3 = c3 * h3;
+        const long no = (long)(n - 1) * G.nsys;
+        const long o = (long)i * G.nsys + s;
+        out[0 * no + o] = y0;
+        out[1 * no + o] = (q1 + q2) + q3;
+        out[2 * no + o] = -(q2 + q3);
+        out[3 * no + o] = -q3;
+        out[4 * no + o] = 0.0;
+        out[5 * no + o] = 0.0;
+        out[6 * no + o] = 0.0;
+        out[7 * no + o] = 0.0;
+    } else {
+        const long o = (long)s * G.o_sys + (long)i * G.o_knot;
+        out[o] = y0;
+        out[G.o_coef + o] = c1;
+        out[2 * G.o_coef + o] = c2;
+        out[3 * G.o_coef + o] = c3;
+    }
+    if (i == n - 2) {
+        c1L = c1;
+        c2L = c2;
+        c3L = c3;
+    }
+}
+
+// The matrix of the not-a-knot system on knots x (prep_row's d, du, dl with
+// any y), eliminated as dgtsv does: F = piv | fact | d | du | dl (n each).
+__host__ __device__ inline int spline_factor(const double *x, int n, double *F)
+{
+    double *piv = F, *fact = F + n, *d = F + 2 * n, *du = F + 3 * n, *dl = F + 4 * n;
+    for (int r = 0; r < n; r++) {
+        const int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
+        double b_, u_, g_, l_;
+        prep_row(r, n, x[i0], x[i0 + 1], x[i0 + 2], 0.0, 0.0, 0.0, b_, u_, g_, l_);
+        d[r] = g_;
+        if (r < n - 1) du[r] = u_;
+        if (r > 0) dl[r - 1] = l_;
+    }
+    return dgtsv_factor(n, d, du, dl, piv, fact);
+}
+
+// One system on shared knots x with the recorded elimination F: the
+// right-hand sides of prep_row swept forward with dgtsv's multipliers, then
+// back substitution and the coefficients.  y(i) = yv[i*ys], b scratch
+// bv[i*bs].  Same operations as spline_system_core (bit-identical).
+__host__ __device__ inline void spline_system_shared(const SplineGroup &G, int s, int n,
+                                                     const double *x, const double *F,
+                                                     const double *yv, long ys, double *bv,
+                                                     long bs)
+{
+#pragma clang fp contract(off)
+#define SY(i) yv[(long)(i) * ys]
+#define SB(i) bv[(long)(i) * bs]
+    const double *piv = F, *fact = F + n, *d = F + 2 * n, *du = F + 3 * n, *dl = F + 4 * n;
+    double b_, u_, g_, l_;
+    prep_row(0, n, x[0], x[1], x[2], SY(0), SY(1), SY(2), b_, u_, g_, l_);
+    double cur = b_;
+    for (int i = 0; i < n - 1; i++) {
+        const int r = i + 1;
+        const int i0 = (r == n - 1) ? n - 3 : r - 1;
+        prep_row(r, n, x[i0], x[i0 + 1], x[i0 + 2], SY(i0), SY(i0 + 1), SY(i0 + 2), b_, u_, g_, l_);
+        if (piv[i] == 0.0) {
+            SB(i) = cur;
+            cur = b_ - fact[i] * cur;
+        } else {
+            SB(i) = b_;
+            cur = cur - fact[i] * b_;
+        }
+    }
+    double c1L = 0.0, c2L = 0.0, c3L = 0.0;
+    double x_next = cur / d[n - 1];
+    double x_next2 = 0.0;
+    for (int ii = n - 2; ii >= 0; ii--) {
+        double xi;
+        if (ii == n - 2)
+            xi = (SB(ii) - du[ii] * x_next) / d[ii];
+        else
+            xi = (SB(ii) - du[ii] * x_next - dl[ii] * x_next2) / d[ii];
+        spline_emit(G, s, n, ii, xi, x_next, x[ii], x[ii + 1], SY(ii), SY(ii + 1), c1L, c2L, c3L);
+        x_next2 = x_next;
+        x_next = xi;
+    }
+    if (!G.dense) {
+        // fill_final_derivs (interpolate.cu:242-280)
+        const double xx = x[n - 1] - x[n - 2];
+        const double x2 = xx * xx;
+        double *out = G.out;
+        const long o = (long)s * G.o_sys + (long)(n - 1) * G.o_knot;
+        out[o] = SY(n - 1);
+        out[G.o_coef + o] = c1L + 2 * c2L * xx + 3 * c3L * x2;
+        out[2 * G.o_coef + o] = (2. * c2L + 6. * c3L * xx) / 2.;
+        out[3 * G.o_coef + o] = c3L;
+    }
+#undef SY
+#undef SB
+}
+
 __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
 {
     const SplineGroup &G = a.g[gi];
@@ -324,11 +446,44 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
     spline_system_core(G, s, a.length, a.col_prefix[gi] + s, a.nsys_total, sc);
 }
 
+// A 64-system block of a group on shared knots: knots and the block's knot
+// values staged in LDS with one coalesced sweep, the matrix eliminated once
+// (thread 0, while the tile loads), then every lane sweeps its own
+// right-hand side out of LDS (no global-memory latency in the recurrences).
+__device__ void spline_block_shared(const SplineArgs &a, int gi, int blk, double *lds)
+{
+    const SplineGroup &G = a.g[gi];
+    const int n = a.length;
+    const int tid = threadIdx.x;
+    double *sx = lds, *F = sx + n, *sy = F + 5 * n, *sb = sy + (long)n * 64;
+    __shared__ int s_info;
+    const int s0 = blk * 64;
+    const int nloc = min(64, G.nsys - s0);
+    for (int i = tid; i < n; i += 64) sx[i] = G.t[(long)i * G.t_knot];
+    for (long idx = tid; idx < (long)n * 64; idx += 64) {
+        const int i = (int)(idx >> 6), ln = (int)(idx & 63);
+        sy[idx] = ln < nloc ? G.y[(long)(s0 + ln) * G.y_sys + (long)i * G.y_knot] : 0.0;
+    }
+    __syncthreads();
+    if (tid == 0) s_info = spline_factor(sx, n, F);
+    __syncthreads();
+    if (s_info != 0) {  // singular: dgtsv's early return, handled system by system
+        spline_one_system(a, gi, s0 + tid);
+        return;
+    }
+    if (tid < nloc) spline_system_shared(G, s0 + tid, n, sx, F, sy + tid, 64, sb + tid, 64);
+}
+
 __global__ __launch_bounds__(64) void spline_build_kernel(SplineArgs a)
 {
+    extern __shared__ __attribute__((aligned(16))) double spl_lds[];
     int gi = 0;
     while (gi + 1 < a.ngroups && (int)blockIdx.x >= a.blk_prefix[gi + 1]) gi++;
     const int blk = (int)blockIdx.x - a.blk_prefix[gi];
+    if (a.shared_lds && a.g[gi].t_sys == 0) {  // uniform over the block
+        spline_block_shared(a, gi, blk, spl_lds);
+        return;
+    }
     spline_one_system(a, gi, blk * 64 + (int)threadIdx.x);
 }
 
@@ -351,6 +506,7 @@ void fill_spline_args(const SplineGroup *groups, int ngroups, int length, double
     a.sDL = scratch4[2];
     a.sB = scratch4[3];
     a.info = info_out;
+    a.shared_lds = 0;
 }
 
 // Batched build: every system of every waveform (2K amplitude arrays, then
@@ -405,7 +561,17 @@ int launch_spline_build(const SplineGroup *groups, int ngroups, int length, doub
     fill_spline_args(groups, ngroups, length, scratch4, info_out, a);
     const int nblk = a.blk_prefix[ngroups];
     if (nblk == 0) return FEW_OK;
-    hipLaunchKernelGGL(spline_build_kernel, dim3(nblk), dim3(64), 0, stream, a);
+    size_t lds = 0;
+    if (length <= SPLINE_SHARED_MAX_L) {
+        for (int g = 0; g < ngroups; g++)
+            if (groups[g].t_sys == 0) a.shared_lds = 1;
+        if (a.shared_lds) {
+            lds = sizeof(double) * (size_t)length * (64 * 2 + 6);
+            FEW_HIP(hipFuncSetAttribute((const void *)spline_build_kernel,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        }
+    }
+    hipLaunchKernelGGL(spline_build_kernel, dim3(nblk), dim3(64), lds, stream, a);
     FEW_HIP(hipGetLastError());
     return FEW_OK;
 }

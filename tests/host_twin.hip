@@ -68,6 +68,26 @@ int twin_prepare_modes(const double *teuk, const double *phase_knots, const doub
     return 0;
 }
 
+// The shared-knot path of the spline build (spline_factor + spline_system_shared,
+// as spline_block_shared runs them), system by system on the host.
+int twin_prepare_modes_shared(const double *teuk, const double *phase_knots, const double *t,
+                              int length, int num_modes, double *interp_out, double *pcoef_out)
+{
+    const long K = num_modes, L = length;
+    SplineGroup g[3];
+    g[0] = SplineGroup{teuk, 2, 2 * K, t, 0, 1, interp_out, L * 2 * K, 1, 2 * K, num_modes, 0};
+    g[1] = SplineGroup{teuk + 1, 2, 2 * K, t, 0, 1, interp_out + K, L * 2 * K, 1, 2 * K, num_modes, 0};
+    g[2] = SplineGroup{phase_knots, L, 1, t, 0, 1, pcoef_out, 0, 0, 0, 3, 1};
+    std::vector<double> F(5 * (size_t)L), b((size_t)L);
+    const int info = spline_factor(t, length, F.data());
+    if (info) return info;
+    for (int gi = 0; gi < 3; gi++)
+        for (int s = 0; s < g[gi].nsys; s++)
+            spline_system_shared(g[gi], s, length, t, F.data(), g[gi].y + (long)s * g[gi].y_sys,
+                                 g[gi].y_knot, b.data(), 1);
+    return 0;
+}
+
 // Interval planning exactly as plan_segments_kernel computes it.
 int twin_plan(const double *h_t, int L, double dt, int out_len, int *starts, int *ends)
 {

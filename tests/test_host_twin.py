@@ -82,6 +82,24 @@ def test_twin_spline_pivoting_bitwise(tw):
     assert np.array_equal(interp, ref)
 
 
+@pytest.mark.parametrize("L,K", [(4, 3), (9, 20), (120, 50), (150, 7)])
+def test_twin_spline_shared_bitwise(tw, L, K):
+    """The shared-knot (LDS) path of the spline build -- one elimination of the
+    common matrix, then right-hand-side sweeps -- equals the oracle bit for bit."""
+    from fastemriwaveforms_amd import synthetic
+    inp = synthetic.make_input(0.01, 10.0, L, "m_nonneg_30", K, 100 + L, name="sh")
+    tw.twin_prepare_modes_shared.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int, P, P]
+    interp = np.zeros((4, L, 2 * K))
+    pcoef = np.zeros((8, L - 1, 3))
+    teuk = np.ascontiguousarray(inp.teuk)
+    pk = np.ascontiguousarray(inp.phase_knots)
+    assert tw.twin_prepare_modes_shared(_p(teuk), _p(pk), _p(inp.t), L, K, _p(interp),
+                                        _p(pcoef)) == 0
+    interp_ref, pcoef_ref = oracle_coefficients(inp)
+    assert np.array_equal(interp, interp_ref)
+    assert np.array_equal(pcoef, pcoef_ref)
+
+
 def test_twin_prepare_modes_bitwise(tw):
     from fastemriwaveforms_amd import synthetic
     inp = synthetic.make_workload("cpu0")
