@@ -16,6 +16,7 @@
 #include "adjoint.hpp"
 #include "few_math.hpp"
 #include "modesum_direct.hpp"
+#include "tridiag.hpp"
 
 namespace few {
 
@@ -57,44 +58,8 @@ __host__ __device__ int tri_factor_transposed(const double *x, int L, double *F)
         dl[i] = sub;
         du[i] = sup;
     }
-    const int n = L;
-    double f, temp;
-    int i;
-    for (i = 0; i < n - 2; i++) {
-        if (fabs(d[i]) >= fabs(dl[i])) {
-            if (d[i] == 0.0) return i + 1;
-            f = dl[i] / d[i];
-            d[i + 1] = d[i + 1] - f * du[i];
-            dl[i] = 0.0;
-            piv[i] = 0.0;
-        } else {
-            f = d[i] / dl[i];
-            d[i] = dl[i];
-            temp = d[i + 1];
-            d[i + 1] = du[i] - f * temp;
-            dl[i] = du[i + 1];
-            du[i + 1] = -f * dl[i];
-            du[i] = temp;
-            piv[i] = 1.0;
-        }
-        fact[i] = f;
-    }
-    i = n - 2;
-    if (fabs(d[i]) >= fabs(dl[i])) {
-        if (d[i] == 0.0) return i + 1;
-        f = dl[i] / d[i];
-        d[i + 1] = d[i + 1] - f * du[i];
-        piv[i] = 0.0;
-    } else {
-        f = d[i] / dl[i];
-        d[i] = dl[i];
-        temp = d[i + 1];
-        d[i + 1] = du[i] - f * temp;
-        du[i] = temp;
-        piv[i] = 1.0;
-    }
-    fact[i] = f;
-    if (d[n - 1] == 0.0) return n;
+    const int info = dgtsv_factor(L, d, du, dl, piv, fact);
+    if (info) return info;
     // fill_B (interpolate.cu:107-145): weights of row i on y[i-1] (-al), y[i] (al - be), y[i+1] (be)
     {
         const double dx1 = x[1] - x[0], dx2 = x[2] - x[1], dd = x[2] - x[0];
@@ -117,9 +82,9 @@ __host__ __device__ int tri_factor_transposed(const double *x, int L, double *F)
 
 // One system: gradient G(p, i) = g[p*gp + i*gk] of its (4, L) coefficients ->
 // dL/dy(i) = gy[i*gyk].  s: L scratch doubles with stride ws.
-__host__ __device__ void spline_adjoint_rhs(const double *x, int L, const double *F,
-                                            const double *g, long gp, long gk, double *gy,
-                                            long gyk, double *s, long ws)
+__host__ __device__ __attribute__((always_inline)) void spline_adjoint_rhs(
+    const double *x, int L, const double *F, const double *g, long gp, long gk, double *gy,
+    long gyk, double *s, long ws)
 {
 #pragma clang fp contract(off)
 #define G(p, i) g[(long)(p) * gp + (long)(i) * gk]
@@ -133,9 +98,8 @@ __host__ __device__ void spline_adjoint_rhs(const double *x, int L, const double
     const double h = x[L - 1] - x[L - 2];
     const double f1 = G(1, L - 1), f2 = G(2, L - 1), f3 = G(3, L - 1);
     double pend_z = 0.0, q_prev = 0.0;
-    for (int i = 0; i < L - 1; i++) {
+    auto interval = [&](int i, double g0, double g1, double g2, double g3) {
         const double dx = x[i + 1] - x[i];
-        double g1 = G(1, i), g2 = G(2, i), g3 = G(3, i);
         if (i == L - 2) {
             g1 += f1;
             g2 += 2.0 * h * f1 + f2;
@@ -145,9 +109,37 @@ __host__ __device__ void spline_adjoint_rhs(const double *x, int L, const double
         S(i) = pend_z + (g1 - 2.0 * g2 / dx + g3 / (dx * dx));
         pend_z = 0.0 + (-g2 / dx + g3 / (dx * dx));
         const double q = gs / dx;
-        GY(i) = (i > 0 ? G(0, i) + q_prev : G(0, i)) - q;
+        GY(i) = (i > 0 ? g0 + q_prev : g0) - q;
         q_prev = q;
+    };
+    // The gradient loads are issued ADJ_PF intervals ahead through a register
+    // ring so the sequential pass does not wait on memory each interval.
+    int i = 0;
+    if (L - 1 >= ADJ_PF) {
+        double r0[ADJ_PF], r1[ADJ_PF], r2[ADJ_PF], r3[ADJ_PF];
+#pragma unroll
+        for (int u = 0; u < ADJ_PF; u++) {
+            r0[u] = G(0, u);
+            r1[u] = G(1, u);
+            r2[u] = G(2, u);
+            r3[u] = G(3, u);
+        }
+        for (; i + ADJ_PF <= L - 1; i += ADJ_PF) {
+#pragma unroll
+            for (int u = 0; u < ADJ_PF; u++) {
+                const double g0 = r0[u], g1 = r1[u], g2 = r2[u], g3 = r3[u];
+                const int nx = i + u + ADJ_PF;
+                if (nx < L - 1) {
+                    r0[u] = G(0, nx);
+                    r1[u] = G(1, nx);
+                    r2[u] = G(2, nx);
+                    r3[u] = G(3, nx);
+                }
+                interval(i + u, g0, g1, g2, g3);
+            }
+        }
     }
+    for (; i < L - 1; i++) interval(i, G(0, i), G(1, i), G(2, i), G(3, i));
     S(L - 1) = pend_z;
     GY(L - 1) = G(0, L - 1) + q_prev;
     // the solve A^T lambda = z: forward sweep with the recorded multipliers
@@ -211,6 +203,29 @@ __global__ __launch_bounds__(64) void tri_factor_kernel(const double *t, int L, 
         const int r = tri_factor_transposed(t, L, F);
         if (info) *info = r;
     }
+}
+
+// One workgroup: 64 systems of group gi on the shared knots, with the knots,
+// the elimination and both right-hand-side tiles (scratch and knot-value
+// gradients) in LDS; the arithmetic is spline_adjoint_rhs's.
+__global__ __launch_bounds__(64) void spline_adjoint_lds_kernel(SplineAdjArgs a)
+{
+    extern __shared__ __attribute__((aligned(16))) double adj_lds[];
+    int gi = 0;
+    while (gi + 1 < a.ngroups && (int)blockIdx.x >= a.blk_prefix[gi + 1]) gi++;
+    const SplineAdjGroup &G = a.grp[gi];
+    const int L = a.L, tid = threadIdx.x;
+    const long s0 = (long)((int)blockIdx.x - a.blk_prefix[gi]) * 64;
+    double *sx = adj_lds, *F = sx + L, *sS = F + 7L * L, *sG = sS + 64L * L;
+    for (int i = tid; i < L; i += 64) sx[i] = a.t[i];
+    __syncthreads();
+    if (tid == 0) tri_factor_transposed(sx, L, F);
+    __syncthreads();
+    const long s = s0 + tid;
+    if (s >= G.nsys) return;
+    spline_adjoint_rhs(sx, L, F, G.g + s * G.g_sys, G.g_p, G.g_knot, sG + tid, 64, sS + tid, 64);
+    double *gy = G.gy + s * G.gy_sys;
+    for (int i = 0; i < L; i++) gy[(long)i * G.gy_knot] = sG[(long)i * 64 + tid];
 }
 
 __global__ __launch_bounds__(64) void spline_adjoint_kernel(SplineAdjArgs a)
@@ -435,6 +450,19 @@ int launch_spline_adjoint(const SplineAdjArgs &a, hipStream_t stream)
     if (total > a.ns) {
         set_error("spline_adjoint: %ld systems exceed %ld scratch columns", total, a.ns);
         return FEW_ERR_ARG;
+    }
+    if (a.L <= ADJ_SHARED_MAX_L) {
+        SplineAdjArgs b = a;
+        b.blk_prefix[0] = 0;
+        for (int g = 0; g < 3; g++)
+            b.blk_prefix[g + 1] = b.blk_prefix[g] + (g < a.ngroups ? (a.grp[g].nsys + 63) / 64 : 0);
+        const size_t lds = sizeof(double) * (size_t)a.L * (8 + 128);
+        FEW_HIP(hipFuncSetAttribute((const void *)spline_adjoint_lds_kernel,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(spline_adjoint_lds_kernel, dim3((unsigned)b.blk_prefix[a.ngroups]),
+                           dim3(64), lds, stream, b);
+        FEW_HIP(hipGetLastError());
+        return FEW_OK;
     }
     hipLaunchKernelGGL(tri_factor_kernel, dim3(1), dim3(64), 0, stream, a.t, a.L, a.factor,
                        nullptr);

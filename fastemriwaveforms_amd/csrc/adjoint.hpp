@@ -66,7 +66,13 @@ struct SplineAdjArgs {
     long ns;                  // scratch columns (total systems)
     double *scratch;          // L * ns doubles: one right-hand side per system
     double *factor;           // 7 * L doubles: the shared elimination (tri_factor_transposed)
+    int blk_prefix[4];        // first workgroup of each group (LDS path)
 };
+
+// Knot counts up to this run the LDS path: knots, elimination and two L x 64
+// tiles of one workgroup's right-hand sides stay in LDS (8 L (8 + 128) bytes).
+#define ADJ_SHARED_MAX_L 150
+#define ADJ_PF 8               // prefetch distance (intervals) of the gradient loads
 
 // Elimination of the transposed not-a-knot matrix of knots x (shared by all
 // systems on those knots) into F (7 x L doubles); returns dgtsv's INFO.
