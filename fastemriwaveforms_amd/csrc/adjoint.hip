@@ -128,13 +128,13 @@ __host__ __device__ __attribute__((always_inline)) void spline_adjoint_rhs(
 #pragma unroll
             for (int u = 0; u < ADJ_PF; u++) {
                 const double g0 = r0[u], g1 = r1[u], g2 = r2[u], g3 = r3[u];
-                const int nx = i + u + ADJ_PF;
-                if (nx < L - 1) {
-                    r0[u] = G(0, nx);
-                    r1[u] = G(1, nx);
-                    r2[u] = G(2, nx);
-                    r3[u] = G(3, nx);
-                }
+                // unconditional (clamped) loads keep the ring free of branches,
+                // which would make the compiler wait for them at once
+                const int nx = min(i + u + ADJ_PF, L - 1);
+                r0[u] = G(0, nx);
+                r1[u] = G(1, nx);
+                r2[u] = G(2, nx);
+                r3[u] = G(3, nx);
                 interval(i + u, g0, g1, g2, g3);
             }
         }

@@ -29,6 +29,7 @@ namespace few {
 #define SPLINE_MAX_GROUPS 4
 #define SPLINE_PF 8  // prefetch distance (steps) of the sequential sweeps
 #define SPLINE_SHARED_MAX_L 150  // LDS path: (128 + 6) * L doubles per 64-system block
+#define SPLINE_TILE_BATCH 24     // knot-value loads in flight per lane while staging a tile
 
 struct SplineArgs {
     SplineGroup g[SPLINE_MAX_GROUPS];
@@ -460,9 +461,19 @@ __device__ void spline_block_shared(const SplineArgs &a, int gi, int blk, double
     const int s0 = blk * 64;
     const int nloc = min(64, G.nsys - s0);
     for (int i = tid; i < n; i += 64) sx[i] = G.t[(long)i * G.t_knot];
-    for (long idx = tid; idx < (long)n * 64; idx += 64) {
-        const int i = (int)(idx >> 6), ln = (int)(idx & 63);
-        sy[idx] = ln < nloc ? G.y[(long)(s0 + ln) * G.y_sys + (long)i * G.y_knot] : 0.0;
+    // lane tid stages the knot values of system s0 + tid, SPLINE_TILE_BATCH
+    // loads in flight at a time (clamped addresses, no branches)
+    {
+        const double *ys = G.y + (long)(s0 + min(tid, nloc - 1)) * G.y_sys;
+        for (int i0 = 0; i0 < n; i0 += SPLINE_TILE_BATCH) {
+            double v[SPLINE_TILE_BATCH];
+#pragma unroll
+            for (int u = 0; u < SPLINE_TILE_BATCH; u++)
+                v[u] = ys[(long)min(i0 + u, n - 1) * G.y_knot];
+#pragma unroll
+            for (int u = 0; u < SPLINE_TILE_BATCH; u++)
+                if (i0 + u < n) sy[(long)(i0 + u) * 64 + tid] = tid < nloc ? v[u] : 0.0;
+        }
     }
     __syncthreads();
     if (tid == 0) s_info = spline_factor(sx, n, F);
