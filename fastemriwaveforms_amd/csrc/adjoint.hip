@@ -81,17 +81,17 @@ __host__ __device__ int tri_factor_transposed(const double *x, int L, double *F)
 }
 
 // One system: gradient G(p, i) = g[p*gp + i*gk] of its (4, L) coefficients ->
-// dL/dy(i) = gy[i*gyk].  s: L scratch doubles with stride ws.
-__host__ __device__ __attribute__((always_inline)) void spline_adjoint_rhs(
-    const double *x, int L, const double *F, const double *g, long gp, long gk, double *gy,
-    long gyk, double *s, long ws)
+// dL/dy(i) = gy[i*gyk].  s: L scratch doubles with stride ws.  Two passes:
+// spline_adjoint_pass1 needs only the knots (fill_coefficients and the final
+// derivatives transposed); spline_adjoint_solve needs the elimination F.
+__host__ __device__ __attribute__((always_inline)) void spline_adjoint_pass1(
+    const double *x, int L, const double *g, long gp, long gk, double *gy, long gyk, double *s,
+    long ws)
 {
 #pragma clang fp contract(off)
 #define G(p, i) g[(long)(p) * gp + (long)(i) * gk]
 #define GY(i) gy[(long)(i) * gyk]
 #define S(i) s[(long)(i) * ws]
-    const double *piv = F, *fact = F + L, *d = F + 2L * L, *du = F + 3L * L, *dl = F + 4L * L;
-    const double *al = F + 5L * L, *be = F + 6L * L;
     // final derivatives (interpolate.cu:242-280) fold into interval L-2;
     // fill_coefficients (interpolate.cu:28-43) transposed, interval by
     // interval: z = dL/d(slopes) into s, the direct y terms into gy
@@ -142,6 +142,15 @@ __host__ __device__ __attribute__((always_inline)) void spline_adjoint_rhs(
     for (; i < L - 1; i++) interval(i, G(0, i), G(1, i), G(2, i), G(3, i));
     S(L - 1) = pend_z;
     GY(L - 1) = G(0, L - 1) + q_prev;
+#undef G
+}
+
+__host__ __device__ __attribute__((always_inline)) void spline_adjoint_solve(
+    const double *x, int L, const double *F, double *gy, long gyk, double *s, long ws)
+{
+#pragma clang fp contract(off)
+    const double *piv = F, *fact = F + L, *d = F + 2L * L, *du = F + 3L * L, *dl = F + 4L * L;
+    const double *al = F + 5L * L, *be = F + 6L * L;
     // the solve A^T lambda = z: forward sweep with the recorded multipliers
     double cur = S(0);
     for (int i = 0; i < L - 1; i++) {
@@ -191,9 +200,16 @@ __host__ __device__ __attribute__((always_inline)) void spline_adjoint_rhs(
     GY(L - 3) = y0;
     GY(L - 2) = y1;
     GY(L - 1) = y2;
-#undef G
 #undef GY
 #undef S
+}
+
+__host__ __device__ __attribute__((always_inline)) void spline_adjoint_rhs(
+    const double *x, int L, const double *F, const double *g, long gp, long gk, double *gy,
+    long gyk, double *s, long ws)
+{
+    spline_adjoint_pass1(x, L, g, gp, gk, gy, gyk, s, ws);
+    spline_adjoint_solve(x, L, F, gy, gyk, s, ws);
 }
 
 __global__ __launch_bounds__(64) void tri_factor_kernel(const double *t, int L, double *F,
@@ -207,8 +223,10 @@ __global__ __launch_bounds__(64) void tri_factor_kernel(const double *t, int L, 
 
 // One workgroup: 64 systems of group gi on the shared knots, with the knots,
 // the elimination and both right-hand-side tiles (scratch and knot-value
-// gradients) in LDS; the arithmetic is spline_adjoint_rhs's.
-__global__ __launch_bounds__(64) void spline_adjoint_lds_kernel(SplineAdjArgs a)
+// gradients) in LDS; the arithmetic is spline_adjoint_rhs's.  Two waves:
+// wave 1 (one lane) eliminates the transposed matrix while wave 0 runs the
+// first pass, which needs only the knots.
+__global__ __launch_bounds__(128) void spline_adjoint_lds_kernel(SplineAdjArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) double adj_lds[];
     int gi = 0;
@@ -217,13 +235,17 @@ __global__ __launch_bounds__(64) void spline_adjoint_lds_kernel(SplineAdjArgs a)
     const int L = a.L, tid = threadIdx.x;
     const long s0 = (long)((int)blockIdx.x - a.blk_prefix[gi]) * 64;
     double *sx = adj_lds, *F = sx + L, *sS = F + 7L * L, *sG = sS + 64L * L;
-    for (int i = tid; i < L; i += 64) sx[i] = a.t[i];
-    __syncthreads();
-    if (tid == 0) tri_factor_transposed(sx, L, F);
+    for (int i = tid; i < L; i += 128) sx[i] = a.t[i];
     __syncthreads();
     const long s = s0 + tid;
-    if (s >= G.nsys) return;
-    spline_adjoint_rhs(sx, L, F, G.g + s * G.g_sys, G.g_p, G.g_knot, sG + tid, 64, sS + tid, 64);
+    if (tid >= 64) {
+        if (tid == 64) tri_factor_transposed(sx, L, F);
+    } else if (s < G.nsys) {
+        spline_adjoint_pass1(sx, L, G.g + s * G.g_sys, G.g_p, G.g_knot, sG + tid, 64, sS + tid, 64);
+    }
+    __syncthreads();
+    if (tid >= 64 || s >= G.nsys) return;
+    spline_adjoint_solve(sx, L, F, sG + tid, 64, sS + tid, 64);
     double *gy = G.gy + s * G.gy_sys;
     for (int i = 0; i < L; i++) gy[(long)i * G.gy_knot] = sG[(long)i * 64 + tid];
 }
@@ -460,7 +482,7 @@ int launch_spline_adjoint(const SplineAdjArgs &a, hipStream_t stream)
         FEW_HIP(hipFuncSetAttribute((const void *)spline_adjoint_lds_kernel,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(spline_adjoint_lds_kernel, dim3((unsigned)b.blk_prefix[a.ngroups]),
-                           dim3(64), lds, stream, b);
+                           dim3(128), lds, stream, b);
         FEW_HIP(hipGetLastError());
         return FEW_OK;
     }

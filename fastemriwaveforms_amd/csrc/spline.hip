@@ -383,36 +383,51 @@ __host__ __device__ inline int spline_factor(const double *x, int n, double *F)
     return dgtsv_factor(n, d, du, dl, piv, fact);
 }
 
-// One system on shared knots x with the recorded elimination F: the
-// right-hand sides of prep_row swept forward with dgtsv's multipliers, then
-// back substitution and the coefficients.  y(i) = yv[i*ys], b scratch
-// bv[i*bs].  Same operations as spline_system_core (bit-identical).
-__host__ __device__ inline void spline_system_shared(const SplineGroup &G, int s, int n,
-                                                     const double *x, const double *F,
-                                                     const double *yv, long ys, double *bv,
-                                                     long bs)
+// One system on shared knots x with the recorded elimination F, in passes
+// whose iterations are independent wherever the algorithm allows it (so the
+// divisions of neighbouring rows overlap instead of forming one chain):
+//   spline_rhs_pass:   prep_row's right-hand side of every row -> b
+//   spline_solve_emit: dgtsv's forward sweep of b with the recorded
+//                      multipliers, back substitution (slopes into b), then
+//                      the coefficients of every interval and the final
+//                      derivatives.
+// y(i) = yv[i*ys], b(i) = bv[i*bs].  Same operations as spline_system_core
+// (bit-identical).
+__host__ __device__ inline void spline_rhs_pass(int n, const double *x, const double *yv, long ys,
+                                                double *bv, long bs)
+{
+#pragma clang fp contract(off)
+#pragma unroll 4
+    for (int r = 0; r < n; r++) {
+        const int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
+        double b_, u_, g_, l_;
+        prep_row(r, n, x[i0], x[i0 + 1], x[i0 + 2], yv[(long)i0 * ys], yv[(long)(i0 + 1) * ys],
+                 yv[(long)(i0 + 2) * ys], b_, u_, g_, l_);
+        bv[(long)r * bs] = b_;
+    }
+}
+
+__host__ __device__ inline void spline_solve_emit(const SplineGroup &G, int s, int n,
+                                                  const double *x, const double *F,
+                                                  const double *yv, long ys, double *bv, long bs)
 {
 #pragma clang fp contract(off)
 #define SY(i) yv[(long)(i) * ys]
 #define SB(i) bv[(long)(i) * bs]
     const double *piv = F, *fact = F + n, *d = F + 2 * n, *du = F + 3 * n, *dl = F + 4 * n;
-    double b_, u_, g_, l_;
-    prep_row(0, n, x[0], x[1], x[2], SY(0), SY(1), SY(2), b_, u_, g_, l_);
-    double cur = b_;
+    double cur = SB(0);
     for (int i = 0; i < n - 1; i++) {
-        const int r = i + 1;
-        const int i0 = (r == n - 1) ? n - 3 : r - 1;
-        prep_row(r, n, x[i0], x[i0 + 1], x[i0 + 2], SY(i0), SY(i0 + 1), SY(i0 + 2), b_, u_, g_, l_);
+        const double nb = SB(i + 1);
         if (piv[i] == 0.0) {
             SB(i) = cur;
-            cur = b_ - fact[i] * cur;
+            cur = nb - fact[i] * cur;
         } else {
-            SB(i) = b_;
-            cur = cur - fact[i] * b_;
+            SB(i) = nb;
+            cur = cur - fact[i] * nb;
         }
     }
-    double c1L = 0.0, c2L = 0.0, c3L = 0.0;
     double x_next = cur / d[n - 1];
+    SB(n - 1) = x_next;
     double x_next2 = 0.0;
     for (int ii = n - 2; ii >= 0; ii--) {
         double xi;
@@ -420,10 +435,15 @@ __host__ __device__ inline void spline_system_shared(const SplineGroup &G, int s
             xi = (SB(ii) - du[ii] * x_next) / d[ii];
         else
             xi = (SB(ii) - du[ii] * x_next - dl[ii] * x_next2) / d[ii];
-        spline_emit(G, s, n, ii, xi, x_next, x[ii], x[ii + 1], SY(ii), SY(ii + 1), c1L, c2L, c3L);
+        SB(ii) = xi;
         x_next2 = x_next;
         x_next = xi;
     }
+    double c1L = 0.0, c2L = 0.0, c3L = 0.0;
+#pragma unroll 4
+    for (int ii = 0; ii < n - 1; ii++)
+        spline_emit(G, s, n, ii, SB(ii), SB(ii + 1), x[ii], x[ii + 1], SY(ii), SY(ii + 1), c1L,
+                    c2L, c3L);
     if (!G.dense) {
         // fill_final_derivs (interpolate.cu:242-280)
         const double xx = x[n - 1] - x[n - 2];
@@ -439,6 +459,15 @@ __host__ __device__ inline void spline_system_shared(const SplineGroup &G, int s
 #undef SB
 }
 
+__host__ __device__ inline void spline_system_shared(const SplineGroup &G, int s, int n,
+                                                     const double *x, const double *F,
+                                                     const double *yv, long ys, double *bv,
+                                                     long bs)
+{
+    spline_rhs_pass(n, x, yv, ys, bv, bs);
+    spline_solve_emit(G, s, n, x, F, yv, ys, bv, bs);
+}
+
 __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
 {
     const SplineGroup &G = a.g[gi];
@@ -447,10 +476,10 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
     spline_system_core(G, s, a.length, a.col_prefix[gi] + s, a.nsys_total, sc);
 }
 
-// A 64-system block of a group on shared knots: knots and the block's knot
-// values staged in LDS with one coalesced sweep, the matrix eliminated once
-// (thread 0, while the tile loads), then every lane sweeps its own
-// right-hand side out of LDS (no global-memory latency in the recurrences).
+// A 64-system block of a group on shared knots, two waves: wave 1 (one
+// lane) eliminates the shared matrix while wave 0 stages its 64 systems'
+// knot values in LDS and computes their right-hand sides; then wave 0 runs
+// the sweeps out of LDS (no global-memory latency in the recurrences).
 __device__ void spline_block_shared(const SplineArgs &a, int gi, int blk, double *lds)
 {
     const SplineGroup &G = a.g[gi];
@@ -460,10 +489,15 @@ __device__ void spline_block_shared(const SplineArgs &a, int gi, int blk, double
     __shared__ int s_info;
     const int s0 = blk * 64;
     const int nloc = min(64, G.nsys - s0);
-    for (int i = tid; i < n; i += 64) sx[i] = G.t[(long)i * G.t_knot];
-    // lane tid stages the knot values of system s0 + tid, SPLINE_TILE_BATCH
-    // loads in flight at a time (clamped addresses, no branches)
-    {
+    for (int i = tid; i < n; i += 128) sx[i] = G.t[(long)i * G.t_knot];
+    __syncthreads();
+    if (tid >= 64) {
+        if (tid == 64) s_info = spline_factor(sx, n, F);
+    } else {
+        // lane tid stages the knot values of system s0 + tid, SPLINE_TILE_BATCH
+        // loads in flight at a time (clamped addresses, no branches); a lane
+        // reads back only its own column, so no barrier is needed before the
+        // right-hand-side pass
         const double *ys = G.y + (long)(s0 + min(tid, nloc - 1)) * G.y_sys;
         for (int i0 = 0; i0 < n; i0 += SPLINE_TILE_BATCH) {
             double v[SPLINE_TILE_BATCH];
@@ -474,18 +508,18 @@ __device__ void spline_block_shared(const SplineArgs &a, int gi, int blk, double
             for (int u = 0; u < SPLINE_TILE_BATCH; u++)
                 if (i0 + u < n) sy[(long)(i0 + u) * 64 + tid] = tid < nloc ? v[u] : 0.0;
         }
+        spline_rhs_pass(n, sx, sy + tid, 64, sb + tid, 64);
     }
     __syncthreads();
-    if (tid == 0) s_info = spline_factor(sx, n, F);
-    __syncthreads();
+    if (tid >= 64) return;
     if (s_info != 0) {  // singular: dgtsv's early return, handled system by system
         spline_one_system(a, gi, s0 + tid);
         return;
     }
-    if (tid < nloc) spline_system_shared(G, s0 + tid, n, sx, F, sy + tid, 64, sb + tid, 64);
+    if (tid < nloc) spline_solve_emit(G, s0 + tid, n, sx, F, sy + tid, 64, sb + tid, 64);
 }
 
-__global__ __launch_bounds__(64) void spline_build_kernel(SplineArgs a)
+__global__ __launch_bounds__(128) void spline_build_kernel(SplineArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) double spl_lds[];
     int gi = 0;
@@ -495,7 +529,7 @@ __global__ __launch_bounds__(64) void spline_build_kernel(SplineArgs a)
         spline_block_shared(a, gi, blk, spl_lds);
         return;
     }
-    spline_one_system(a, gi, blk * 64 + (int)threadIdx.x);
+    if (threadIdx.x < 64) spline_one_system(a, gi, blk * 64 + (int)threadIdx.x);
 }
 
 void fill_spline_args(const SplineGroup *groups, int ngroups, int length, double *scratch4[4],
@@ -582,7 +616,7 @@ int launch_spline_build(const SplineGroup *groups, int ngroups, int length, doub
                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         }
     }
-    hipLaunchKernelGGL(spline_build_kernel, dim3(nblk), dim3(64), lds, stream, a);
+    hipLaunchKernelGGL(spline_build_kernel, dim3(nblk), dim3(128), lds, stream, a);
     FEW_HIP(hipGetLastError());
     return FEW_OK;
 }

@@ -21,22 +21,33 @@ __host__ __device__ inline int dgtsv_factor(int n, double *d, double *du, double
                                             double *fact)
 {
 #pragma clang fp contract(off)
+    // di / dui: the current (already updated) d[i] and du[i], carried in
+    // registers so each step waits only on its own arithmetic
+    double di = d[0], dui = n > 1 ? du[0] : 0.0;
     double f, temp;
     int i;
     for (i = 0; i < n - 2; i++) {
-        if (fabs(d[i]) >= fabs(dl[i])) {
-            if (d[i] == 0.0) return i + 1;
-            f = dl[i] / d[i];
-            d[i + 1] = d[i + 1] - f * du[i];
+        const double dli = dl[i], dn = d[i + 1], dun = du[i + 1];
+        if (fabs(di) >= fabs(dli)) {
+            if (di == 0.0) {
+                d[i] = di;
+                du[i] = dui;
+                return i + 1;
+            }
+            f = dli / di;
+            d[i] = di;
+            di = dn - f * dui;
             dl[i] = 0.0;
+            du[i] = dui;
+            dui = dun;
             piv[i] = 0.0;
         } else {
-            f = d[i] / dl[i];
-            d[i] = dl[i];
-            temp = d[i + 1];
-            d[i + 1] = du[i] - f * temp;
-            dl[i] = du[i + 1];
-            du[i + 1] = -f * dl[i];
+            f = di / dli;
+            d[i] = dli;
+            temp = dn;
+            di = dui - f * temp;
+            dl[i] = dun;
+            dui = -f * dun;
             du[i] = temp;
             piv[i] = 1.0;
         }
@@ -44,22 +55,30 @@ __host__ __device__ inline int dgtsv_factor(int n, double *d, double *du, double
     }
     if (n > 1) {
         i = n - 2;
-        if (fabs(d[i]) >= fabs(dl[i])) {
-            if (d[i] == 0.0) return i + 1;
-            f = dl[i] / d[i];
-            d[i + 1] = d[i + 1] - f * du[i];
+        const double dli = dl[i], dn = d[i + 1];
+        if (fabs(di) >= fabs(dli)) {
+            if (di == 0.0) {
+                d[i] = di;
+                du[i] = dui;
+                return i + 1;
+            }
+            f = dli / di;
+            d[i] = di;
+            di = dn - f * dui;
+            du[i] = dui;
             piv[i] = 0.0;
         } else {
-            f = d[i] / dl[i];
-            d[i] = dl[i];
-            temp = d[i + 1];
-            d[i + 1] = du[i] - f * temp;
+            f = di / dli;
+            d[i] = dli;
+            temp = dn;
+            di = dui - f * temp;
             du[i] = temp;
             piv[i] = 1.0;
         }
         fact[i] = f;
     }
-    if (d[n - 1] == 0.0) return n;
+    d[n - 1] = di;
+    if (di == 0.0) return n;
     return 0;
 }
 
