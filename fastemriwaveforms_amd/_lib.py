@@ -65,6 +65,13 @@ def _declare(lib: ctypes.CDLL) -> None:
                                       c_int),
         "few_waveform_adjoint_plan_async": ([_d, _d, _d, _d, _d, _d, _d, _d, c_int, c_int, c_int, _d,
                                              c_double, _d, _d, _d, c_int, _d, c_size, _d], c_int),
+        "few_fd_workspace_bytes": ([c_int, c_int, ctypes.POINTER(c_size)], c_int),
+        "few_get_waveform_generic_fd_async": ([_d, _d, _d, _d, _d, _d, _d, c_int, c_double, _d,
+                                               c_int, c_int, _d, _d, _d, c_int, _d, c_int, c_int,
+                                               c_int, _d, c_size, _d], c_int),
+        "few_get_waveform_generic_fd": ([_d, _d, _d, _d, _d, _d, _d, c_int, c_double, _d, c_int,
+                                         c_int, _d, _d, _d, c_int, _d, c_int, c_int, c_int, _d],
+                                        c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -105,5 +112,6 @@ def exported_symbols() -> list[str]:
         "few_batch_workspace_bytes", "few_batch_run_async", "few_adjoint_workspace_bytes",
         "few_waveform_adjoint_async", "few_spline_adjoint_async", "few_modes_adjoint_async",
         "few_plan_adjoint_workspace_bytes", "few_waveform_adjoint_plan_async",
-        "few_spline_eval_async", "few_direct_mode_sum_async",
+        "few_spline_eval_async", "few_direct_mode_sum_async", "few_fd_workspace_bytes",
+        "few_get_waveform_generic_fd_async", "few_get_waveform_generic_fd",
     ]

@@ -17,6 +17,7 @@
 
 #include "adjoint.hpp"
 #include "eval.hpp"
+#include "fd.hpp"
 #include "few_common.hpp"
 #include "few_rocm.h"
 #include "modesum_direct.hpp"
@@ -843,6 +844,114 @@ int few_direct_mode_sum_async(const void *teuk, const void *ylms, const double *
                    phi_theta, phi_r, l, m, k, n, num_samples, num_modes,
                    static_cast<double2 *>(out)};
     return launch_dense_sum(a, reinterpret_cast<hipStream_t>(stream));
+}
+
+int few_fd_workspace_bytes(int num_teuk_modes, int num_segments, size_t *bytes)
+{
+    if (!bytes || num_teuk_modes < 0 || num_segments < 0) {
+        set_error("fd_workspace_bytes: invalid arguments");
+        return FEW_ERR_ARG;
+    }
+    *bytes = fd_workspace_bytes(num_teuk_modes, num_segments);
+    return FEW_OK;
+}
+
+static int check_fd_args(const void *waveform, const double *interp_array,
+                         const double *phase_interp_t, const double *phase_interp_coeffs,
+                         const int *m, const int *k, const int *n, int num_teuk_modes,
+                         const double *old_time_arr, int init_length, int data_length,
+                         const double *frequencies, const int *seg_start_inds,
+                         const int *seg_end_inds, int num_segments, const void *ylm_all,
+                         int zero_index)
+{
+    if (!waveform || !interp_array || !phase_interp_t || !phase_interp_coeffs || !m || !k || !n ||
+        !old_time_arr || !frequencies || !seg_start_inds || !seg_end_inds || !ylm_all) {
+        set_error("get_waveform_generic_fd: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (num_teuk_modes < 0 || init_length < 2 || data_length < 1 || num_segments < 0 ||
+        num_segments > init_length - 1 || zero_index < 0 || zero_index >= data_length) {
+        set_error("get_waveform_generic_fd: invalid sizes (modes %d, init_length %d, "
+                  "data_length %d, segments %d, zero index %d)",
+                  num_teuk_modes, init_length, data_length, num_segments, zero_index);
+        return FEW_ERR_ARG;
+    }
+    return FEW_OK;
+}
+
+int few_get_waveform_generic_fd_async(void *waveform, const double *interp_array,
+                                      const double *phase_interp_t,
+                                      const double *phase_interp_coeffs, const int *m,
+                                      const int *k, const int *n, int num_teuk_modes,
+                                      double delta_t, const double *old_time_arr, int init_length,
+                                      int data_length, const double *frequencies,
+                                      const int *seg_start_inds, const int *seg_end_inds,
+                                      int num_segments, const void *ylm_all, int zero_index,
+                                      int include_minus_m, int separate_modes, void *workspace,
+                                      size_t workspace_bytes, void *stream)
+{
+    (void)delta_t;  // carried by the reference's interface, unused by its kernel
+    int rc = check_fd_args(waveform, interp_array, phase_interp_t, phase_interp_coeffs, m, k, n,
+                           num_teuk_modes, old_time_arr, init_length, data_length, frequencies,
+                           seg_start_inds, seg_end_inds, num_segments, ylm_all, zero_index);
+    if (rc) return rc;
+    if (!workspace) {
+        set_error("get_waveform_generic_fd_async: null workspace");
+        return FEW_ERR_ARG;
+    }
+    FdArgs a;
+    a.wave = static_cast<double2 *>(waveform);
+    a.interp = interp_array;
+    a.phase_t = phase_interp_t;
+    a.pcoef = phase_interp_coeffs;
+    a.m = m;
+    a.k = k;
+    a.n = n;
+    a.t_knots = old_time_arr;
+    a.freqs = frequencies;
+    a.seg_start = seg_start_inds;
+    a.seg_end = seg_end_inds;
+    a.ylm = static_cast<const double2 *>(ylm_all);
+    a.prefix = nullptr;
+    a.K = num_teuk_modes;
+    a.L = init_length;
+    a.nseg = num_segments;
+    a.data_length = data_length;
+    a.zero_index = zero_index;
+    a.include_minus_m = include_minus_m && !separate_modes;
+    a.separate_modes = separate_modes;
+    return launch_fd_sum(a, workspace, workspace_bytes, reinterpret_cast<hipStream_t>(stream));
+}
+
+int few_get_waveform_generic_fd(void *waveform, double *interp_array, double *phase_interp_t,
+                                double *phase_interp_coeffs, int *m, int *k, int *n,
+                                int num_teuk_modes, double delta_t, double *old_time_arr,
+                                int init_length, int data_length, double *frequencies,
+                                int *seg_start_inds, int *seg_end_inds, int num_segments,
+                                void *ylm_all, int zero_index, int include_minus_m,
+                                int separate_modes, void *stream)
+{
+    int rc = check_fd_args(waveform, interp_array, phase_interp_t, phase_interp_coeffs, m, k, n,
+                           num_teuk_modes, old_time_arr, init_length, data_length, frequencies,
+                           seg_start_inds, seg_end_inds, num_segments, ylm_all, zero_index);
+    if (rc) return rc;
+    int cur = 0;
+    FEW_HIP(hipGetDevice(&cur));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    const size_t need = fd_workspace_bytes(num_teuk_modes, num_segments);
+    void *ws = nullptr;
+    rc = library_workspace(cur, need, &ws);
+    if (rc) return rc;
+    rc = few_get_waveform_generic_fd_async(waveform, interp_array, phase_interp_t,
+                                           phase_interp_coeffs, m, k, n, num_teuk_modes, delta_t,
+                                           old_time_arr, init_length, data_length, frequencies,
+                                           seg_start_inds, seg_end_inds, num_segments, ylm_all,
+                                           zero_index, include_minus_m, separate_modes, ws, need,
+                                           stream);
+    if (rc) return rc;
+    FEW_HIP(hipStreamSynchronize(s));
+    return FEW_OK;
 }
 
 int few_release_workspace(void)

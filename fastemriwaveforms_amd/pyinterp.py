@@ -1,8 +1,9 @@
 """Backend methods with the reference's call signatures (src/few/cutils/pyinterp.pyx:25-92).
 
-``interpolate_arrays_wrap`` and ``get_waveform_wrap`` accept exactly the
-positional arguments the reference's Cython wrappers take, so FEW's
-``CubicSplineInterpolant`` / ``InterpolatedModeSum`` call them unchanged.
+``interpolate_arrays_wrap``, ``get_waveform_wrap`` and
+``get_waveform_generic_fd_wrap`` accept exactly the positional arguments the
+reference's Cython wrappers take, so FEW's ``CubicSplineInterpolant`` /
+``InterpolatedModeSum`` / ``FDInterpolatedModeSum`` call them unchanged.
 Array arguments may be
 
 * torch tensors on the GPU -- passed by device pointer (no copies), or
@@ -135,5 +136,41 @@ def get_waveform_wrap(*args, **kwargs):
 
 
 def get_waveform_generic_fd_wrap(*args, **kwargs):
-    """The frequency-domain sum (pyinterp.pyx:64-92) is not part of this backend."""
-    raise NotImplementedError("get_waveform_generic_fd is not implemented by the rocm backend")
+    """get_waveform_generic_fd(waveform, interp_array, phase_interp_t, phase_interp_coeffs,
+    m_arr_in, k_arr_in, n_arr_in, num_teuk_modes, delta_t, old_time_arr, init_length,
+    data_length, frequencies, mode_start_inds, mode_end_inds, num_segments, Ylm_all,
+    zero_index, include_minus_m, separate_modes).
+
+    src/few/cutils/pyinterp.pyx:64-92.  waveform is accumulated into in place
+    (overwritten per mode with separate_modes).
+    """
+    (waveform, interp_array, phase_interp_t, phase_interp_coeffs, m_arr_in, k_arr_in, n_arr_in,
+     num_teuk_modes, delta_t, old_time_arr, init_length, data_length, frequencies,
+     mode_start_inds, mode_end_inds, num_segments, Ylm_all, zero_index, include_minus_m,
+     separate_modes) = args
+    K, L, Nf, S = int(num_teuk_modes), int(init_length), int(data_length), int(num_segments)
+    wave = _Staged(waveform, torch.complex128, True)
+    interp = _Staged(interp_array, torch.float64, False)
+    pt = _Staged(phase_interp_t, torch.float64, False)
+    pcoef = _Staged(phase_interp_coeffs, torch.float64, False)
+    m = _Staged(m_arr_in, torch.int32, False)
+    k = _Staged(k_arr_in, torch.int32, False)
+    n = _Staged(n_arr_in, torch.int32, False)
+    tk = _Staged(old_time_arr, torch.float64, False)
+    fr = _Staged(frequencies, torch.float64, False)
+    s0 = _Staged(mode_start_inds, torch.int32, False)
+    s1 = _Staged(mode_end_inds, torch.int32, False)
+    ylm = _Staged(Ylm_all, torch.complex128, False)
+    if (wave.dev.numel() < (K * Nf if separate_modes else Nf)
+            or interp.dev.numel() < 4 * (2 * K + 3) * L or pt.dev.numel() < L
+            or pcoef.dev.numel() < 8 * 3 * (L - 1) or min(m.dev.numel(), k.dev.numel(),
+                                                          n.dev.numel()) < K
+            or tk.dev.numel() < S + 1 or fr.dev.numel() < Nf
+            or min(s0.dev.numel(), s1.dev.numel()) < K * S or ylm.dev.numel() < 2 * K):
+        raise ValueError("get_waveform_generic_fd: array smaller than the sizes imply")
+    rc = lib().few_get_waveform_generic_fd(
+        wave.ptr, interp.ptr, pt.ptr, pcoef.ptr, m.ptr, k.ptr, n.ptr, K, float(delta_t), tk.ptr,
+        L, Nf, fr.ptr, s0.ptr, s1.ptr, S, ylm.ptr, int(zero_index), int(bool(include_minus_m)),
+        int(bool(separate_modes)), _stream_ptr())
+    check(rc, "get_waveform_generic_fd")
+    wave.finish()

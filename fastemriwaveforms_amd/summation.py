@@ -3,6 +3,8 @@
 * :class:`CubicSplineInterpolant` -- summation/interpolatedmodesum.py:12-257
 * :class:`SummationBase`          -- summation/base.py:5-144
 * :class:`InterpolatedModeSum`    -- summation/interpolatedmodesum.py:260-375
+* :class:`DirectModeSum`          -- summation/directmodesum.py:24-126
+  (the frequency-domain sum is in :mod:`fastemriwaveforms_amd.fdinterp`)
 
 Same constructor / call signatures, argument meanings and errors as the
 reference.  Arrays live on the GPU as torch tensors (numpy inputs are moved
@@ -173,10 +175,9 @@ class SummationBase(ParallelModuleBase):
             raise ValueError(
                 "{} waveform domain not available. Choices are 'td' (time domain) or 'fd' "
                 "(frequency domain).".format(output_type))
-        if output_type == "fd":
-            # the frequency-domain sum is outside this backend
-            raise NotImplementedError("output_type='fd' is not implemented by the rocm backend")
         self.output_type = output_type
+        if self.output_type == "fd":
+            self.odd_len = True
         self.num_pts = None
         self.num_pts_pad = None
         self.waveform = None
@@ -200,8 +201,18 @@ class SummationBase(ParallelModuleBase):
         self.dt = dt
         if self.odd_len and (self.num_pts + self.num_pts_pad) % 2 == 0:
             self.num_pts_pad = self.num_pts_pad + 1
-        self.waveform = torch.zeros(self.num_pts + self.num_pts_pad, dtype=torch.complex128,
-                                    device=_device())
+        nwave = self.num_pts + self.num_pts_pad
+        if self.output_type == "fd" and kwargs.get("f_arr") is not None:
+            # user frequencies (summation/base.py:117-127), dt taken from them as there
+            frequency = as_device(kwargs["f_arr"], torch.float64)
+            dt = float(frequency.max() / 2)
+            if not bool((frequency == torch.sort(frequency).values).all()):
+                raise ValueError("Frequency array must be sorted in ascending order.")
+            dfs = torch.diff(frequency)
+            if not bool(torch.allclose(dfs, dfs[0].expand_as(dfs), rtol=1e-10)):
+                raise ValueError("Frequency array must be equally spaced.")
+            nwave = len(frequency)
+        self.waveform = torch.zeros(nwave, dtype=torch.complex128, device=_device())
         self.sum(t, *args, dt=dt, **kwargs)
         return self.waveform
 

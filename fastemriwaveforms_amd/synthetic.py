@@ -10,8 +10,9 @@ cutils/global.h:23):
 * batch    -- :func:`make_batch`: uneven durations / knots / mode counts.
 
 Distributions: knots t0 = 0, t_{L-1} = T, interior sorted uniform draws.
-f(t) = 1e-3 + 2e-3 (3 tau^2 - 2 tau^3) Hz with tau = t/T and
-Phi_phi = 2 pi int f (analytic quartic); Phi_r = kappa Phi_phi with
+f(t) = 1e-3 + 2e-3 (3 tau^2 - 2 tau^3) Hz with tau = t/T (or, with
+chirp="rising", 1e-3 + 1e-3 (tau + tau^2 - tau^3 / 2)) and Phi_phi = 2 pi int f (analytic);
+the frequencies at the knots are kept in ``freqs``; Phi_r = kappa Phi_phi with
 kappa ~ U(0.8, 0.95); Phi_theta = 0.  Knot amplitudes
 CN(0,1) 10^(-|n|/10) 10^(-(l-2)/3) (1 + sum_j a_j tau^j), a_j ~ N(0, 0.1^2).
 Y+ and Y- have unit modulus and uniform phase.
@@ -48,6 +49,7 @@ class ModeSumInput:
     T_years: float
     num_pts: int
     name: str = ""
+    freqs: np.ndarray | None = None  # (3, L) f_phi, f_theta, f_r [Hz] at the knots
 
     @property
     def L(self) -> int:
@@ -89,14 +91,26 @@ def num_points(t_first: float, t_last: float, T_years: float, dt: float) -> int:
     return int((t_last - t_first) / dt) + 1
 
 
-def phase_phi(t: np.ndarray, Ts: float) -> np.ndarray:
-    """2 pi int_0^t f, f = 1e-3 + 2e-3 (3 tau^2 - 2 tau^3), tau = t/Ts."""
+def phase_phi(t: np.ndarray, Ts: float, chirp: str = "smooth") -> np.ndarray:
+    """2 pi int_0^t f with f = 1e-3 + 2e-3 (3 tau^2 - 2 tau^3) ("smooth", f' = 0
+    at both ends) or f = 1e-3 + 1e-3 (tau + tau^2 - tau^3 / 2) ("rising": f' > 0
+    throughout and a cubic term in every knot interval), tau = t/Ts."""
     tau = t / Ts
+    if chirp == "rising":
+        return 2.0 * np.pi * (1e-3 * t + 1e-3 * Ts * (tau ** 2 / 2 + tau ** 3 / 3 - tau ** 4 / 8))
     return 2.0 * np.pi * (1e-3 * t + 2e-3 * Ts * (tau ** 3 - 0.5 * tau ** 4))
 
 
+def freq_phi(t: np.ndarray, Ts: float, chirp: str = "smooth") -> np.ndarray:
+    """f = d phase_phi / dt / (2 pi) [Hz]."""
+    tau = t / Ts
+    if chirp == "rising":
+        return 1e-3 + 1e-3 * (tau + tau ** 2 - 0.5 * tau ** 3)
+    return 1e-3 + 2e-3 * (3 * tau ** 2 - 2 * tau ** 3)
+
+
 def make_input(T_years: float, dt: float, L: int, mode_kind: str, keep, seed: int,
-               minus_m_ylm: bool = True, name: str = "") -> ModeSumInput:
+               minus_m_ylm: bool = True, name: str = "", chirp: str = "smooth") -> ModeSumInput:
     rng = np.random.default_rng(seed)
     Ts = T_years * YRSID_SI
     interior = np.sort(rng.uniform(0.0, 1.0, L - 2)) * Ts
@@ -119,14 +133,16 @@ def make_input(T_years: float, dt: float, L: int, mode_kind: str, keep, seed: in
     if not minus_m_ylm:
         ym = np.zeros(K, dtype=np.complex128)
     kappa = rng.uniform(0.8, 0.95)
-    Pphi = phase_phi(t, Ts)
+    Pphi = phase_phi(t, Ts, chirp)
     phase_knots = np.stack([Pphi, np.zeros_like(Pphi), kappa * Pphi])
+    fphi = freq_phi(t, Ts, chirp)
+    freqs = np.stack([fphi, np.zeros_like(fphi), kappa * fphi])
     return ModeSumInput(
         t=t, teuk=np.ascontiguousarray(teuk.astype(np.complex128)),
         ylms=np.concatenate([yp, ym]).astype(np.complex128),
         phase_knots=np.ascontiguousarray(phase_knots), l=l, m=m,
         k=np.zeros(K, np.int32), n=n, dt=float(dt), T_years=float(T_years),
-        num_pts=num_points(t[0], t[-1], T_years, dt), name=name)
+        num_pts=num_points(t[0], t[-1], T_years, dt), name=name, freqs=freqs)
 
 
 def make_workload(name: str, seed: int | None = None) -> ModeSumInput:

@@ -284,6 +284,53 @@ int few_direct_mode_sum_async(const void *teuk, const void *ylms, const double *
                               const int *m, const int *k, const int *n, int num_samples,
                               int num_modes, void *out, void *stream);
 
+/* ---- frequency-domain interpolated sum --------------------------------- */
+
+/* Bytes of workspace few_get_waveform_generic_fd_async needs (the per-(mode,
+ * segment) bin counts and their prefix): written to *bytes. */
+int few_fd_workspace_bytes(int num_teuk_modes, int num_segments, size_t *bytes);
+
+/* Replaces get_waveform_generic_fd (src/few/cutils/interpolate.hh:33-38,
+ * bound at src/few/cutils/pyinterp.pyx:64-92; kernel interpolate.cu:1142-1500).
+ * Same arguments, all device pointers:
+ *   waveform        (data_length,) complex128, accumulated into; with
+ *                   separate_modes (num_teuk_modes, data_length), each mode's
+ *                   bins overwritten
+ *   interp_array    (4, 2K + 3, init_length) spline coefficients of Re A, Im A,
+ *                   f_phi, f_theta, f_r (fdinterp.py:508-512 layout)
+ *   phase_interp_t  (init_length,)  phase_interp_coeffs (8, 3, init_length - 1)
+ *   m, k, n         (K,) int32      old_time_arr (init_length,) knot times
+ *   frequencies     (data_length,)  seg_start_inds / seg_end_inds (K, num_segments)
+ *                   int32 inclusive frequency-index ranges
+ *   ylm_all         (2K,) complex128, Y- already times (-1)^l
+ *   zero_index      index of f = 0; the -m term of bin i lands in bin 2 zero - i
+ * delta_t is accepted and unused, as in the reference kernel.  Differences
+ * from the reference, both on inputs the reference mishandles: bins outside
+ * [0, data_length) are skipped before the frequency is read, and a -m bin
+ * outside the array is dropped instead of written out of bounds.  Colliding
+ * bins are added with FP64 atomics, so their summation order is not fixed
+ * (as with the reference's atomic adds).  Stream-ordered. */
+int few_get_waveform_generic_fd_async(void *waveform, const double *interp_array,
+                                      const double *phase_interp_t,
+                                      const double *phase_interp_coeffs, const int *m,
+                                      const int *k, const int *n, int num_teuk_modes,
+                                      double delta_t, const double *old_time_arr, int init_length,
+                                      int data_length, const double *frequencies,
+                                      const int *seg_start_inds, const int *seg_end_inds,
+                                      int num_segments, const void *ylm_all, int zero_index,
+                                      int include_minus_m, int separate_modes, void *workspace,
+                                      size_t workspace_bytes, void *stream);
+
+/* The reference's synchronous call shape (interpolate.hh:33-38): library-owned
+ * workspace, returns after the stream has finished. */
+int few_get_waveform_generic_fd(void *waveform, double *interp_array, double *phase_interp_t,
+                                double *phase_interp_coeffs, int *m, int *k, int *n,
+                                int num_teuk_modes, double delta_t, double *old_time_arr,
+                                int init_length, int data_length, double *frequencies,
+                                int *seg_start_inds, int *seg_end_inds, int num_segments,
+                                void *ylm_all, int zero_index, int include_minus_m,
+                                int separate_modes, void *stream);
+
 /* Frees the library-owned workspaces of the synchronous drop-in entries. */
 int few_release_workspace(void);
 

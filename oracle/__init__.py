@@ -17,6 +17,16 @@ that file):
 * ``cubic_to_dense``      -- cubic piece -> 8-coefficient dense-output form
   consumed at src/few/cutils/interpolate.cu:642.
 
+``few_oracle_fd.c`` restates the frequency-domain sum:
+
+* ``fd_waveform``         -- src/few/cutils/interpolate.cu:1142-1563 (CPU branch),
+* ``fd_spa_func`` / ``fd_cube_roots`` -- its building blocks
+  (interpolate.cu:1041-1090, 1113-1134), pinned against scipy's
+  ``kv(1/3, .)`` and ``numpy.roots``.
+
+``fd_segment_inds`` and ``fd_post`` restate fdinterp.py's Python glue
+(summation/fdinterp.py:202-465 and 514-542) in numpy.
+
 Python-level helpers restating the reference's Python glue live here too:
 ``spline_eval`` (summation/interpolatedmodesum.py:161-257) and
 ``num_points`` (summation/base.py:83-102).
@@ -70,6 +80,14 @@ def lib() -> ctypes.CDLL:
         _lib.orc_waveform_adjoint.restype = ctypes.c_int
         _lib.orc_spline_adjoint.argtypes = [d, d, ctypes.c_int, ctypes.c_int, d]
         _lib.orc_spline_adjoint.restype = ctypes.c_int
+        _lib.orc_fd_waveform.argtypes = [d, d, d, d, i, i, i, ctypes.c_int, d, ctypes.c_int,
+                                         ctypes.c_int, d, i, i, ctypes.c_int, d, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int]
+        _lib.orc_fd_waveform.restype = ctypes.c_int
+        _lib.orc_fd_spa_func.argtypes = [d, ctypes.c_long, d]
+        _lib.orc_fd_spa_func.restype = None
+        _lib.orc_fd_cube_roots.argtypes = [d, ctypes.c_long, d]
+        _lib.orc_fd_cube_roots.restype = None
         _lib.orc_max_threads.argtypes = []
         _lib.orc_max_threads.restype = ctypes.c_int
     return _lib
@@ -224,6 +242,139 @@ def spline_adjoint(t, g_coef):
 
 def max_threads() -> int:
     return int(lib().orc_max_threads())
+
+
+def fd_waveform(interp_fd, phase_t, phase_coeffs_fd, m, k, n, t, frequencies, start_inds,
+                end_inds, ylms, zero_index, include_minus_m=True, separate_modes=False):
+    """get_waveform_generic_fd's CPU branch (interpolate.cu:1142-1563).
+
+    interp_fd: (4, 2K+3, L); phase_coeffs_fd: (8, 3, L-1); start/end_inds:
+    (K, L-1) inclusive frequency-index ranges; ylms: (2K,) complex (the -m
+    half already multiplied by (-1)^l).  Returns the complex waveform, (Nf,)
+    or (K, Nf) with separate_modes."""
+    interp_fd = _f64(interp_fd)
+    _, nint, L = interp_fd.shape
+    K = (nint - 3) // 2
+    Nf = len(frequencies)
+    wave = np.zeros((K, Nf) if separate_modes else (Nf,), dtype=np.complex128)
+    ylm = np.ascontiguousarray(np.asarray(ylms, dtype=np.complex128))
+    rc = lib().orc_fd_waveform(
+        _dp(wave.view(np.float64)), _dp(interp_fd), _dp(_f64(phase_t)), _dp(_f64(phase_coeffs_fd)),
+        _ip(_i32(m)), _ip(_i32(k)), _ip(_i32(n)), K, _dp(_f64(t)), L, Nf, _dp(_f64(frequencies)),
+        _ip(_i32(start_inds)), _ip(_i32(end_inds)), L - 1, _dp(ylm.view(np.float64)),
+        int(zero_index), int(bool(include_minus_m)), int(bool(separate_modes)))
+    if rc != 0:
+        raise RuntimeError(f"oracle fd_waveform failed: {rc}")
+    return wave
+
+
+def fd_spa_func(x):
+    """SPAFunc (interpolate.cu:1041-1090) at each x."""
+    x = _f64(np.atleast_1d(x))
+    out = np.zeros(len(x), dtype=np.complex128)
+    lib().orc_fd_spa_func(_dp(x), len(x), _dp(out.view(np.float64)))
+    return out
+
+
+def fd_cube_roots(abcd):
+    """cube_roots (interpolate.cu:1113-1134) of rows (a, b, c, d) -> (n, 3) complex."""
+    abcd = _f64(np.atleast_2d(abcd))
+    out = np.zeros((len(abcd), 3), dtype=np.complex128)
+    lib().orc_fd_cube_roots(_dp(abcd), len(abcd), _dp(out.view(np.float64)))
+    return out
+
+
+def fd_frequency_grid(num_pts, num_pts_pad, dt):
+    """summation/fdinterp.py:412-421 -- the default frequency array."""
+    Len = num_pts + num_pts_pad
+    return np.hstack((np.arange(-(Len // 2), 0), np.arange(0, (Len - 1) // 2 + 1))) / (Len * dt)
+
+
+def fd_segment_inds(t, f_phi, f_theta, f_r, m, k, n, frequency, f_arr_given=False):
+    """summation/fdinterp.py:241-465 in numpy: (start_inds, end_inds), each
+    (K, L-1) int, plus the zero index.
+
+    Follows the reference step by step, including its quirks: the turnover
+    correction is applied only when exactly one turnover exists in total (with
+    more, concatenating a (n, 2) with a (1, n) array raises ValueError, which
+    the reference catches and ignores), and the indices are truncated with
+    astype(int) after abs()."""
+    t = _f64(t)
+    m = np.asarray(m, dtype=np.int64)
+    k = np.asarray(k, dtype=np.int64)
+    n = np.asarray(n, dtype=np.int64)
+    y_freqs = np.asarray([f_phi, f_theta, f_r], dtype=np.float64)
+    fs, _, _ = interpolate_arrays(t, y_freqs)
+    t_new = np.linspace(t.min(), t.max(), 5000)
+    new_freqs = spline_eval(np.tile(t, (3, 1)), fs, t_new)
+    seg_freqs = m[:, None] * f_phi[None, :] + k[:, None] * f_theta[None, :] + n[:, None] * f_r[None, :]
+    segs = np.sort(np.asarray([seg_freqs[:, :-1], seg_freqs[:, 1:]]).transpose(1, 2, 0), axis=-1)
+    new_freqs_per_mode = (m[:, None] * new_freqs[0][None, :] + k[:, None] * new_freqs[1][None, :]
+                          + n[:, None] * new_freqs[2][None, :])
+    t_of_max = t_new[new_freqs_per_mode.argmax(axis=-1)]
+    t_of_min = t_new[new_freqs_per_mode.argmin(axis=-1)]
+    checks = [np.where((t_of_max > 0.0) & (t_of_max < t[-1]))[0],
+              np.where((t_of_min > 0.0) & (t_of_min < t[-1]))[0]]
+    fix_list = []
+    if len(checks[0]) > 0:
+        fix_list.append(np.searchsorted(t, t_of_max[checks[0]]) - 1)
+    if len(checks[1]) > 0:
+        fix_list.append(np.searchsorted(t, t_of_min[checks[1]]) - 1)
+    c3, c2, c1, y0 = fs[3], fs[2], fs[1], fs[0]  # (3, L) each
+    try:
+        fix = np.concatenate(fix_list)
+        chk = np.concatenate(checks)
+        a = m[chk] * c3[0, fix] + k[chk] * c3[1, fix] + n[chk] * c3[2, fix]
+        b = m[chk] * c2[0, fix] + k[chk] * c2[1, fix] + n[chk] * c2[2, fix]
+        c = m[chk] * c1[0, fix] + k[chk] * c1[1, fix] + n[chk] * c1[2, fix]
+        d = m[chk] * y0[0, fix] + k[chk] * y0[1, fix] + n[chk] * y0[2, fix]
+        inner = (2 * b) ** 2 - 4 * (3 * a) * c
+        if np.any(inner < -1e10):
+            raise RuntimeError("turnover discriminant below -1e10 (the reference stops here)")
+        inner[inner < 0.0] = 0.0
+        r1 = (-(2 * b) + np.sqrt(inner)) / (2 * (3 * a))
+        r2 = (-(2 * b) - np.sqrt(inner)) / (2 * (3 * a))
+        tr1 = t[fix] + r1
+        tr2 = t[fix] + r2
+        keep1 = (t[fix] < tr1) & (t[fix + 1] > tr1)
+        keep2 = (t[fix] < tr2) & (t[fix + 1] > tr2)
+        if np.any(keep1 & keep2):
+            raise RuntimeError("two turnover roots in one segment (the reference stops here)")
+        elif not np.any(keep1 | keep2):
+            pass
+        else:
+            t_fix = tr1 * keep1 + tr2 * keep2
+            x_fix = t_fix - t[fix]
+            f_ext = a * x_fix**3 + b * x_fix**2 + c * x_fix + d
+            srt = np.sort(np.concatenate([segs[chk, fix], np.array([f_ext])], axis=-1), axis=-1)
+            segs[chk, fix] = srt[:, np.array([0, 2])]
+    except ValueError:
+        pass
+    assert 1 == np.sum(frequency == 0.0)
+    ind_zero = np.where(frequency == 0)[0][0]
+    assert ind_zero == int(len(frequency) / 2)
+    first = frequency[0]
+    df = frequency[1] - frequency[0]
+    inds_check = np.abs((segs - first) / df).astype(int)
+    if f_arr_given:
+        mask_pos = segs > 0.0
+        pos = np.abs((segs - frequency[ind_zero + 1]) / df).astype(int) + ind_zero + 1
+        inds_check[mask_pos] = pos[mask_pos]
+    start_inds = (inds_check[:, :, 0].copy() + 1).astype(int)
+    end_inds = inds_check[:, :, 1].copy().astype(int)
+    zero_index = int(np.where(frequency == 0.0)[0][0])
+    return start_inds, end_inds, zero_index
+
+
+def fd_post(waveform, frequency, mask_positive=False):
+    """summation/fdinterp.py:526-542 -- h+ and hx from the two-sided sum."""
+    fd_sig = -np.flip(waveform)
+    p = np.real(fd_sig + np.flip(fd_sig)) / 2.0 + 1j * np.imag(fd_sig - np.flip(fd_sig)) / 2.0
+    c = -np.imag(fd_sig + np.flip(fd_sig)) / 2.0 + 1j * np.real(fd_sig - np.flip(fd_sig)) / 2.0
+    if mask_positive:
+        mask = frequency >= 0.0
+        return np.vstack((p[mask], c[mask]))
+    return np.vstack((p, c))
 
 
 # ---------------------------------------------------------------------------

@@ -41,3 +41,37 @@ def spread_indices(N, L_bounds=None, count=2000, seed=0):
         for b in L_bounds:
             parts.append(np.clip(np.arange(b - 2, b + 2), 0, N - 1))
     return np.unique(np.concatenate(parts)).astype(np.int64)
+
+
+def fd_frequencies(inp):
+    """Fundamental frequencies [Hz] at the knots matching the synthetic phases."""
+    return inp.freqs[0], inp.freqs[1], inp.freqs[2]
+
+
+def oracle_fd_inputs(inp):
+    """(interp (4, 2K+3, L), phase coefficients (8, 3, L-1)) as fdinterp.py hands
+    them to get_waveform_generic_fd (summation/fdinterp.py:196-205, 508-519)."""
+    f_phi, f_theta, f_r = fd_frequencies(inp)
+    y_all = np.concatenate([inp.teuk.T.real, inp.teuk.T.imag, f_phi[None], f_theta[None],
+                            f_r[None]])
+    interp, _, info = oracle.interpolate_arrays(inp.t, y_all)
+    assert info == 0
+    pc, _, info = oracle.interpolate_arrays(inp.t, inp.phase_knots)
+    assert info == 0
+    dense = oracle.cubic_to_dense(pc, inp.t)           # (8, L-1, 3)
+    return np.ascontiguousarray(interp), np.ascontiguousarray(np.transpose(dense, (0, 2, 1)))
+
+
+def oracle_fd(inp, frequency, f_arr_given=False, include_minus_m=True, separate_modes=False):
+    """The whole frequency-domain sum through the oracle: (two-sided sum,
+    (start, end, zero_index))."""
+    f_phi, f_theta, f_r = fd_frequencies(inp)
+    interp, pcoef = oracle_fd_inputs(inp)
+    start, end, zero = oracle.fd_segment_inds(inp.t, f_phi, f_theta, f_r, inp.m, inp.k, inp.n,
+                                              frequency, f_arr_given=f_arr_given)
+    if separate_modes:
+        include_minus_m = False
+    wave = oracle.fd_waveform(interp, inp.t, pcoef, inp.m, inp.k, inp.n, inp.t, frequency,
+                              start, end, inp.ylms, zero, include_minus_m=include_minus_m,
+                              separate_modes=separate_modes)
+    return wave, (start, end, zero)

@@ -14,6 +14,7 @@
 
 #include "spline.hip"
 #include "adjoint.hip"
+#include "fd.hip"
 #include "modesum_direct.hpp"
 #include "few_math.hpp"
 
@@ -183,6 +184,83 @@ int twin_spline_adjoint(const double *t, const double *g_coef, int nsys, int L, 
 void twin_sincos_cw(const double *x, double *s, double *c, int *ok, long n)
 {
     for (long i = 0; i < n; i++) ok[i] = sincos_cw(x[i], s[i], c[i]) ? 1 : 0;
+}
+
+// The frequency-domain sum with the kernel's per-item and per-bin routines,
+// in the CPU branch's loop order (modes, segments, bins, roots) with plain
+// sequential adds: bit-comparable with the oracle.
+int twin_fd_waveform(double *wave, const double *interp, const double *phase_t,
+                     const double *pcoef, const int *m, const int *k, const int *n, int K,
+                     const double *t_knots, int L, int Nf, const double *freqs,
+                     const int *seg_start, const int *seg_end, int nseg, const double *ylm,
+                     int zero_index, int include_minus_m, int separate_modes)
+{
+    FdArgs a;
+    a.wave = reinterpret_cast<double2 *>(wave);
+    a.interp = interp;
+    a.phase_t = phase_t;
+    a.pcoef = pcoef;
+    a.m = m;
+    a.k = k;
+    a.n = n;
+    a.t_knots = t_knots;
+    a.freqs = freqs;
+    a.seg_start = seg_start;
+    a.seg_end = seg_end;
+    a.ylm = reinterpret_cast<const double2 *>(ylm);
+    a.prefix = nullptr;
+    a.K = K;
+    a.L = L;
+    a.nseg = nseg;
+    a.data_length = Nf;
+    a.zero_index = zero_index;
+    a.include_minus_m = include_minus_m && !separate_modes;
+    a.separate_modes = separate_modes;
+    const fcx xi = fd_xi();
+    for (int q = 0; q < K; q++) {
+        for (int j = 0; j < nseg; j++) {
+            const long item = (long)q * nseg + j;
+            int lo, hi;
+            fd_item_range(a, item, lo, hi);
+            if (hi < lo) continue;
+            FdItem it;
+            fd_load_item(a, q, j, it);
+            for (int i = lo; i <= hi; i++) {
+                const int mi = fd_minus_index(zero_index, i);
+                const bool conj_term = (it.mm != 0.0) && a.include_minus_m;
+                fd_bin(it, freqs[i], xi, [&](fcx plus, fcx minus) {
+                    if (separate_modes) {
+                        wave[2 * ((long)q * Nf + i)] = plus.re;
+                        wave[2 * ((long)q * Nf + i) + 1] = plus.im;
+                        return;
+                    }
+                    wave[2L * i] += plus.re;
+                    wave[2L * i + 1] += plus.im;
+                    if (conj_term && mi >= 0 && mi < Nf) {
+                        wave[2L * mi] += minus.re;
+                        wave[2L * mi + 1] += minus.im;
+                    }
+                });
+            }
+        }
+    }
+    return 0;
+}
+
+void twin_fd_cube_roots(const double *abcd, long n, double *out)
+{
+    const fcx xi = fd_xi();
+    for (long i = 0; i < n; i++)
+        fd_cube_roots(abcd[4 * i], abcd[4 * i + 1], abcd[4 * i + 2], abcd[4 * i + 3], xi, out + 3 * i);
+}
+
+void twin_fd_spa(const double *x, long n, double *out)
+{
+    for (long i = 0; i < n; i++) {
+        const fcx r = fd_spa(x[i]);
+        out[2 * i] = r.re;
+        out[2 * i + 1] = r.im;
+    }
 }
 
 void twin_sincos_turns(const double *f, double *s, double *c, long n)
