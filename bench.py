@@ -11,7 +11,10 @@ The headline path is the mixed-precision factorised sum (FP64 phases and
 spline build; the n-contraction on FP16 MFMA with hi/lo split operands and
 FP32 accumulation, or FP32 MFMA when the mode set's tile does not fit LDS);
 the plain FP64 GPU sum (the reference's arithmetic) is timed beside it as the
-denominator of the speed-up.  Prints ONE JSON line.
+denominator of the speed-up.  Further legs: the 256-waveform uneven batch, the
+differentiation (forward + backward), the frequency-domain sum.  The oracle
+(oracle/, test infrastructure) runs only in the CPU leg: as the CPU baseline and
+as the checker of values the GPU legs kept.  Prints ONE JSON line.
 """
 
 from __future__ import annotations
@@ -37,6 +40,14 @@ DIRECT_FLOP = 37  # algorithmic FLOP per mode-sample of the direct sum (sincos c
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+def spread_indices(N, count=2000, seed=0):
+    """Sample indices spread over [0, N): both ends, a uniform grid, random draws."""
+    rng = np.random.default_rng(seed)
+    parts = [np.arange(min(20, N)), np.arange(max(0, N - 20), N),
+             np.linspace(0, N - 1, count // 2).astype(np.int64), rng.integers(0, N, count // 2)]
+    return np.unique(np.concatenate(parts)).astype(np.int64)
 
 
 def cpu_baseline(inp, seconds_target: float = 12.0):
@@ -70,6 +81,37 @@ def cpu_baseline(inp, seconds_target: float = 12.0):
                       "restating interpolate.cu's CPU branch (-O2 -ffp-contract=off), 1 thread"}
 
 
+def cpu_checks(inp, head, batch_chk, fd_chk):
+    """The CPU leg's checker role: the oracle's values at the samples / bins the
+    GPU legs kept, compared with them (rel-L2), and the oracle's frequency-domain
+    sum timed on the same check (1 thread)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    from helpers import oracle_waveform
+    nth = min(16, os.cpu_count() or 1)
+    res = {}
+    if head is not None:
+        idx, h_idx = head
+        res["headline_ref"] = oracle_waveform(inp, idx, nthreads=nth)
+        res["headline"] = oracle.rel_l2(h_idx, res["headline_ref"])
+    if batch_chk is not None:
+        errs = [oracle.rel_l2(vals, oracle_waveform(x, idx, nthreads=nth))
+                for x, idx, vals in batch_chk]
+        res["batch_max_of_8"] = float(max(errs))
+    if fd_chk is not None:
+        c = fd_chk
+        t0 = time.perf_counter()
+        ref = oracle.fd_waveform(c["interp"], c["t"], c["pcoef"], c["m"], c["k"], c["n"], c["t"],
+                                 c["freq"], c["start"], c["end"], c["ylms"], c["zero"])
+        el = time.perf_counter() - t0
+        res["fd"] = oracle.rel_l2(c["gpu"], ref)
+        res["fd_cpu"] = {"value": c["bins"] / el, "unit": "mode-bins/s", "cores": 1,
+                         "kind": "port", "sample": f"{c['modes']} modes x their {c['bins']} "
+                         f"frequency bins of the same workload ({el:.1f} s); "
+                         "oracle/few_oracle_fd.c restating interpolate.cu:1142-1500, 1 thread"}
+    return res
+
+
 def time_path(prob, precision, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -100,8 +142,6 @@ def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256):
     3843 set, dt = 10 s): the whole batch in one launch per stage, replayed as one
     captured HIP graph, variant from the autotuner; FP64 control run beside it."""
     import torch
-    import oracle
-    from helpers import oracle_waveform, spread_indices
     from fastemriwaveforms_amd import autotune, synthetic
     from fastemriwaveforms_amd.pipeline import (BATCH_STAGE_PLAN, BATCH_STAGE_SPLINES,
                                                 BATCH_STAGE_SUM, BatchProblem)
@@ -138,21 +178,19 @@ def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256):
     stage_ms = {"spline": ev[0].elapsed_time(ev[1]), "plan_tiles": ev[1].elapsed_time(ev[2]),
                 "sum": ev[2].elapsed_time(ev[3])}
     h16 = bp.wave.clone()
-    # oracle check on a subset: 8 waveforms spread over the batch, spread samples
-    errs = []
+    # kept for the CPU leg's oracle check: 8 waveforms spread over the batch, spread samples
+    chk = []
     for bi in np.linspace(0, count - 1, 8).astype(int):
         inp = inputs[bi]
         idx = spread_indices(inp.num_pts, count=300, seed=int(bi))
-        ref = oracle_waveform(inp, idx, nthreads=min(16, os.cpu_count() or 1))
-        errs.append(oracle.rel_l2(bp.waveform(int(bi)).cpu().numpy()[idx], ref))
+        chk.append((inp, idx, bp.waveform(int(bi)).cpu().numpy()[idx]))
     out = {"waveforms": count, "samples": bp.samples, "mode_samples": bp.mode_samples,
            "intervals": int(bp.totals[8]), "variant": best["variant"], "graph_ms_per_step": ms_graph,
            "waveforms_per_s": count / (ms_graph * 1e-3),
            "mode_samples_per_s": bp.mode_samples / (ms_graph * 1e-3), "stage_ms": stage_ms,
-           "rel_l2_vs_oracle_max_of_8": float(max(errs))}
+           "rel_l2_vs_oracle_max_of_8": None}
     log(f"[bench] batch mixed16 graph: {ms_graph:.2f} ms/step, {out['waveforms_per_s']:.1f} "
-        f"waveforms/s, {out['mode_samples_per_s']:.4g} mode-samples/s, stages {stage_ms}, "
-        f"rel-L2 (8 waveforms vs oracle) max {max(errs):.3e}")
+        f"waveforms/s, {out['mode_samples_per_s']:.4g} mode-samples/s, stages {stage_ms}")
     if fp64:
         reps = 2
         bp.run("fp64")
@@ -172,7 +210,104 @@ def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256):
             f"mixed16 rel-L2 vs fp64 (all samples) {float(diff):.3e}")
     del bp, h16, g
     torch.cuda.empty_cache()
-    return out
+    return out, chk
+
+
+def fd_leg(dev, reps: int = 3, check_modes: int = 6):
+    """Frequency-domain sum (FDInterpolatedModeSum, get_waveform_generic_fd) at the
+    full1 sizes -- T = 1 yr, dt = 10 s (3,155,815 bins), 120 knots, 3843 modes --
+    with the "rising" synthetic chirp (monotonic mode frequencies, as an inspiral's).
+    Timed end to end (splines, bin bookkeeping, the sum, h+/hx) and the sum kernel
+    alone; a few modes' bins are kept for the CPU leg's oracle check."""
+    import ctypes
+    import torch
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd._lib import check, lib
+    from fastemriwaveforms_amd.fdinterp import FDInterpolatedModeSum, segment_frequency_inds
+    from fastemriwaveforms_amd.pipeline import ModeSumProblem
+    from fastemriwaveforms_amd.summation import CubicSplineInterpolant
+    inp = synthetic.make_input(1.0, 10.0, 120, "m_nonneg_30", None, 1001, name="full1-fd",
+                               chirp="rising")
+    K, L = inp.K, inp.L
+    prob = ModeSumProblem(inp, dev)
+    prob.build_splines()
+    phase_coeffs = prob.pcoef.permute(1, 2, 0).contiguous()      # (L-1, 3, 8)
+    ylms_in = inp.ylms.copy()
+    ylms_in[K:] *= (-1.0) ** inp.l
+    freqs = tuple(torch.from_numpy(f).to(dev) for f in inp.freqs)
+    summ = FDInterpolatedModeSum(output_type="fd")
+    call = lambda: summ(prob.t, prob.teuk, ylms_in, prob.t, phase_coeffs, inp.l, inp.m, inp.k,
+                        inp.n, T=inp.T_years, dt=inp.dt, frequencies=freqs)
+    call()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    a.record()
+    for _ in range(reps):
+        call()
+    b.record()
+    torch.cuda.synchronize()
+    ms_e2e = a.elapsed_time(b) / reps
+    wall_e2e = (time.perf_counter() - t0) / reps * 1e3
+    # the sum kernel alone on the same inputs
+    freq = summ.frequency
+    Nf = len(freq)
+    y_all = torch.cat([prob.teuk.real.T, prob.teuk.imag.T, freqs[0][None], freqs[1][None],
+                       freqs[2][None]])
+    sp = CubicSplineInterpolant(prob.t, y_all)
+    interp = sp.interp_array.reshape(sp.reshape_shape).permute(0, 2, 1).contiguous()
+    s0, s1, zero = segment_frequency_inds(prob.t, *freqs, inp.m, inp.k, inp.n, freq)
+    pcoef = phase_coeffs.permute(2, 1, 0).contiguous()            # (8, 3, L-1)
+    yl = prob.ylm
+    nb = ctypes.c_size_t(0)
+    check(lib().few_fd_workspace_bytes(K, L - 1, ctypes.byref(nb)), "fd workspace")
+    ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)
+    stream = int(torch.cuda.current_stream().cuda_stream)
+
+    def kernel(wave, st, en):
+        check(lib().few_get_waveform_generic_fd_async(
+            wave.data_ptr(), interp.data_ptr(), prob.t.data_ptr(), pcoef.data_ptr(),
+            prob.m.data_ptr(), prob.k.data_ptr(), prob.n.data_ptr(), K, inp.dt,
+            prob.t.data_ptr(), L, Nf, freq.data_ptr(), st.data_ptr(), en.data_ptr(), L - 1,
+            yl.data_ptr(), zero, 1, 0, ws.data_ptr(), ws.numel(), stream), "fd sum")
+
+    wave = torch.zeros(Nf, dtype=torch.complex128, device=dev)
+    kernel(wave, s0, s1)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(reps):
+        kernel(wave, s0, s1)
+    b.record()
+    torch.cuda.synchronize()
+    ms_k = a.elapsed_time(b) / reps
+    lo = s0.clamp(min=0).long()
+    hi = s1.clamp(max=Nf - 1).long()
+    bins = int((hi - lo + 1).clamp(min=0).sum())
+    out = {"workload": f"full1 sizes in the frequency domain: T={inp.T_years} yr, dt={inp.dt} s, "
+                       f"{Nf} bins, L={L} knots, K={K} modes, rising chirp",
+           "mode_bins": bins, "sum_kernel_ms": ms_k, "mode_bins_per_s": bins / (ms_k * 1e-3),
+           "end_to_end_ms": ms_e2e, "end_to_end_wall_ms": wall_e2e,
+           "waveforms_per_s": 1e3 / ms_e2e, "rel_l2_vs_oracle": None}
+    log(f"[bench] fd: {bins:.4g} mode-bins, sum kernel {ms_k:.1f} ms "
+        f"({out['mode_bins_per_s']:.4g} mode-bins/s), end to end {ms_e2e:.1f} ms")
+    # a few modes for the oracle check (the other modes' ranges emptied)
+    pick = np.linspace(0, K - 1, check_modes).astype(int)
+    st, en = s0.clone(), s1.clone()
+    mask = torch.ones(K, dtype=torch.bool, device=dev)
+    mask[torch.from_numpy(pick).to(dev)] = False
+    st[mask] = 1
+    en[mask] = 0
+    wave.zero_()
+    kernel(wave, st, en)
+    torch.cuda.synchronize()
+    chk_bins = int((en.clamp(max=Nf - 1).long() - st.clamp(min=0).long() + 1).clamp(min=0).sum())
+    chk = {"interp": interp.cpu().numpy(), "t": inp.t, "pcoef": pcoef.cpu().numpy(),
+           "m": inp.m, "k": inp.k, "n": inp.n, "freq": freq.cpu().numpy(),
+           "start": st.cpu().numpy(), "end": en.cpu().numpy(), "ylms": inp.ylms, "zero": zero,
+           "gpu": wave.cpu().numpy(), "bins": chk_bins, "modes": check_modes}
+    del prob, summ, interp, wave, ws
+    torch.cuda.empty_cache()
+    return out, chk
 
 
 def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
@@ -336,6 +471,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiation leg (forward + backward, FD check)")
+    ap.add_argument("--no-fd", action="store_true",
+                    help="skip the frequency-domain sum leg")
     ap.add_argument("--fd-coords", type=int, default=64,
                     help="coordinates of the finite-difference check (0 skips it)")
     ap.add_argument("--grad-batch", type=int, default=16,
@@ -351,9 +488,6 @@ def main():
         sys.exit(2)
 
     import torch
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle
-    from helpers import oracle_waveform, spread_indices
     from fastemriwaveforms_amd import synthetic
     from fastemriwaveforms_amd.pipeline import ModeSumProblem
 
@@ -386,18 +520,21 @@ def main():
         f"prepare {ms['prepare']:.3f} sum {ms['sum']:.3f} ms | {value:.4g} mode-samples/s")
 
     idx = spread_indices(N, count=1500, seed=5)
-    ref = oracle_waveform(inp, idx, nthreads=min(16, os.cpu_count() or 1))
-    err = oracle.rel_l2(h[idx], ref)
-    log(f"[bench] rel-L2 vs oracle ({len(idx)} samples) {err:.3e}")
+    head_chk = (idx, h[idx].copy())
 
     fp64 = None
+    fp64_chk = None
     if not args.no_fp64 and args.precision != "fp64":
         ms64_step, ms64 = time_path(prob, "fp64", max(2, args.steps // 3), 1)
         h64 = prob.wave.cpu().numpy()
+        fa, fb = np.fft.fft(h), np.fft.fft(h64)
+        overlap = np.vdot(fa, fb) / np.sqrt(np.vdot(fa, fa) * np.vdot(fb, fb))
         fp64 = {"ms_per_step": ms64_step, "mode_samples_per_s": K * N / (ms64_step * 1e-3),
-                "kernel_ms": ms64, "rel_l2_vs_oracle": oracle.rel_l2(h64[idx], ref),
-                "headline_rel_l2_vs_fp64_all_samples": oracle.rel_l2(h, h64),
-                "headline_mismatch_vs_fp64": float(oracle.mismatch(h, h64))}
+                "kernel_ms": ms64, "rel_l2_vs_oracle": None,
+                "headline_rel_l2_vs_fp64_all_samples":
+                    float(np.linalg.norm(h - h64) / np.linalg.norm(h64)),
+                "headline_mismatch_vs_fp64": float(1.0 - overlap.real)}
+        fp64_chk = h64[idx].copy()
         log(f"[bench] fp64: step {ms64_step:.3f} ms, speed-up {ms64_step / ms_step:.2f}x, "
             f"headline rel-L2 vs fp64 {fp64['headline_rel_l2_vs_fp64_all_samples']:.3e}")
 
@@ -429,17 +566,38 @@ def main():
                     "pipe": "FP64 VALU", "algorithmic": f"{DIRECT_FLOP} FLOP/mode-sample x {K * N} "
                                                         "mode-samples, sincos counted 0"}
 
-    batch = None
+    batch = batch_chk = None
     if not args.no_batch:
-        batch = batch_leg(dev, max(2, args.steps // 2), 1, not args.no_fp64, args.batch_count)
+        batch, batch_chk = batch_leg(dev, max(2, args.steps // 2), 1, not args.no_fp64,
+                                     args.batch_count)
+
+    fd = fd_chk = None
+    if not args.no_fd:
+        fd, fd_chk = fd_leg(dev)
 
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg(dev, batch=args.grad_batch, fd_coords=args.fd_coords)
 
     cpu = None
+    err = None
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle, 1 thread)")
+        log("[bench] CPU leg: oracle checks and baseline (1 thread)")
+        chk = cpu_checks(inp, head_chk, batch_chk, fd_chk)
+        err = chk["headline"]
+        log(f"[bench] rel-L2 vs oracle ({len(idx)} samples) {err:.3e}")
+        if fp64 is not None:
+            ref = chk["headline_ref"]
+            fp64["rel_l2_vs_oracle"] = float(np.linalg.norm(fp64_chk - ref) / np.linalg.norm(ref))
+        if batch is not None:
+            batch["rel_l2_vs_oracle_max_of_8"] = chk["batch_max_of_8"]
+            log(f"[bench] batch rel-L2 (8 waveforms vs oracle) max {chk['batch_max_of_8']:.3e}")
+        if fd is not None:
+            fd["rel_l2_vs_oracle"] = chk["fd"]
+            fd["cpu_baseline"] = chk["fd_cpu"]
+            fd["speedup_vs_cpu_1thread"] = fd["mode_bins_per_s"] / chk["fd_cpu"]["value"]
+            log(f"[bench] fd rel-L2 vs oracle ({fd_chk['modes']} modes) {chk['fd']:.3e}, "
+                f"CPU {chk['fd_cpu']['value']:.4g} mode-bins/s")
         cpu = cpu_baseline(inp)
         log(f"[bench] CPU {cpu['value']:.4g} mode-samples/s")
 
@@ -467,6 +625,7 @@ def main():
         "speedup_vs_fp64_plain_gpu": (fp64["ms_per_step"] / ms_step) if fp64 else None,
         "batch": batch,
         "adjoint": adjoint,
+        "fd": fd,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

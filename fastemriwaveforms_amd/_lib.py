@@ -72,6 +72,11 @@ def _declare(lib: ctypes.CDLL) -> None:
         "few_get_waveform_generic_fd": ([_d, _d, _d, _d, _d, _d, _d, c_int, c_double, _d, c_int,
                                          c_int, _d, _d, _d, c_int, _d, c_int, c_int, c_int, _d],
                                         c_int),
+        "few_interp2d_workspace_bytes": ([c_int], c_size),
+        "few_interp2d_async": ([_d, _d, c_int, _d, c_int, _d, c_int, c_int, _d, c_int, _d, c_int,
+                                c_int, c_int, _d, c_size, _d], c_int),
+        "few_interp2D_wrap": ([_d, _d, c_int, _d, c_int, _d, c_int, c_int, _d, c_int, _d, c_int,
+                               c_int, c_int, _d], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -114,4 +119,5 @@ def exported_symbols() -> list[str]:
         "few_plan_adjoint_workspace_bytes", "few_waveform_adjoint_plan_async",
         "few_spline_eval_async", "few_direct_mode_sum_async", "few_fd_workspace_bytes",
         "few_get_waveform_generic_fd_async", "few_get_waveform_generic_fd",
+        "few_interp2d_workspace_bytes", "few_interp2d_async", "few_interp2D_wrap",
     ]

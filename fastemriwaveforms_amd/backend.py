@@ -106,7 +106,7 @@ class RocmBackend(Backend):
         _lib.lib()  # raises BackendNotInstalled when libfew_rocm.so is missing
         return BackendMethods(
             pyWaveform=_not_in_backend("pyWaveform"),
-            interp2D=_not_in_backend("interp2D"),
+            interp2D=pyinterp.interp2D,
             interpolate_arrays_wrap=pyinterp.interpolate_arrays_wrap,
             get_waveform_wrap=pyinterp.get_waveform_wrap,
             get_waveform_generic_fd_wrap=pyinterp.get_waveform_generic_fd_wrap,

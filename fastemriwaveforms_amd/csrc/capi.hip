@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "adjoint.hpp"
+#include "amp2d.hpp"
 #include "eval.hpp"
 #include "fd.hpp"
 #include "few_common.hpp"
@@ -949,6 +950,68 @@ int few_get_waveform_generic_fd(void *waveform, double *interp_array, double *ph
                                            seg_start_inds, seg_end_inds, num_segments, ylm_all,
                                            zero_index, include_minus_m, separate_modes, ws, need,
                                            stream);
+    if (rc) return rc;
+    FEW_HIP(hipStreamSynchronize(s));
+    return FEW_OK;
+}
+
+size_t few_interp2d_workspace_bytes(int mx) { return amp2d_workspace_bytes(mx); }
+
+static int check_interp2d(const double *z, const double *tx, int nx, const double *ty, int ny,
+                          const double *c, int kx, int ky, const double *x, int mx,
+                          const double *y, int my, int num_indiv_c, int len_indiv_c)
+{
+    if (!z || !tx || !ty || !c || !x || !y) {
+        set_error("interp2D: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (mx != my) {
+        set_error("interp2D: mx and my must be the same value.");
+        return FEW_ERR_ARG;
+    }
+    if (kx < 0 || ky < 0 || kx > 5 || ky > 5) {
+        set_error("interp2D: spline degrees must be within 0..5 (got %d, %d)", kx, ky);
+        return FEW_ERR_ARG;
+    }
+    if (nx < 2 * (kx + 1) || ny < 2 * (ky + 1)) {
+        set_error("interp2D: too few knots (%d, %d) for degrees (%d, %d)", nx, ny, kx, ky);
+        return FEW_ERR_ARG;
+    }
+    if (mx < 0 || num_indiv_c < 0 ||
+        (long)len_indiv_c < (long)(nx - kx - 1) * (ny - ky - 1)) {
+        set_error("interp2D: invalid sizes (mx %d, sets %d, coefficients per set %d, need %ld)",
+                  mx, num_indiv_c, len_indiv_c, (long)(nx - kx - 1) * (ny - ky - 1));
+        return FEW_ERR_ARG;
+    }
+    return FEW_OK;
+}
+
+int few_interp2d_async(double *z, const double *tx, int nx, const double *ty, int ny,
+                       const double *c, int kx, int ky, const double *x, int mx, const double *y,
+                       int my, int num_indiv_c, int len_indiv_c, void *workspace,
+                       size_t workspace_bytes, void *stream)
+{
+    int rc = check_interp2d(z, tx, nx, ty, ny, c, kx, ky, x, mx, y, my, num_indiv_c, len_indiv_c);
+    if (rc) return rc;
+    return launch_amp2d(z, tx, nx, ty, ny, c, kx, ky, x, mx, y, num_indiv_c, len_indiv_c, workspace,
+                        workspace_bytes, reinterpret_cast<hipStream_t>(stream));
+}
+
+int few_interp2D_wrap(double *z, const double *tx, int nx, const double *ty, int ny, double *c,
+                      int kx, int ky, const double *x, int mx, const double *y, int my,
+                      int num_indiv_c, int len_indiv_c, void *stream)
+{
+    int rc = check_interp2d(z, tx, nx, ty, ny, c, kx, ky, x, mx, y, my, num_indiv_c, len_indiv_c);
+    if (rc) return rc;
+    int cur = 0;
+    FEW_HIP(hipGetDevice(&cur));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    const size_t need = amp2d_workspace_bytes(mx);
+    void *ws = nullptr;
+    rc = library_workspace(cur, need > 0 ? need : 1, &ws);
+    if (rc) return rc;
+    rc = launch_amp2d(z, tx, nx, ty, ny, c, kx, ky, x, mx, y, num_indiv_c, len_indiv_c, ws, need, s);
     if (rc) return rc;
     FEW_HIP(hipStreamSynchronize(s));
     return FEW_OK;

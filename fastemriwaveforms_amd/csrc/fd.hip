@@ -192,9 +192,26 @@ __host__ __device__ inline fcx fcx_pow(fcx x, double y)
     return fcx_exp(fcx_mul(fcx_make(y, 0.0), fcx_log(x)));
 }
 
+// The argument-independent factors of SPAFunc: exp(-2 pi i / 3), exp(-pi i / 3)
+// and exp(-3 pi i / 4) sqrt(pi / 2), computed once per thread with the same
+// operations the reference repeats on every call.
+struct FdSpaConst {
+    fcx e1, e2, pref_asym;
+};
+
+__host__ __device__ inline FdSpaConst fd_spa_const()
+{
+    const fcx II = fcx_make(0.0, 1.0);
+    FdSpaConst k;
+    k.e1 = fcx_exp(fcx_divr(fcx_scale(II, -2. * M_PI), 3.));
+    k.e2 = fcx_exp(fcx_divr(fcx_scale(II, -M_PI), 3.));
+    k.pref_asym = fcx_scale(fcx_exp(fcx_scale(fcx_scale(II, -0.75), M_PI)), sqrt(0.5 * M_PI));
+    return k;
+}
+
 // SPAFunc (interpolate.cu:1041-1090): i sqrt(x) e^{-ix} K_1/3(-ix) by its
 // small-argument series (|x| <= 7) or its asymptotic series.
-__host__ __device__ inline fcx fd_spa(double x)
+__host__ __device__ inline fcx fd_spa(double x, const FdSpaConst &kc)
 {
 #pragma clang fp contract(off)
     const fcx II = fcx_make(0.0, 1.0);
@@ -202,13 +219,13 @@ __host__ __device__ inline fcx fd_spa(double x)
     const double Gamm13 = -4.06235381827920125;  // Gamma(-1/3)
     if (fabs(x) <= 7.) {
         const fcx tmp_p = fcx_pow(fcx_make(x, 0.0), 1. / 6.);
-        fcx pref1 = fcx_mul(fcx_exp(fcx_divr(fcx_scale(II, -2. * M_PI), 3.)), tmp_p);
+        fcx pref1 = fcx_mul(kc.e1, tmp_p);
         pref1 = fcx_mul(pref1, tmp_p);
         pref1 = fcx_mul(pref1, tmp_p);
         pref1 = fcx_mul(pref1, tmp_p);
         pref1 = fcx_mul(pref1, tmp_p);
         pref1 = fcx_divr(fcx_scale(pref1, Gamm13), 1.2599210498948732);  // 2^(1/3)
-        fcx pref2 = fcx_mul(fcx_exp(fcx_divr(fcx_scale(II, -M_PI), 3.)), tmp_p);
+        fcx pref2 = fcx_mul(kc.e2, tmp_p);
         pref2 = fcx_divr(fcx_scale(pref2, Gamp13), 1.5874010519681994748);  // 2^(2/3)
         const double x2 = x * x;
         // Taylor coefficients of the two Bessel-series factors (even powers)
@@ -218,7 +235,7 @@ __host__ __device__ inline fcx fd_spa(double x)
         return fcx_mul(ex, fcx_add(fcx_scale(pref1, ser1), fcx_scale(pref2, ser2)));
     }
     const fcx y = fcx_make(1. / x, 0.0);
-    const fcx pref = fcx_scale(fcx_exp(fcx_scale(fcx_scale(II, -0.75), M_PI)), sqrt(0.5 * M_PI));
+    const fcx pref = kc.pref_asym;
     // asymptotic series c_0 + y (c_1 + ... + y c_8): alternating imaginary/real terms
     fcx ser = fcx_add(fcx_make(-0.87766696951001691647, 0.0), fcx_mul(y, fcx_scale(II, 3.0794530301731669934)));
     ser = fcx_add(fcx_scale(II, -0.29159139923075051147), fcx_mul(y, ser));
@@ -331,7 +348,8 @@ __host__ __device__ inline double fd_fddot(const double *r, double s, double s2,
 // (up to three), each the +f term and the -f (conjugate) term, handed to
 // emit(plus, minus) in root order.
 template <class Emit>
-__host__ __device__ inline void fd_bin(const FdItem &it, double f, fcx xi, Emit emit)
+__host__ __device__ inline void fd_bin(const FdItem &it, double f, fcx xi, const FdSpaConst &kc,
+                                       Emit emit)
 {
 #pragma clang fp contract(off)
     double roots[3];
@@ -359,7 +377,7 @@ __host__ __device__ inline void fd_bin(const FdItem &it, double f, fcx xi, Emit 
                              it.nn * fd_fddot(it.pr, s, s2, s3, s4, w3);
         const double targ = 2. * M_PI * fdot * fdot * fdot / (3. * fddot * fddot);
         const double pre = -1.0 * fdot / fabs(fddot) * 2. / sqrt(3.);
-        const fcx amp2 = fcx_div(fcx_scale(fd_spa(targ), pre), fcx_sqrt(fcx_make(targ, 0.0)));
+        const fcx amp2 = fcx_div(fcx_scale(fd_spa(targ, kc), pre), fcx_sqrt(fcx_make(targ, 0.0)));
         const fcx ph = fcx_exp(fcx_scale(fcx_make(0.0, 1.0), 2. * M_PI * f * t - phase));
         const fcx te = fcx_mul(fcx_mul(fcx_make(Are, Aim), amp2), ph);
         emit(fcx_mul(it.yp, te), fcx_mul(it.ym, fcx_make(te.re, -te.im)));
@@ -457,6 +475,7 @@ __global__ __launch_bounds__(FD_THREADS) void fd_sum_kernel(FdArgs a, long nitem
     const long w1 = min(total, w0 + per);
     if (w0 >= w1) return;
     const fcx xi = fd_xi();
+    const FdSpaConst kc = fd_spa_const();
     long item = -1;
     long item_end = 0, item_begin = 0;
     int lo = 0;
@@ -485,7 +504,7 @@ __global__ __launch_bounds__(FD_THREADS) void fd_sum_kernel(FdArgs a, long nitem
         const int sep = a.separate_modes;
         const long sep_o = (long)q * a.data_length + i;
         const int nf = a.data_length;
-        fd_bin(it, f, xi, [&](fcx plus, fcx minus) {
+        fd_bin(it, f, xi, kc, [&](fcx plus, fcx minus) {
             if (sep) {
                 wave[sep_o] = make_double2(plus.re, plus.im);
             } else {

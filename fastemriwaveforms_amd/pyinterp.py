@@ -22,7 +22,8 @@ import torch
 from . import _lib
 from ._lib import check, lib
 
-__all__ = ["interpolate_arrays_wrap", "get_waveform_wrap", "get_waveform_generic_fd_wrap"]
+__all__ = ["interpolate_arrays_wrap", "get_waveform_wrap", "get_waveform_generic_fd_wrap",
+           "interp2D"]
 
 
 def _device() -> torch.device:
@@ -174,3 +175,25 @@ def get_waveform_generic_fd_wrap(*args, **kwargs):
         int(bool(separate_modes)), _stream_ptr())
     check(rc, "get_waveform_generic_fd")
     wave.finish()
+
+
+def interp2D(z, tx, nx, ty, ny, c, kx, ky, x, mx, y, my, num_indiv_c, len_indiv_c):
+    """interp2D(z, tx, nx, ty, ny, c, kx, ky, x, mx, y, my, num_indiv_c, len_indiv_c).
+
+    src/few/cutils/pyampinterp2D.pyx:15-29 (kernel AmpInterp2D.cu:86-206).
+    z (num_indiv_c * mx) is written in place; mx != my raises ValueError as
+    the reference's invalid_argument does.
+    """
+    if int(mx) != int(my):
+        raise ValueError("mx and my must be the same value.")
+    zs = _Staged(z, torch.float64, True)
+    st = [_Staged(a, torch.float64, False) for a in (tx, ty, c, x, y)]
+    if (zs.dev.numel() < int(num_indiv_c) * int(mx) or st[0].dev.numel() < int(nx)
+            or st[1].dev.numel() < int(ny) or st[2].dev.numel() < int(num_indiv_c) * int(len_indiv_c)
+            or st[3].dev.numel() < int(mx) or st[4].dev.numel() < int(my)):
+        raise ValueError("interp2D: array smaller than the sizes imply")
+    rc = lib().few_interp2D_wrap(zs.ptr, st[0].ptr, int(nx), st[1].ptr, int(ny), st[2].ptr, int(kx),
+                                 int(ky), st[3].ptr, int(mx), st[4].ptr, int(my), int(num_indiv_c),
+                                 int(len_indiv_c), _stream_ptr())
+    check(rc, "interp2D")
+    zs.finish()

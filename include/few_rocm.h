@@ -331,6 +331,32 @@ int few_get_waveform_generic_fd(void *waveform, double *interp_array, double *ph
                                 void *ylm_all, int zero_index, int include_minus_m,
                                 int separate_modes, void *stream);
 
+/* ---- bicubic amplitude interpolation ----------------------------------- */
+
+/* Bytes of workspace few_interp2d_async needs for mx points (their B-spline
+ * intervals and weights). */
+size_t few_interp2d_workspace_bytes(int mx);
+
+/* Replaces interp2D_wrap (src/few/cutils/AmpInterp2D.hh, kernel
+ * AmpInterp2D.cu:86-206, bound at pyampinterp2D.pyx:9-29).  Same arguments,
+ * device pointers: z (num_indiv_c * mx) written, z[c_i * mx + i] the spline
+ * c_i (len_indiv_c coefficients, FITPACK layout) at (x[i], y[i]); knots tx
+ * (nx), ty (ny), degrees kx, ky.  Arguments outside the box of the knots are
+ * clamped to it, as in fpbisp.  Errors (the reference throws or reads out of
+ * bounds): mx != my, degrees outside 0..5, fewer than 2 (k + 1) knots, or
+ * len_indiv_c below (nx - kx - 1)(ny - ky - 1).  Bit-identical to the
+ * reference's arithmetic.  Stream-ordered. */
+int few_interp2d_async(double *z, const double *tx, int nx, const double *ty, int ny,
+                       const double *c, int kx, int ky, const double *x, int mx, const double *y,
+                       int my, int num_indiv_c, int len_indiv_c, void *workspace,
+                       size_t workspace_bytes, void *stream);
+
+/* The reference's synchronous call shape: library-owned workspace, returns
+ * after the stream has finished. */
+int few_interp2D_wrap(double *z, const double *tx, int nx, const double *ty, int ny, double *c,
+                      int kx, int ky, const double *x, int mx, const double *y, int my,
+                      int num_indiv_c, int len_indiv_c, void *stream);
+
 /* Frees the library-owned workspaces of the synchronous drop-in entries. */
 int few_release_workspace(void);
 

@@ -24,6 +24,11 @@ that file):
   (interpolate.cu:1041-1090, 1113-1134), pinned against scipy's
   ``kv(1/3, .)`` and ``numpy.roots``.
 
+``few_oracle_amp.c`` restates the bicubic amplitude interpolation:
+
+* ``interp2d``            -- src/few/cutils/AmpInterp2D.cu:1-208 (CPU branch,
+  FITPACK fpbspl / fpbisp), pinned against scipy's FITPACK ``bisplev``.
+
 ``fd_segment_inds`` and ``fd_post`` restate fdinterp.py's Python glue
 (summation/fdinterp.py:202-465 and 514-542) in numpy.
 
@@ -88,6 +93,10 @@ def lib() -> ctypes.CDLL:
         _lib.orc_fd_spa_func.restype = None
         _lib.orc_fd_cube_roots.argtypes = [d, ctypes.c_long, d]
         _lib.orc_fd_cube_roots.restype = None
+        _lib.orc_interp2d.argtypes = [d, d, ctypes.c_int, d, ctypes.c_int, d, ctypes.c_int,
+                                      ctypes.c_int, d, ctypes.c_int, d, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int]
+        _lib.orc_interp2d.restype = ctypes.c_int
         _lib.orc_max_threads.argtypes = []
         _lib.orc_max_threads.restype = ctypes.c_int
     return _lib
@@ -242,6 +251,21 @@ def spline_adjoint(t, g_coef):
 
 def max_threads() -> int:
     return int(lib().orc_max_threads())
+
+
+def interp2d(tx, ty, c, x, y, kx=3, ky=3):
+    """interp2D (AmpInterp2D.cu:86-162, CPU branch): c (num_sets, len_per_set)
+    tensor-product B-spline coefficients on knots tx, ty; returns z
+    (num_sets, len(x))."""
+    tx, ty, x, y = _f64(tx), _f64(ty), _f64(x), _f64(y)
+    c = _f64(np.atleast_2d(c))
+    num, length = c.shape
+    z = np.zeros((num, len(x)))
+    rc = lib().orc_interp2d(_dp(z), _dp(tx), len(tx), _dp(ty), len(ty), _dp(c), kx, ky, _dp(x),
+                            len(x), _dp(y), len(y), num, length)
+    if rc != 0:
+        raise ValueError("mx and my must be the same value.")
+    return z
 
 
 def fd_waveform(interp_fd, phase_t, phase_coeffs_fd, m, k, n, t, frequencies, start_inds,

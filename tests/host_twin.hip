@@ -15,6 +15,7 @@
 #include "spline.hip"
 #include "adjoint.hip"
 #include "fd.hip"
+#include "amp2d.hip"
 #include "modesum_direct.hpp"
 #include "few_math.hpp"
 
@@ -217,6 +218,7 @@ int twin_fd_waveform(double *wave, const double *interp, const double *phase_t,
     a.include_minus_m = include_minus_m && !separate_modes;
     a.separate_modes = separate_modes;
     const fcx xi = fd_xi();
+    const FdSpaConst kc = fd_spa_const();
     for (int q = 0; q < K; q++) {
         for (int j = 0; j < nseg; j++) {
             const long item = (long)q * nseg + j;
@@ -228,7 +230,7 @@ int twin_fd_waveform(double *wave, const double *interp, const double *phase_t,
             for (int i = lo; i <= hi; i++) {
                 const int mi = fd_minus_index(zero_index, i);
                 const bool conj_term = (it.mm != 0.0) && a.include_minus_m;
-                fd_bin(it, freqs[i], xi, [&](fcx plus, fcx minus) {
+                fd_bin(it, freqs[i], xi, kc, [&](fcx plus, fcx minus) {
                     if (separate_modes) {
                         wave[2 * ((long)q * Nf + i)] = plus.re;
                         wave[2 * ((long)q * Nf + i) + 1] = plus.im;
@@ -247,6 +249,20 @@ int twin_fd_waveform(double *wave, const double *interp, const double *phase_t,
     return 0;
 }
 
+// interp2D with the kernels' per-point basis and per-set sum, in the CPU
+// branch's order (sets outer, points inner).
+int twin_interp2d(double *z, const double *tx, int nx, const double *ty, int ny, const double *c,
+                  int kx, int ky, const double *x, int mx, const double *y, int num_c, int len_c)
+{
+    std::vector<double> b(14);
+    for (int ci = 0; ci < num_c; ci++)
+        for (int i = 0; i < mx; i++) {
+            amp_point_basis(tx, nx, ty, ny, kx, ky, x[i], y[i], b.data());
+            z[(long)ci * mx + i] = amp_sum(c + (long)ci * len_c, ny - ky - 1, kx, ky, b.data());
+        }
+    return 0;
+}
+
 void twin_fd_cube_roots(const double *abcd, long n, double *out)
 {
     const fcx xi = fd_xi();
@@ -256,8 +272,9 @@ void twin_fd_cube_roots(const double *abcd, long n, double *out)
 
 void twin_fd_spa(const double *x, long n, double *out)
 {
+    const FdSpaConst kc = fd_spa_const();
     for (long i = 0; i < n; i++) {
-        const fcx r = fd_spa(x[i]);
+        const fcx r = fd_spa(x[i], kc);
         out[2 * i] = r.re;
         out[2 * i + 1] = r.im;
     }

@@ -21,7 +21,7 @@ from helpers import fd_frequencies, oracle_fd, oracle_fd_inputs
 
 pytestmark = pytest.mark.gpu
 
-FD_TOL = 1e-10
+FD_TOL = 2e-11
 
 
 def _rel_l2(a, b):
