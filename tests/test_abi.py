@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     src = open(os.path.join(ROOT, "include", "few_rocm.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(few_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(few_[A-Za-z0-9_]+)\s*\(", src)))
 
 
 def test_header_declares_entry_points():
