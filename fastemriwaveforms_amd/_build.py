@@ -22,7 +22,8 @@ ARCH = os.environ.get("FEW_ROCM_ARCH", "gfx950")
 
 SOURCES = ["capi.hip", "spline.hip", "modesum_direct.hip", "modesum_fact.hip", "modesum_fact16.hip", "batch.hip",
            "adjoint.hip", "adjoint16.hip",
-           "eval.hip", "fd.hip", "amp2d.hip"]
+           "eval.hip", "fd.hip", "amp2d.hip",
+           "aak.hip"]
 
 
 def hipcc() -> str:

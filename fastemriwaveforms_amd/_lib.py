@@ -77,6 +77,12 @@ def _declare(lib: ctypes.CDLL) -> None:
                                 c_int, c_int, _d, c_size, _d], c_int),
         "few_interp2D_wrap": ([_d, _d, c_int, _d, c_int, _d, c_int, c_int, _d, c_int, _d, c_int,
                                c_int, c_int, _d], c_int),
+        "few_aak_workspace_bytes": ([c_int], c_size),
+        "few_aak_waveform_async": ([_d, _d] + [c_double] * 8 + [c_int, c_int, c_int, c_int,
+                                                                 c_double, _d, _d, c_size, _d],
+                                   c_int),
+        "few_aak_get_waveform": ([_d, _d] + [c_double] * 8 + [c_int, c_int, c_int, c_int,
+                                                               c_double, _d, _d], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -120,4 +126,5 @@ def exported_symbols() -> list[str]:
         "few_spline_eval_async", "few_direct_mode_sum_async", "few_fd_workspace_bytes",
         "few_get_waveform_generic_fd_async", "few_get_waveform_generic_fd",
         "few_interp2d_workspace_bytes", "few_interp2d_async", "few_interp2D_wrap",
+        "few_aak_workspace_bytes", "few_aak_waveform_async", "few_aak_get_waveform",
     ]

@@ -105,7 +105,7 @@ class RocmBackend(Backend):
             raise BackendUnavailableException("'rocm' backend could not be imported.") from e
         _lib.lib()  # raises BackendNotInstalled when libfew_rocm.so is missing
         return BackendMethods(
-            pyWaveform=_not_in_backend("pyWaveform"),
+            pyWaveform=pyinterp.pyWaveform,
             interp2D=pyinterp.interp2D,
             interpolate_arrays_wrap=pyinterp.interpolate_arrays_wrap,
             get_waveform_wrap=pyinterp.get_waveform_wrap,

@@ -15,6 +15,7 @@
 #include <mutex>
 #include <vector>
 
+#include "aak.hpp"
 #include "adjoint.hpp"
 #include "amp2d.hpp"
 #include "eval.hpp"
@@ -1012,6 +1013,83 @@ int few_interp2D_wrap(double *z, const double *tx, int nx, const double *ty, int
     rc = library_workspace(cur, need > 0 ? need : 1, &ws);
     if (rc) return rc;
     rc = launch_amp2d(z, tx, nx, ty, ny, c, kx, ky, x, mx, y, num_indiv_c, len_indiv_c, ws, need, s);
+    if (rc) return rc;
+    FEW_HIP(hipStreamSynchronize(s));
+    return FEW_OK;
+}
+
+size_t few_aak_workspace_bytes(int init_len) { return aak_workspace_bytes(init_len); }
+
+static int check_aak(const void *waveform, const double *interp_array, int nmodes, int init_len,
+                     int out_len, double delta_t, const double *h_t)
+{
+    if (!waveform || !interp_array || !h_t) {
+        set_error("aak: null pointer argument");
+        return FEW_ERR_ARG;
+    }
+    if (init_len < 2 || out_len < 0 || nmodes < 0 || !(delta_t > 0.0)) {
+        set_error("aak: invalid sizes (init_len %d, out_len %d, nmodes %d, dt %g)", init_len,
+                  out_len, nmodes, delta_t);
+        return FEW_ERR_ARG;
+    }
+    return FEW_OK;
+}
+
+int few_aak_waveform_async(void *waveform, const double *interp_array, double M_phys,
+                           double S_phys, double mu, double qS, double phiS, double qK,
+                           double phiK, double dist, int nmodes, int mich, int init_len,
+                           int out_len, double delta_t, const double *d_h_t, void *workspace,
+                           size_t workspace_bytes, void *stream)
+{
+    int rc = check_aak(waveform, interp_array, nmodes, init_len, out_len, delta_t, d_h_t);
+    if (rc) return rc;
+    if (!workspace) {
+        set_error("aak_waveform_async: null workspace");
+        return FEW_ERR_ARG;
+    }
+    AakArgs a;
+    a.wave = static_cast<double2 *>(waveform);
+    a.interp = interp_array;
+    a.h_t = d_h_t;
+    a.M_phys = M_phys;
+    a.S_phys = S_phys;
+    a.mu = mu;
+    a.qS = qS;
+    a.phiS = phiS;
+    a.qK = qK;
+    a.phiK = phiK;
+    a.dist = dist;
+    a.dt = delta_t;
+    a.nmodes = nmodes;
+    a.mich = mich ? 1 : 0;
+    a.L = init_len;
+    a.out_len = out_len;
+    a.plan = nullptr;
+    return launch_aak(a, workspace, workspace_bytes, reinterpret_cast<hipStream_t>(stream));
+}
+
+int few_aak_get_waveform(void *waveform, double *interp_array, double M_phys, double S_phys,
+                         double mu, double qS, double phiS, double qK, double phiK, double dist,
+                         int nmodes, int mich, int init_len, int out_len, double delta_t,
+                         double *h_t, void *stream)
+{
+    int rc = check_aak(waveform, interp_array, nmodes, init_len, out_len, delta_t, h_t);
+    if (rc) return rc;
+    int cur = 0;
+    FEW_HIP(hipGetDevice(&cur));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    const size_t plan = aak_workspace_bytes(init_len);
+    const size_t need = ((plan + 255) & ~(size_t)255) + sizeof(double) * (size_t)init_len;
+    void *ws = nullptr;
+    rc = library_workspace(cur, need, &ws);
+    if (rc) return rc;
+    // h_t is a host array in the reference interface (gpuAAK.cu:536-552)
+    double *d_ht = reinterpret_cast<double *>(static_cast<char *>(ws) + ((plan + 255) & ~(size_t)255));
+    FEW_HIP(hipMemcpyAsync(d_ht, h_t, sizeof(double) * init_len, hipMemcpyHostToDevice, s));
+    rc = few_aak_waveform_async(waveform, interp_array, M_phys, S_phys, mu, qS, phiS, qK, phiK,
+                                dist, nmodes, mich, init_len, out_len, delta_t, d_ht, ws, plan,
+                                stream);
     if (rc) return rc;
     FEW_HIP(hipStreamSynchronize(s));
     return FEW_OK;

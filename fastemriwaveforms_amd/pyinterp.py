@@ -23,7 +23,7 @@ from . import _lib
 from ._lib import check, lib
 
 __all__ = ["interpolate_arrays_wrap", "get_waveform_wrap", "get_waveform_generic_fd_wrap",
-           "interp2D"]
+           "interp2D", "pyWaveform"]
 
 
 def _device() -> torch.device:
@@ -197,3 +197,27 @@ def interp2D(z, tx, nx, ty, ny, c, kx, ky, x, mx, y, my, num_indiv_c, len_indiv_
                                  int(len_indiv_c), _stream_ptr())
     check(rc, "interp2D")
     zs.finish()
+
+
+def pyWaveform(*args, **kwargs):
+    """pyWaveform(waveform, interp_array, M_phys, S_phys, mu, qS, phiS, qK, phiK, dist, nmodes,
+    mich, init_len, out_len, delta_t, h_t).
+
+    src/few/cutils/gpuAAKWrap.pyx:17-33 (kernel gpuAAK.cu:181-577).  waveform
+    samples of the used knot segments are overwritten in place; h_t is a host
+    array in the reference (device tensors are copied to the host here).
+    """
+    (waveform, interp_array, M_phys, S_phys, mu, qS, phiS, qK, phiK, dist, nmodes, mich,
+     init_len, out_len, delta_t, h_t) = args
+    wave = _Staged(waveform, torch.complex128, True)
+    interp = _Staged(interp_array, torch.float64, False)
+    ht = _host_f64(h_t)
+    L, N = int(init_len), int(out_len)
+    if len(ht) < L or wave.dev.numel() < N or interp.dev.numel() < 8 * (L - 1) * 6:
+        raise ValueError("pyWaveform: array smaller than the sizes imply")
+    rc = lib().few_aak_get_waveform(wave.ptr, interp.ptr, float(M_phys), float(S_phys), float(mu),
+                                    float(qS), float(phiS), float(qK), float(phiK), float(dist),
+                                    int(nmodes), int(bool(mich)), L, N, float(delta_t),
+                                    ht.ctypes.data, _stream_ptr())
+    check(rc, "pyWaveform")
+    wave.finish()

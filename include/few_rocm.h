@@ -357,6 +357,34 @@ int few_interp2D_wrap(double *z, const double *tx, int nx, const double *ty, int
                       int kx, int ky, const double *x, int mx, const double *y, int my,
                       int num_indiv_c, int len_indiv_c, void *stream);
 
+/* ---- Augmented Analytic Kludge summation -------------------------------- */
+
+/* Bytes of workspace few_aak_waveform_async needs (segment start indices). */
+size_t few_aak_workspace_bytes(int init_len);
+
+/* Replaces get_waveform of src/few/cutils/gpuAAK.hh (kernel gpuAAK.cu:181-577,
+ * bound as pyWaveform at gpuAAKWrap.pyx:17-33).  Same arguments, all device
+ * pointers here (d_h_t included): waveform (out_len,) complex128, each sample
+ * of a used knot segment overwritten with hI - i hII; interp_array (8,
+ * init_len - 1, 6) dense-output coefficients of (p, e, Y, Phi_phi,
+ * Phi_theta, Phi_r); masses in solar masses, dist in Gpc; nmodes harmonics;
+ * mich selects the long-wavelength LISA response.  Segments follow
+ * gpuAAK.cu:458-483; when no knot reaches the last sample, the segments stop
+ * at the last knot (the reference reads past its arrays there).
+ * Stream-ordered. */
+int few_aak_waveform_async(void *waveform, const double *interp_array, double M_phys,
+                           double S_phys, double mu, double qS, double phiS, double qK,
+                           double phiK, double dist, int nmodes, int mich, int init_len,
+                           int out_len, double delta_t, const double *d_h_t, void *workspace,
+                           size_t workspace_bytes, void *stream);
+
+/* The reference's synchronous call shape (gpuAAK.hh: get_waveform), h_t on the
+ * host; library-owned workspace; returns after the stream has finished. */
+int few_aak_get_waveform(void *waveform, double *interp_array, double M_phys, double S_phys,
+                         double mu, double qS, double phiS, double qK, double phiK, double dist,
+                         int nmodes, int mich, int init_len, int out_len, double delta_t,
+                         double *h_t, void *stream);
+
 /* Frees the library-owned workspaces of the synchronous drop-in entries. */
 int few_release_workspace(void);
 

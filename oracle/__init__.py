@@ -29,6 +29,13 @@ that file):
 * ``interp2d``            -- src/few/cutils/AmpInterp2D.cu:1-208 (CPU branch,
   FITPACK fpbspl / fpbisp), pinned against scipy's FITPACK ``bisplev``.
 
+``few_oracle_aak.c`` restates the AAK summation:
+
+* ``aak_waveform``        -- src/few/cutils/gpuAAK.cu:181-577 (CPU branch), with
+  ``bessel_jn`` (Numerical Recipes, gpuAAK.cu:8-118, pinned against scipy's
+  ``jv``) and ``aak_rot_coeff`` (gpuAAK.cu:136-176, pinned by vector
+  identities); the circular limit is pinned by the quadrupole closed form.
+
 ``fd_segment_inds`` and ``fd_post`` restate fdinterp.py's Python glue
 (summation/fdinterp.py:202-465 and 514-542) in numpy.
 
@@ -97,6 +104,17 @@ def lib() -> ctypes.CDLL:
                                       ctypes.c_int, d, ctypes.c_int, d, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]
         _lib.orc_interp2d.restype = ctypes.c_int
+        _lib.orc_bessel_jn.argtypes = [ctypes.c_int, ctypes.c_double]
+        _lib.orc_bessel_jn.restype = ctypes.c_double
+        _lib.orc_bessel_j1_export.argtypes = [ctypes.c_double]
+        _lib.orc_bessel_j1_export.restype = ctypes.c_double
+        _lib.orc_aak_rot_coeff.argtypes = [d, d, d, d, d, d] + [ctypes.c_double] * 6
+        _lib.orc_aak_rot_coeff.restype = None
+        _lib.orc_aak_start_inds.argtypes = [i, d, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+        _lib.orc_aak_start_inds.restype = ctypes.c_int
+        _lib.orc_aak_waveform.argtypes = [d, d] + [ctypes.c_double] * 8 + [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, d]
+        _lib.orc_aak_waveform.restype = ctypes.c_int
         _lib.orc_max_threads.argtypes = []
         _lib.orc_max_threads.restype = ctypes.c_int
     return _lib
@@ -251,6 +269,44 @@ def spline_adjoint(t, g_coef):
 
 def max_threads() -> int:
     return int(lib().orc_max_threads())
+
+
+def bessel_jn(n, x):
+    """Numerical Recipes J_n (gpuAAK.cu:62-118); n = -1 gives gpuAAK's -J1."""
+    if n == -1:
+        return -lib().orc_bessel_j1_export(float(x))
+    return lib().orc_bessel_jn(int(n), float(x))
+
+
+def aak_rot_coeff(iota, theta_S, phi_S, theta_K, phi_K, alpha):
+    """gpuAAK.cu:136-176: (rot (4,), n, L, S, nxL, nxS)."""
+    out = [np.zeros(4)] + [np.zeros(3) for _ in range(5)]
+    lib().orc_aak_rot_coeff(*(_dp(o) for o in out), float(iota), float(theta_S), float(phi_S),
+                            float(theta_K), float(phi_K), float(alpha))
+    return out
+
+
+def aak_start_inds(h_t, delta_t, out_len):
+    """gpuAAK.cu:458-483: (start indices, number of knots used)."""
+    h_t = _f64(h_t)
+    st = np.zeros(len(h_t) + 1, dtype=np.int32)
+    npts = lib().orc_aak_start_inds(_ip(st), _dp(h_t), float(delta_t), len(h_t), int(out_len))
+    return st[:npts], npts
+
+
+def aak_waveform(interp_coeffs, M, S, mu, qS, phiS, qK, phiK, dist, nmodes, mich, h_t, out_len,
+                 delta_t):
+    """get_waveform of gpuAAK.cu (CPU branch).  interp_coeffs: (8, L-1, 6) dense-output
+    coefficients of (p, e, Y, Phi_phi, Phi_theta, Phi_r).  Returns (out_len,) complex."""
+    interp = _f64(interp_coeffs)
+    wave = np.zeros(out_len, dtype=np.complex128)
+    h_t = _f64(h_t)
+    rc = lib().orc_aak_waveform(_dp(wave.view(np.float64)), _dp(interp), M, S, mu, qS, phiS, qK,
+                                phiK, dist, int(nmodes), int(bool(mich)), len(h_t), int(out_len),
+                                float(delta_t), _dp(h_t))
+    if rc != 0:
+        raise RuntimeError(f"oracle aak_waveform failed: {rc}")
+    return wave
 
 
 def interp2d(tx, ty, c, x, y, kx=3, ky=3):

@@ -75,3 +75,34 @@ def oracle_fd(inp, frequency, f_arr_given=False, include_minus_m=True, separate_
                               start, end, inp.ylms, zero, include_minus_m=include_minus_m,
                               separate_modes=separate_modes)
     return wave, (start, end, zero)
+
+
+def aak_input(T_years=0.05, L=40, seed=9, e0=0.4, e1=0.1, Y0=0.7, dt=10.0):
+    """Synthetic sparse AAK trajectory: knot times and the (L-1, 6, 8) dense-output
+    coefficients of (p, e, Y, Phi_phi, Phi_theta, Phi_r) the trajectory would hand
+    to AAKSummation (cubic splines of smooth knot values, converted to the
+    dense form), plus source parameters."""
+    from fastemriwaveforms_amd import synthetic
+    rng = np.random.default_rng(seed)
+    Ts = T_years * synthetic.YRSID_SI
+    t = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 1.0, L - 2)) * Ts, [Ts]])
+    tau = t / Ts
+    p = 12.0 - 4.0 * tau
+    e = e0 + (e1 - e0) * tau ** 2
+    Y = Y0 + 0.05 * tau
+    Pphi = synthetic.phase_phi(t, Ts, "rising")
+    knots = np.stack([p, e, Y, Pphi, 0.97 * Pphi, 0.83 * Pphi])
+    pc, _, info = oracle.interpolate_arrays(t, knots)
+    assert info == 0
+    dense = oracle.cubic_to_dense(pc, t)                      # (8, L-1, 6)
+    coeffs = np.ascontiguousarray(np.transpose(dense, (1, 2, 0)))  # (L-1, 6, 8)
+    params = dict(M=1e6, S=0.5, mu=10.0, qS=0.9, phiS=1.3, qK=0.6, phiK=2.1, dist=1.0)
+    n_out = synthetic.num_points(t[0], t[-1], T_years, dt)
+    return t, coeffs, params, n_out
+
+
+def oracle_aak(t, coeffs, params, n_out, nmodes=20, mich=False, dt=10.0):
+    P = params
+    dense = np.ascontiguousarray(np.transpose(coeffs, (2, 0, 1)))   # (8, L-1, 6)
+    return oracle.aak_waveform(dense, P["M"], P["S"], P["mu"], P["qS"], P["phiS"], P["qK"],
+                               P["phiK"], P["dist"], nmodes, mich, t, n_out, dt)

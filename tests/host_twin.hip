@@ -16,6 +16,7 @@
 #include "adjoint.hip"
 #include "fd.hip"
 #include "amp2d.hip"
+#include "aak.hip"
 #include "modesum_direct.hpp"
 #include "few_math.hpp"
 
@@ -260,6 +261,43 @@ int twin_interp2d(double *z, const double *tx, int nx, const double *ty, int ny,
             amp_point_basis(tx, nx, ty, ny, kx, ky, x[i], y[i], b.data());
             z[(long)ci * mx + i] = amp_sum(c + (long)ci * len_c, ny - ky - 1, kx, ky, b.data());
         }
+    return 0;
+}
+
+// The AAK sum with the kernel's plan and per-sample routine, segment by
+// segment in the CPU branch's order.
+int twin_aak_waveform(double *wave, const double *interp, double M, double S, double mu,
+                      double qS, double phiS, double qK, double phiK, double dist, int nmodes,
+                      int mich, int L, int out_len, double dt, const double *h_t)
+{
+    AakArgs a;
+    a.wave = reinterpret_cast<double2 *>(wave);
+    a.interp = interp;
+    a.h_t = h_t;
+    a.M_phys = M;
+    a.S_phys = S;
+    a.mu = mu;
+    a.qS = qS;
+    a.phiS = phiS;
+    a.qK = qK;
+    a.phiK = phiK;
+    a.dist = dist;
+    a.dt = dt;
+    a.nmodes = nmodes;
+    a.mich = mich;
+    a.L = L;
+    a.out_len = out_len;
+    std::vector<int> st((size_t)L + 1);
+    a.plan = st.data();
+    const int npts = aak_plan(h_t, L, dt, out_len, st.data());
+    const long np = (long)(L - 1) * 6;
+    for (int j = 0; j < npts - 1; j++) {
+        double sc[6][8];
+        for (int c = 0; c < 8; c++)
+            for (int p = 0; p < 6; p++) sc[p][c] = interp[c * np + (long)j * 6 + p];
+        for (long i = st[j]; i < st[j + 1] && i < out_len; i++)
+            a.wave[i] = aak_sample(a, sc, h_t[j], h_t[j + 1] - h_t[j], i);
+    }
     return 0;
 }
 
