@@ -81,7 +81,7 @@ def cpu_baseline(inp, seconds_target: float = 12.0):
                       "restating interpolate.cu's CPU branch (-O2 -ffp-contract=off), 1 thread"}
 
 
-def cpu_checks(inp, head, batch_chk, fd_chk):
+def cpu_checks(inp, head, batch_chk, fd_chk, aak_chk=None, amp_chk=None):
     """The CPU leg's checker role: the oracle's values at the samples / bins the
     GPU legs kept, compared with them (rel-L2), and the oracle's frequency-domain
     sum timed on the same check (1 thread)."""
@@ -109,6 +109,29 @@ def cpu_checks(inp, head, batch_chk, fd_chk):
                          "kind": "port", "sample": f"{c['modes']} modes x their {c['bins']} "
                          f"frequency bins of the same workload ({el:.1f} s); "
                          "oracle/few_oracle_fd.c restating interpolate.cu:1142-1500, 1 thread"}
+    if aak_chk is not None:
+        c = aak_chk
+        p = c["prm"]
+        t0 = time.perf_counter()
+        ref = oracle.aak_waveform(c["dense"], p["M"], p["S"], p["mu"], p["qS"], p["phiS"],
+                                  p["qK"], p["phiK"], p["dist"], c["nmodes"], True, c["t"],
+                                  c["nchk"], c["dt"])
+        el = time.perf_counter() - t0
+        res["aak"] = oracle.rel_l2(c["gpu"], ref)
+        res["aak_cpu"] = {"value": c["nchk"] / el, "unit": "samples/s", "cores": 1,
+                          "kind": "port", "sample": f"the first {c['nchk']} samples of the same "
+                          f"workload ({el:.2f} s); oracle/few_oracle_aak.c restating gpuAAK.cu's "
+                          "CPU branch, 1 thread"}
+    if amp_chk is not None:
+        c = amp_chk
+        t0 = time.perf_counter()
+        ref = oracle.interp2d(c["tx"], c["ty"], c["c"], c["x"], c["y"])
+        el = time.perf_counter() - t0
+        res["interp2d"] = bool(np.array_equal(c["gpu"], ref))
+        res["interp2d_cpu"] = {"value": ref.size / el, "unit": "evaluations/s", "cores": 1,
+                               "kind": "port", "sample": f"{c['c'].shape[0]} of the coefficient "
+                               f"sets at all points ({el * 1e3:.1f} ms); oracle/few_oracle_amp.c, "
+                               "1 thread"}
     return res
 
 
@@ -449,6 +472,103 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
     return out
 
 
+def aak_leg(dev, reps: int = 3, nmodes: int = 30):
+    """AAK summation (AAKSummation, gpuAAK.cu's get_waveform) at 1 yr, dt = 10 s
+    (3,155,815 samples), 100 knots, 30 harmonics, LISA long-wavelength
+    response; a slice of the output is kept for the CPU leg's oracle check."""
+    import torch
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd._lib import check, lib
+    rng = np.random.default_rng(77)
+    T, L, dt = 1.0, 100, 10.0
+    Ts = T * synthetic.YRSID_SI
+    t = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 1.0, L - 2)) * Ts, [Ts]])
+    tau = t / Ts
+    pk = synthetic.phase_phi(t, Ts, "rising")
+    knots = np.stack([12.0 - 4.0 * tau, 0.4 - 0.3 * tau ** 2, 0.7 + 0.05 * tau, pk, 0.97 * pk,
+                      0.83 * pk])
+    # the trajectory's dense-output coefficients, from GPU cubic splines of the knots
+    t_d = torch.from_numpy(t).to(dev)
+    knots_d = torch.from_numpy(np.ascontiguousarray(knots)).to(dev)
+    dense = torch.empty((8, L - 1, 6), dtype=torch.float64, device=dev)
+    scratch = torch.empty(4 * L * 6, dtype=torch.float64, device=dev)
+    check(lib().few_spline_build_async(knots_d.data_ptr(), L, 1, t_d.data_ptr(), 0, 1,
+                                       dense.data_ptr(), 0, 0, 0, 6, L, 1, scratch.data_ptr(),
+                                       int(torch.cuda.current_stream().cuda_stream)), "aak knots")
+    n_out = synthetic.num_points(t[0], t[-1], T, dt)
+    prm = dict(M=1e6, S=0.5, mu=10.0, qS=0.9, phiS=1.3, qK=0.6, phiK=2.1, dist=1.0)
+    wave = torch.zeros(n_out, dtype=torch.complex128, device=dev)
+    ws = torch.empty(int(lib().few_aak_workspace_bytes(L)), dtype=torch.uint8, device=dev)
+    stream = int(torch.cuda.current_stream().cuda_stream)
+
+    def run():
+        check(lib().few_aak_waveform_async(
+            wave.data_ptr(), dense.data_ptr(), prm["M"], prm["S"], prm["mu"], prm["qS"],
+            prm["phiS"], prm["qK"], prm["phiK"], prm["dist"], nmodes, 1, L, n_out, dt,
+            t_d.data_ptr(), ws.data_ptr(), ws.numel(), stream), "aak sum")
+
+    run()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        run()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    out = {"workload": f"AAK, T={T} yr, dt={dt} s, {n_out} samples, L={L} knots, "
+                       f"{nmodes} harmonics, LISA long-wavelength response",
+           "ms": ms, "samples_per_s": n_out / (ms * 1e-3),
+           "harmonic_samples_per_s": n_out * nmodes / (ms * 1e-3), "rel_l2_vs_oracle": None}
+    log(f"[bench] aak: {ms:.2f} ms, {out['samples_per_s']:.4g} samples/s "
+        f"({out['harmonic_samples_per_s']:.4g} harmonic-samples/s)")
+    nchk = 4000
+    chk = {"t": t, "dense": dense.cpu().numpy(), "prm": prm, "nmodes": nmodes, "dt": dt,
+           "n_out": n_out, "nchk": nchk, "gpu": wave[:nchk].cpu().numpy()}
+    del wave, dense, ws
+    return out, chk
+
+
+def interp2d_leg(dev, reps: int = 20):
+    """interp2D at the size of a full mode set: 2 x 3843 coefficient sets on a
+    38 x 48 knot grid (1,496 coefficients each), 120 trajectory points."""
+    import torch
+    from fastemriwaveforms_amd._lib import check, lib
+    rng = np.random.default_rng(5)
+    tx = np.concatenate([[0.0] * 4, np.sort(rng.uniform(0, 1, 30)), [1.0] * 4])
+    ty = np.concatenate([[0.0] * 4, np.sort(rng.uniform(0, 1, 40)), [1.0] * 4])
+    nc = (len(tx) - 4) * (len(ty) - 4)
+    sets, mx = 2 * 3843, 120
+    c = rng.normal(size=(sets, nc))
+    x, y = rng.uniform(0, 1, mx), rng.uniform(0, 1, mx)
+    d = lambda v: torch.from_numpy(np.ascontiguousarray(v)).to(dev)
+    txd, tyd, cd, xd, yd = d(tx), d(ty), d(c), d(x), d(y)
+    z = torch.empty(sets * mx, dtype=torch.float64, device=dev)
+    ws = torch.empty(int(lib().few_interp2d_workspace_bytes(mx)), dtype=torch.uint8, device=dev)
+    stream = int(torch.cuda.current_stream().cuda_stream)
+
+    def run():
+        check(lib().few_interp2d_async(z.data_ptr(), txd.data_ptr(), len(tx), tyd.data_ptr(),
+                                       len(ty), cd.data_ptr(), 3, 3, xd.data_ptr(), mx,
+                                       yd.data_ptr(), mx, sets, nc, ws.data_ptr(), ws.numel(),
+                                       stream), "interp2D")
+
+    run()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        run()
+    b.record()
+    torch.cuda.synchronize()
+    us = a.elapsed_time(b) / reps * 1e3
+    out = {"workload": f"{sets} coefficient sets x {nc} coefficients, {mx} points, bicubic",
+           "us": us, "evaluations_per_s": sets * mx / (us * 1e-6), "bitwise_vs_oracle": None}
+    log(f"[bench] interp2D: {us:.1f} us, {out['evaluations_per_s']:.4g} evaluations/s")
+    chk = {"tx": tx, "ty": ty, "c": c[:64], "x": x, "y": y, "gpu": z.reshape(sets, mx)[:64].cpu().numpy()}
+    return out, chk
+
+
 def ModeSumFunction_apply(eng, t_ri, p_k):
     from fastemriwaveforms_amd.autograd import ModeSumFunction
     return ModeSumFunction.apply(eng, t_ri, p_k)
@@ -473,6 +593,8 @@ def main():
                     help="skip the differentiation leg (forward + backward, FD check)")
     ap.add_argument("--no-fd", action="store_true",
                     help="skip the frequency-domain sum leg")
+    ap.add_argument("--no-aak", action="store_true",
+                    help="skip the AAK and interp2D legs")
     ap.add_argument("--fd-coords", type=int, default=64,
                     help="coordinates of the finite-difference check (0 skips it)")
     ap.add_argument("--grad-batch", type=int, default=16,
@@ -575,6 +697,11 @@ def main():
     if not args.no_fd:
         fd, fd_chk = fd_leg(dev)
 
+    aak = aak_chk = amp = amp_chk = None
+    if not args.no_aak:
+        aak, aak_chk = aak_leg(dev)
+        amp, amp_chk = interp2d_leg(dev)
+
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_leg(dev, batch=args.grad_batch, fd_coords=args.fd_coords)
@@ -583,7 +710,7 @@ def main():
     err = None
     if not args.no_cpu:
         log("[bench] CPU leg: oracle checks and baseline (1 thread)")
-        chk = cpu_checks(inp, head_chk, batch_chk, fd_chk)
+        chk = cpu_checks(inp, head_chk, batch_chk, fd_chk, aak_chk, amp_chk)
         err = chk["headline"]
         log(f"[bench] rel-L2 vs oracle ({len(idx)} samples) {err:.3e}")
         if fp64 is not None:
@@ -598,6 +725,16 @@ def main():
             fd["speedup_vs_cpu_1thread"] = fd["mode_bins_per_s"] / chk["fd_cpu"]["value"]
             log(f"[bench] fd rel-L2 vs oracle ({fd_chk['modes']} modes) {chk['fd']:.3e}, "
                 f"CPU {chk['fd_cpu']['value']:.4g} mode-bins/s")
+        if aak is not None:
+            aak["rel_l2_vs_oracle"] = chk["aak"]
+            aak["cpu_baseline"] = chk["aak_cpu"]
+            aak["speedup_vs_cpu_1thread"] = aak["samples_per_s"] / chk["aak_cpu"]["value"]
+            amp["bitwise_vs_oracle"] = chk["interp2d"]
+            amp["cpu_baseline"] = chk["interp2d_cpu"]
+            amp["speedup_vs_cpu_1thread"] = amp["evaluations_per_s"] / chk["interp2d_cpu"]["value"]
+            log(f"[bench] aak rel-L2 vs oracle {chk['aak']:.3e}, CPU {chk['aak_cpu']['value']:.4g} "
+                f"samples/s; interp2D bitwise {chk['interp2d']}, CPU "
+                f"{chk['interp2d_cpu']['value']:.4g} evaluations/s")
         cpu = cpu_baseline(inp)
         log(f"[bench] CPU {cpu['value']:.4g} mode-samples/s")
 
@@ -626,6 +763,8 @@ def main():
         "batch": batch,
         "adjoint": adjoint,
         "fd": fd,
+        "aak": aak,
+        "interp2d": amp,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
