@@ -319,7 +319,7 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
     const long tb = (long)fact16_tile_bytes(bp.uni);
     const int sub = ((stage & 2) ? 1 : 0) | ((stage & 4) ? 2 : 0);
     if (!sub) return FEW_OK;
-    return launch_batch_fact16(bp.dev, r, p, tiles, tb, fact16_rstrh(bp.uni), fact16_kb(bp.uni),
+    return launch_batch_fact16(bp.dev, r, p, tiles, tb, fact16_kb(bp.uni),
                                variant, sub, s);
 }
 

@@ -181,14 +181,14 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
             PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
             int rc = launch_plan(p, stream);
             if (rc) return rc;
-            Fact16PrepArgs pa{interp, ylm, h_t, L, tiles, tb, fact16_rstrh(ph), fact16_kb(ph), ph.RB,
+            Fact16PrepArgs pa{interp, ylm, h_t, L, tiles, tb, fact16_kb(ph), ph.RB,
                               plan->dev};
             rc = launch_fact16_prep(pa, stream);
             if (rc) return rc;
         }
         if (!(stage & 2)) return FEW_OK;
-        Fact16Args fa{wave, pcoef, segs, prefix, L - 1, dt, tiles, tb, fact16_rstrh(ph),
-                      fact16_kb(ph), plan->dev};
+        Fact16Args fa{wave, pcoef, segs, prefix, L - 1, dt, tiles, tb, fact16_kb(ph),
+                      plan->dev};
         return launch_fact16_sum(fa, ph.RB, (int)max_blocks, variant, stream);
     }
     if (precision == FEW_PRECISION_MIXED) {

@@ -96,23 +96,17 @@ struct FactPrepArgs {
 int launch_fact_prep(const FactPrepArgs &a, hipStream_t stream);
 
 // ---- FP16 split-operand variant (modesum_fact16.hip)
-// Tile of one interval: two FP16 planes of RB*16 rows x RSTRH halfs, then
-// RB*8 floats of inverse row scales, padded to 256 bytes.
+// Tile of one interval: two FP16 planes (hi, lo), each KB x RB blocks of
+// 16 real rows x 32 halfs (block (kb, rb) at (kb * RB + rb) KiB), then RB*8
+// floats of inverse row scales, padded to 256 bytes.  The four 16-byte
+// chunks of a row are XOR-swizzled by (row >> 2) & 3, which puts the 16 rows
+// of a fragment read on distinct LDS banks without padding.
 inline int fact16_kb(const FactPlanHost &p) { return (2 * p.Nc + 31) / 32; }
-inline size_t fact16_tile_bytes_for(const FactPlanHost &p, int rstrh)
+inline size_t fact16_tile_bytes(const FactPlanHost &p)
 {
-    const size_t b = 2 * (size_t)p.RB * 16 * rstrh * 2 + (size_t)p.RB * 8 * 4;
+    const size_t b = 2 * (size_t)fact16_kb(p) * p.RB * 1024 + (size_t)p.RB * 8 * 4;
     return (b + 255) & ~(size_t)255;
 }
-// Row stride in halfs: 8 halfs of padding put the 16 rows of a fragment read
-// on distinct LDS banks; it is dropped only when the padded tile (plus the 24
-// phase coefficients) would not fit the 160 KB of LDS.
-inline int fact16_rstrh(const FactPlanHost &p)
-{
-    const int kb = fact16_kb(p);
-    return fact16_tile_bytes_for(p, 32 * kb + 8) + 24 * 8 <= 160 * 1024 ? 32 * kb + 8 : 32 * kb;
-}
-inline size_t fact16_tile_bytes(const FactPlanHost &p) { return fact16_tile_bytes_for(p, fact16_rstrh(p)); }
 
 struct Fact16PrepArgs {
     const double *interp;
@@ -121,7 +115,7 @@ struct Fact16PrepArgs {
     int L;
     unsigned char *tiles;
     long tile_bytes;
-    int RSTRH, KB, RB;
+    int KB, RB;
     FactPlanDev plan;
 };
 
@@ -134,7 +128,7 @@ struct Fact16Args {
     double dt;
     const unsigned char *tiles;
     long tile_bytes;
-    int RSTRH, KB;
+    int KB;
     FactPlanDev plan;
 };
 

@@ -12,9 +12,14 @@
 // output accumulates in FP64, as in the FP32 kernel.
 //
 // Tile layout (per knot interval, identical in global memory and LDS):
-//   plane hi, plane lo: RB*16 real rows x RSTRH halfs, real row 2R+e, real
-//       column k = 2a+f, holding [[Re, -Im], [Im, Re]] of C[R, a] -- the A
-//       fragment of a lane is 8 consecutive halfs (one ds_read_b128);
+//   plane hi, plane lo: RB*16 real rows x 32*KB real columns, real row
+//       2R+e, real column k = 2a+f, holding [[Re, -Im], [Im, Re]] of
+//       C[R, a]; stored as KB x RB blocks of 16 rows x 32 halfs (kb-major,
+//       one KiB each) so that every fragment address inside the K loop is a
+//       lane base plus a compile-time offset; the four 16-byte chunks of a
+//       row are XOR-swizzled by (row >> 2) & 3 (distinct LDS banks for the 16
+//       rows a fragment read touches).  The A fragment of a lane is 8
+//       consecutive halfs (one ds_read_b128);
 //   inv_sigma: RB*8 floats, 1/scale of complex row R.
 
 #include <math.h>
@@ -27,7 +32,23 @@
 namespace few {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef _Float16 v2h __attribute__((ext_vector_type(2)));
+
+// Halfs per 16 x 32 block of a tile plane.
+constexpr int F16_BLK = 512;
+
+// Offset (in halfs) of real row rho, real column k inside one tile plane.
+__device__ __forceinline__ long f16_tile_off(int rho, int k, int RB)
+{
+    const int rr = rho & 15;
+    const int chunk = ((k & 31) >> 3) ^ ((rr >> 2) & 3);
+    return ((long)((k >> 5) * RB + (rho >> 4)) * 16 + rr) * 32 + chunk * 8 + (k & 7);
+}
+
+// Lane part of a fragment read: row li of the block, 16-byte chunk g.
+__device__ __forceinline__ int f16_lane_off(int li, int g) { return li * 32 + ((g ^ ((li >> 2) & 3)) << 3); }
 
 #define INV_2PI16 0.15915494309189533577
 
@@ -37,7 +58,7 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 // by a power of two and writes the split FP16 planes.
 __device__ void fact16_prep_block(const FactPlanDev &P, const double *interp, const double2 *ylm,
                                   const double *h_t, int L, int j, int r, unsigned char *tile,
-                                  int RSTRH, int KB, int RB)
+                                  int KB, int RB)
 {
     __shared__ double s_max[4][256];
     const int tid = threadIdx.x;
@@ -88,9 +109,10 @@ __device__ void fact16_prep_block(const FactPlanDev &P, const double *interp, co
         }
         __syncthreads();
     }
+    const long plane = (long)KB * RB * F16_BLK;
     _Float16 *hi = reinterpret_cast<_Float16 *>(tile);
-    _Float16 *lo = hi + (long)RB * 16 * RSTRH;
-    float *inv_sigma = reinterpret_cast<float *>(lo + (long)RB * 16 * RSTRH);
+    _Float16 *lo = hi + plane;
+    float *inv_sigma = reinterpret_cast<float *>(lo + plane);
 #pragma unroll
     for (int p = 0; p < 4; p++) {
         const double m = s_max[p][0];
@@ -112,8 +134,8 @@ __device__ void fact16_prep_block(const FactPlanDev &P, const double *interp, co
                 const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
                 const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
                 // row 2R:   (Re, -Im);  row 2R+1: (Im, Re)
-                const long o0 = (long)(2 * R) * RSTRH + 2 * col;
-                const long o1 = (long)(2 * R + 1) * RSTRH + 2 * col;
+                const long o0 = f16_tile_off(2 * R, 2 * col, RB);
+                const long o1 = f16_tile_off(2 * R + 1, 2 * col, RB);
                 hi[o0] = rh; hi[o0 + 1] = -ih; lo[o0] = rl; lo[o0 + 1] = -il;
                 hi[o1] = ih; hi[o1 + 1] = rh;  lo[o1] = il; lo[o1 + 1] = rl;
             }
@@ -125,7 +147,7 @@ __global__ __launch_bounds__(256) void fact16_prep_kernel(Fact16PrepArgs a)
 {
     const int j = blockIdx.x;
     fact16_prep_block(a.plan, a.interp, a.ylm, a.h_t, a.L, j, blockIdx.y,
-                      a.tiles + (long)j * a.tile_bytes, a.RSTRH, a.KB, a.RB);
+                      a.tiles + (long)j * a.tile_bytes, a.KB, a.RB);
 }
 
 int launch_fact16_prep(const Fact16PrepArgs &a, hipStream_t stream)
@@ -143,13 +165,13 @@ __device__ __forceinline__ void split_pack(const double (&vr)[4], const double (
 {
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        const float xr = f32_opaque((float)vr[t]);
-        const float xi = f32_opaque((float)vi[t]);
-        const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
-        bh[2 * t] = hr;
-        bh[2 * t + 1] = hi_;
-        bl[2 * t] = (_Float16)(xr - (float)hr);
-        bl[2 * t + 1] = (_Float16)(xi - (float)hi_);
+        const v2f x = {f32_opaque((float)vr[t]), f32_opaque((float)vi[t])};
+        const v2h h = __builtin_convertvector(x, v2h);  // one packed conversion
+        const v2h l = __builtin_convertvector(x - __builtin_convertvector(h, v2f), v2h);
+        bh[2 * t] = h.x;
+        bh[2 * t + 1] = h.y;
+        bl[2 * t] = l.x;
+        bl[2 * t + 1] = l.y;
     }
 }
 
@@ -166,11 +188,60 @@ constexpr int F16_NB = 2;        // 16-sample column blocks per wave
 // with the full register file, 8 waves two per SIMD.
 constexpr int f16_tile_samples(int waves) { return 16 * F16_NB * waves; }
 
+// One 32-column step of the n-contraction: B fragments from v_n (then v_n
+// advanced by z^16 when another step follows), A fragments from the tile
+// block column at aH / aL, and the three split products per row block.
+template <int RB, bool FIRST>
+__device__ __forceinline__ void f16_kstep(const _Float16 *aH, const _Float16 *aL, bool advance,
+                                          double (&vr)[F16_NB][4], double (&vi)[F16_NB][4],
+                                          const double (&zr16)[F16_NB],
+                                          const double (&zi16)[F16_NB], v4f (&acc)[RB][F16_NB])
+{
+    v8h bh[F16_NB], bl[F16_NB];
+#pragma unroll
+    for (int c = 0; c < F16_NB; c++) split_pack(vr[c], vi[c], bh[c], bl[c]);
+    if (advance) {
+#pragma unroll
+        for (int c = 0; c < F16_NB; c++)
+#pragma unroll
+            for (int t2 = 0; t2 < 4; t2++) {
+                double nr, ni;
+                cmul(vr[c][t2], vi[c][t2], zr16[c], zi16[c], nr, ni);
+                vr[c][t2] = nr;
+                vi[c][t2] = ni;
+            }
+    }
+    // A fragments of the next row block are loaded while the current one
+    // feeds its MFMAs; the scheduling barrier keeps the loads from being
+    // hoisted further (register pressure).
+    v8h ahn = *reinterpret_cast<const v8h *>(aH);
+    v8h aln = *reinterpret_cast<const v8h *>(aL);
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const v8h ah = ahn, al = aln;
+        if (rb + 1 < RB) {
+            ahn = *reinterpret_cast<const v8h *>(aH + (rb + 1) * F16_BLK);
+            aln = *reinterpret_cast<const v8h *>(aL + (rb + 1) * F16_BLK);
+        }
+#pragma unroll
+        for (int c = 0; c < F16_NB; c++)
+            acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                al, bh[c], FIRST ? v4f{0.f, 0.f, 0.f, 0.f} : acc[rb][c], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < F16_NB; c++)
+            acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[c], acc[rb][c], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < F16_NB; c++)
+            acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[c], acc[rb][c], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // One tile (tile q of the interval whose coefficient tile and phase
 // coefficients sit in LDS) of the FP16 split-operand factorised sum.
 template <int RB, int WAVES, bool ARITH>
 __device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const unsigned char *sTile,
-                                                 const double *sPh, int RSTRH, int KB,
+                                                 const double *sPh, int KB,
                                                  const Segment &sg, int q, double dt,
                                                  double2 *wave)
 {
@@ -180,9 +251,10 @@ __device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const uns
     const int wv = tid >> 6;
     const int li = lane & 15;
     const int g = lane >> 4;
+    const long plane = (long)KB * RB * F16_BLK;
     const _Float16 *sHi = reinterpret_cast<const _Float16 *>(sTile);
-    const _Float16 *sLo = sHi + (long)RB * 16 * RSTRH;
-    const float *sInv = reinterpret_cast<const float *>(sLo + (long)RB * 16 * RSTRH);
+    const _Float16 *sLo = sHi + plane;
+    const float *sInv = reinterpret_cast<const float *>(sLo + plane);
     const double inv_pw = 1.0 / sg.pw;
     const double inv_hw = 1.0 / sg.hw;
 
@@ -215,50 +287,12 @@ __device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const uns
     }
 
     v4f acc[RB][F16_NB];
-#pragma unroll
-    for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-        for (int c = 0; c < F16_NB; c++) acc[rb][c] = v4f{0.f, 0.f, 0.f, 0.f};
-
-    const _Float16 *aH = sHi + (long)li * RSTRH + 8 * g;
-    const _Float16 *aL = sLo + (long)li * RSTRH + 8 * g;
-    for (int kb = 0; kb < KB; kb++) {
-        v8h bh[F16_NB], bl[F16_NB];
-#pragma unroll
-        for (int c = 0; c < F16_NB; c++) {
-            split_pack(vr[c], vi[c], bh[c], bl[c]);
-#pragma unroll
-            for (int t2 = 0; t2 < 4; t2++) {
-                double nr, ni;
-                cmul(vr[c][t2], vi[c][t2], zr16[c], zi16[c], nr, ni);
-                vr[c][t2] = nr;
-                vi[c][t2] = ni;
-            }
-        }
-        // A fragments of the next row block are loaded while the current
-        // one feeds its MFMAs; the scheduling barrier keeps the loads
-        // from being hoisted further (register pressure).
-        v8h ahn = *reinterpret_cast<const v8h *>(aH + 32 * kb);
-        v8h aln = *reinterpret_cast<const v8h *>(aL + 32 * kb);
-#pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
-            const v8h ah = ahn, al = aln;
-            if (rb + 1 < RB) {
-                ahn = *reinterpret_cast<const v8h *>(aH + (long)(rb + 1) * 16 * RSTRH + 32 * kb);
-                aln = *reinterpret_cast<const v8h *>(aL + (long)(rb + 1) * 16 * RSTRH + 32 * kb);
-            }
-#pragma unroll
-            for (int c = 0; c < F16_NB; c++)
-                acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[c], acc[rb][c], 0, 0, 0);
-#pragma unroll
-            for (int c = 0; c < F16_NB; c++)
-                acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[c], acc[rb][c], 0, 0, 0);
-#pragma unroll
-            for (int c = 0; c < F16_NB; c++)
-                acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[c], acc[rb][c], 0, 0, 0);
-            __builtin_amdgcn_sched_barrier(0);
-        }
-    }
+    const int lo_lane = f16_lane_off(li, g);
+    // kb = 0 starts the accumulators from zero (no separate clearing)
+    f16_kstep<RB, true>(sHi + lo_lane, sLo + lo_lane, KB > 1, vr, vi, zr16, zi16, acc);
+    for (int kb = 1; kb < KB; kb++)
+        f16_kstep<RB, false>(sHi + kb * RB * F16_BLK + lo_lane, sLo + kb * RB * F16_BLK + lo_lane,
+                             kb + 1 < KB, vr, vi, zr16, zi16, acc);
 
     // ---- epilogue: h = sum_{r,p} s^p u_r W / (256 sigma_R)
     const int f = g & 1;
@@ -276,36 +310,42 @@ __device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const uns
         const float spa = (p0 == 0 ? 1.f : s2f) * (1.0f / 256.0f);
         const float spb = (p0 == 0 ? s1f : s2f * s1f) * (1.0f / 256.0f);
         double hre = 0.0, him = 0.0;
-        double u_re = 0.0, u_im = 0.0, e_re = 0.0, e_im = 0.0;
         if constexpr (ARITH) {
+            // Rows of one lane are r = 2 rb + (g >> 1) with m_r = m_first + 2 rb:
+            // sum_rb u_r z_rb = u_first * sum_rb e^rb z_rb (e = exp(-2 i Phi_phi)),
+            // summed by Horner's rule in FP64.
             double sn, cs;
+            sincos_turns(reduce_turns(2.0 * uph), sn, cs);
+            const double e_re = cs, e_im = -sn;
+#pragma unroll
+            for (int rb = RB - 1; rb >= 0; rb--) {
+                // rows r >= Nr hold zeros (W = 0, scale 1)
+                const int R0 = 8 * rb + 2 * g;
+                const float ka = spa * sInv[R0];
+                const float kb2 = spb * sInv[R0 + 1];
+                const v4f W = acc[rb][c];
+                const double zr = (double)(ka * W[0] + kb2 * W[2]);
+                const double zi = (double)(ka * W[1] + kb2 * W[3]);
+                const double nr = fma(hre, e_re, fma(-him, e_im, zr));
+                const double ni = fma(hre, e_im, fma(him, e_re, zi));
+                hre = nr;
+                him = ni;
+            }
             const double m_first = (double)(P.m0 + (g >> 1));
             sincos_turns(reduce_turns(m_first * uph + (double)P.k0 * uth), sn, cs);
-            u_re = cs;
-            u_im = -sn;
-            sincos_turns(reduce_turns(2.0 * uph), sn, cs);
-            e_re = cs;
-            e_im = -sn;
-        }
+            double nr, ni;
+            cmul(cs, -sn, hre, him, nr, ni);
+            hre = nr;
+            him = ni;
+        } else {
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
-            const int r = 2 * rb + (g >> 1);
-            float ur, ui;
-            if constexpr (ARITH) {
-                ur = (float)u_re;
-                ui = (float)u_im;
-                double nr, ni;
-                cmul(u_re, u_im, e_re, e_im, nr, ni);
-                u_re = nr;
-                u_im = ni;
-            } else {
+            for (int rb = 0; rb < RB; rb++) {
+                const int r = 2 * rb + (g >> 1);
                 const int2 mk = r < P.Nr ? P.row_mk[r] : make_int2(0, 0);
                 double sn, cs;
                 sincos_turns(reduce_turns((double)mk.x * uph + (double)mk.y * uth), sn, cs);
-                ur = (float)cs;
-                ui = (float)(-sn);
-            }
-            if (r < P.Nr) {
+                const float ur = (float)cs, ui = (float)(-sn);
+                // rows r >= Nr hold zeros (W = 0, scale 1)
                 const int R0 = 8 * rb + 2 * g;
                 const float ka = spa * sInv[R0];
                 const float kb2 = spb * sInv[R0 + 1];
@@ -364,7 +404,7 @@ void fact16_kernel(Fact16Args a)
             __syncthreads();
             stale = false;
         }
-        fact16_tile_body<RB, WAVES, ARITH>(a.plan, sTile, sPh, a.RSTRH, a.KB, a.segs[j],
+        fact16_tile_body<RB, WAVES, ARITH>(a.plan, sTile, sPh, a.KB, a.segs[j],
                                            tile - a.prefix[j], a.dt, a.wave);
     }
 }
@@ -373,7 +413,7 @@ void fact16_kernel(Fact16Args a)
 template <int RB, int WAVES, bool ARITH>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
 void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned char *tiles,
-                         long tile_bytes, int RSTRH, int KB)
+                         long tile_bytes, int KB)
 {
     constexpr int NT = 64 * WAVES;
     extern __shared__ __attribute__((aligned(16))) unsigned char sTile[];
@@ -415,15 +455,14 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
             __syncthreads();
             stale = false;
         }
-        fact16_tile_body<RB, WAVES, ARITH>(P, sTile, sPh, RSTRH, KB, r.segs[d.seg_off + j],
+        fact16_tile_body<RB, WAVES, ARITH>(P, sTile, sPh, KB, r.segs[d.seg_off + j],
                                            lt - pref[j], d.dt, p.wave + d.wave_off / 2);
     }
 }
 
 __global__ __launch_bounds__(256) void batch_fact16_prep_kernel(BatchDev B, BatchPtrs p,
                                                                  unsigned char *tiles,
-                                                                 long tile_bytes, int RSTRH, int KB,
-                                                                 int RB)
+                                                                 long tile_bytes, int KB, int RB)
 {
     const int gi = blockIdx.x;
     const int b = upper_index(B.seg_prefix, B.B, gi);
@@ -431,7 +470,7 @@ __global__ __launch_bounds__(256) void batch_fact16_prep_kernel(BatchDev B, Batc
     const int j = gi - B.seg_prefix[b];
     fact16_prep_block(batch_wave_plan(B, d), p.interp + d.interp_off,
                       reinterpret_cast<const double2 *>(p.ylm + d.ylm_off), p.t + d.t_off, d.L, j,
-                      blockIdx.y, tiles + (long)gi * tile_bytes, RSTRH, KB, RB);
+                      blockIdx.y, tiles + (long)gi * tile_bytes, KB, RB);
 }
 
 int fact16_tile(int variant) { return 16 * F16_NB * (variant == 4 ? 4 : 8); }
@@ -494,7 +533,7 @@ int launch_fact16_sum(const Fact16Args &a, int RB, int max_blocks, int variant, 
 
 template <int WAVES, bool ARITH>
 static int launch_batch_fact16_w(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
-                                 const unsigned char *tiles, long tile_bytes, int RSTRH, int KB,
+                                 const unsigned char *tiles, long tile_bytes, int KB,
                                  int blocks, size_t lds, hipStream_t stream)
 {
     switch (B.uni.RB) {
@@ -503,7 +542,7 @@ static int launch_batch_fact16_w(const BatchDev &B, const BatchRunDev &r, const 
         FEW_HIP(hipFuncSetAttribute((const void *)batch_fact16_kernel<X, WAVES, ARITH>,          \
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
         hipLaunchKernelGGL((batch_fact16_kernel<X, WAVES, ARITH>), dim3(blocks),                \
-                           dim3(64 * WAVES), lds, stream, B, r, p, tiles, tile_bytes, RSTRH, KB); \
+                           dim3(64 * WAVES), lds, stream, B, r, p, tiles, tile_bytes, KB);        \
         break;
         FEW_CASEB(1) FEW_CASEB(2) FEW_CASEB(3) FEW_CASEB(4) FEW_CASEB(5) FEW_CASEB(6)
         FEW_CASEB(7) FEW_CASEB(8) FEW_CASEB(9) FEW_CASEB(10) FEW_CASEB(11) FEW_CASEB(12)
@@ -518,14 +557,14 @@ static int launch_batch_fact16_w(const BatchDev &B, const BatchRunDev &r, const 
 }
 
 int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
-                        unsigned char *tiles, long tile_bytes, int RSTRH, int KB, int variant,
+                        unsigned char *tiles, long tile_bytes, int KB, int variant,
                         int stage, hipStream_t stream)
 {
     if (stage & 1) {
         if (B.total_intervals > 0) {
             dim3 grid(B.total_intervals, 2 * B.uni.RB);
             hipLaunchKernelGGL(batch_fact16_prep_kernel, grid, dim3(256), 0, stream, B, p, tiles,
-                               tile_bytes, RSTRH, KB, B.uni.RB);
+                               tile_bytes, KB, B.uni.RB);
             FEW_HIP(hipGetLastError());
         }
     }
@@ -538,10 +577,10 @@ int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs
     const int blocks = device_cus16() * (lds <= 80 * 1024 ? 2 : 1);
     const bool arith = B.uni.rows_arith != 0;
     if (variant == 4)
-        return arith ? launch_batch_fact16_w<4, true>(B, r, p, tiles, tile_bytes, RSTRH, KB, blocks, lds, stream)
-                     : launch_batch_fact16_w<4, false>(B, r, p, tiles, tile_bytes, RSTRH, KB, blocks, lds, stream);
-    return arith ? launch_batch_fact16_w<8, true>(B, r, p, tiles, tile_bytes, RSTRH, KB, blocks, lds, stream)
-                 : launch_batch_fact16_w<8, false>(B, r, p, tiles, tile_bytes, RSTRH, KB, blocks, lds, stream);
+        return arith ? launch_batch_fact16_w<4, true>(B, r, p, tiles, tile_bytes, KB, blocks, lds, stream)
+                     : launch_batch_fact16_w<4, false>(B, r, p, tiles, tile_bytes, KB, blocks, lds, stream);
+    return arith ? launch_batch_fact16_w<8, true>(B, r, p, tiles, tile_bytes, KB, blocks, lds, stream)
+                 : launch_batch_fact16_w<8, false>(B, r, p, tiles, tile_bytes, KB, blocks, lds, stream);
 }
 
 }  // namespace few
