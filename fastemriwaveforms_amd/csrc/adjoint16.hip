@@ -12,11 +12,12 @@
 // (three products, FP32 accumulation), exactly like the forward kernel.
 //
 // One workgroup owns one chunk of <= A16_CHUNK samples of one interval and
-// NBC waves (one 16-column block of n each).  Per 16-sample step the workgroup
-// builds in LDS: the phasor tables u (Nr x 16), w = sigma^s * scale * conj(g)
-// (7 x 16), the split v table (NBC*16 columns x 16 samples), and from u and w
-// the split A tile (rows 2R, 2R+1 of complex row R = 7 r + s); then every
-// wave runs its RBM x 3 MFMAs.  Phases are FP64 (turns, reduced), sincos is
+// NBC * NH waves (16-column block cb of n, every NH-th 16-row block).  Per
+// 16-sample step the workgroup builds in LDS, one job per thread, the split
+// A tile (rows 2R, 2R+1 of complex row R = 7 r + s, a = sigma^s * scale *
+// conj(g) * u_r; a job is one (sample, r)) and the split v table (NBC*16
+// columns x 16 samples; a job is one (sample, 8 columns)); then every wave
+// runs its (RBM / NH) x 3 MFMAs.  Phases are FP64 (turns, reduced), sincos is
 // the FP64 sincos_turns.  The chunk's FP32 moments are unscaled (powers of
 // two, exact) and written per chunk; later kernels add the chunks of each
 // interval in FP64 in a fixed order and finish per mode:
@@ -40,15 +41,19 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 #define A16_AST 40             // A-tile row stride in halfs (32 + 8 padding)
 #define A16_VST 40             // v-table column stride in halfs (32 + 8 padding: the 16 lanes of a
                                // fragment read start on distinct 4-bank groups)
+#define A16_VG 8               // v-table columns per job: one sincos, then the z recurrence
 #define INV_2PI_A 0.15915494309189533577
 
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef _Float16 v2h __attribute__((ext_vector_type(2)));
+
 struct A16Layout {
-    size_t a_hi, a_lo, v_hi, v_lo, u, w, pp, pt, pr, sig, cg, ph, red, total;
+    size_t a_hi, a_lo, v_hi, v_lo, pp, pt, pr, sig, cg, z, ph, red, total;
 };
 
 static __host__ __device__ inline size_t a16_align(size_t x) { return (x + 15) & ~(size_t)15; }
 
-__host__ __device__ inline A16Layout a16_layout(int RBM, int NBC, int Nr)
+__host__ __device__ inline A16Layout a16_layout(int RBM, int NBC, int /*Nr*/)
 {
     A16Layout L;
     size_t o = 0;
@@ -56,13 +61,12 @@ __host__ __device__ inline A16Layout a16_layout(int RBM, int NBC, int Nr)
     L.a_lo = o; o = a16_align(o + (size_t)RBM * 16 * A16_AST * 2);
     L.v_hi = o; o = a16_align(o + (size_t)NBC * 16 * A16_VST * 2);
     L.v_lo = o; o = a16_align(o + (size_t)NBC * 16 * A16_VST * 2);
-    L.u = o;    o = a16_align(o + (size_t)Nr * 16 * 8);
-    L.w = o;    o = a16_align(o + (size_t)A16_MOM * 16 * 8);
     L.pp = o;   o = a16_align(o + (size_t)A16_SUPER * 8);
     L.pt = o;   o = a16_align(o + (size_t)A16_SUPER * 8);
     L.pr = o;   o = a16_align(o + (size_t)A16_SUPER * 8);
     L.sig = o;  o = a16_align(o + (size_t)A16_SUPER * 4);
     L.cg = o;   o = a16_align(o + (size_t)A16_SUPER * 8);
+    L.z = o;    o = a16_align(o + (size_t)A16_SUPER * 16);
     L.ph = o;   o = a16_align(o + 24 * 8);
     L.red = o;  o = a16_align(o + 16 * 4);
     L.total = o;
@@ -71,21 +75,33 @@ __host__ __device__ inline A16Layout a16_layout(int RBM, int NBC, int Nr)
 
 size_t adjoint16_lds_bytes(int RBM, int NBC, int Nr) { return a16_layout(RBM, NBC, Nr).total; }
 
-__device__ __forceinline__ void split16(float x, _Float16 &h, _Float16 &l)
+// Waves per 16-column block: two (each taking every other row block) while
+// the workgroup stays within 8 waves, else one.
+static inline int a16_nh(int NBC) { return NBC <= 4 ? 2 : 1; }
+
+// hi/lo FP16 parts of a pair of floats (one packed conversion each)
+__device__ __forceinline__ void split_pair(v2f x, v2h &h, v2h &l)
 {
-    h = (_Float16)x;
-    l = (_Float16)(x - (float)h);
+    h = __builtin_convertvector(x, v2h);
+    l = __builtin_convertvector(x - __builtin_convertvector(h, v2f), v2h);
 }
 
-template <int RBM>
+// One workgroup = one chunk of samples.  Per 16-sample step every thread
+// takes one job -- (sample, row r): u_r and the 7 moment rows of the A tile;
+// or (sample, 8 columns): the split v table -- then each wave (column block
+// cb, row parity `part`) runs its MFMAs.  Two barriers per step.
+template <int RBM, int NH>
 __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
 {
+    constexpr int RBH = (RBM + NH - 1) / NH;  // row blocks per wave
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     const FactPlanDev &P = a.plan;
     const int NBC = a.NBC;
-    const int NT = 64 * NBC;
+    const int NW = NBC * NH;
+    const int NT = 64 * NW;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6, li = lane & 15, g = lane >> 4;
+    const int cb = wv % NBC, part = wv / NBC;
     const int c = blockIdx.x;
     const int total = a.prefix[a.nseg];
     if (c >= total) return;  // uniform over the workgroup
@@ -99,13 +115,12 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
     _Float16 *sAl = reinterpret_cast<_Float16 *>(smem + Lay.a_lo);
     _Float16 *sVh = reinterpret_cast<_Float16 *>(smem + Lay.v_hi);
     _Float16 *sVl = reinterpret_cast<_Float16 *>(smem + Lay.v_lo);
-    float2 *sU = reinterpret_cast<float2 *>(smem + Lay.u);
-    float2 *sW = reinterpret_cast<float2 *>(smem + Lay.w);
     double *sPp = reinterpret_cast<double *>(smem + Lay.pp);
     double *sPt = reinterpret_cast<double *>(smem + Lay.pt);
     double *sPr = reinterpret_cast<double *>(smem + Lay.pr);
     float *sSig = reinterpret_cast<float *>(smem + Lay.sig);
     float2 *sCg = reinterpret_cast<float2 *>(smem + Lay.cg);
+    double2 *sZ = reinterpret_cast<double2 *>(smem + Lay.z);
     double *sPh = reinterpret_cast<double *>(smem + Lay.ph);
     float *sRed = reinterpret_cast<float *>(smem + Lay.red);
 
@@ -128,7 +143,7 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
     }
     __syncthreads();
     float gmax = 0.f;
-    for (int w = 0; w < NBC; w++) gmax = fmaxf(gmax, sRed[w]);
+    for (int w = 0; w < NW; w++) gmax = fmaxf(gmax, sRed[w]);
     int ex = 0;
     if (gmax > 0.f) frexpf(gmax, &ex);  // gmax in [2^(ex-1), 2^ex)
     const float scale = ldexpf(1.f, 13 - ex);
@@ -136,9 +151,15 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
 
     const double inv_pw = 1.0 / sg.pw;
     const double inv_hw = 1.0 / sg.hw;
-    v4f acc[RBM];
+    v4f acc[RBH];
 #pragma unroll
-    for (int rb = 0; rb < RBM; rb++) acc[rb] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < RBH; k++) acc[k] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    // job slots: (sample, row) jobs, then (sample, column group) jobs, each
+    // kind starting on a wave boundary so no wave mixes the two
+    const int u_slots = ((Nr * 16 + 63) >> 6) << 6;
+    const int v_jobs = (NBC * 16 / A16_VG) * 16;
+    const int n_slots = u_slots + (((v_jobs + 63) >> 6) << 6);
 
     for (int sbase = lo; sbase < hi; sbase += A16_SUPER) {
         __syncthreads();
@@ -148,7 +169,11 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
             const double s = (t - sg.pt) * inv_pw;
             sPp[q] = dense_phase(sPh, s, 1.0 - s) * INV_2PI_A;
             sPt[q] = P.has_k ? dense_phase(sPh + 8, s, 1.0 - s) * INV_2PI_A : 0.0;
-            sPr[q] = dense_phase(sPh + 16, s, 1.0 - s) * INV_2PI_A;
+            const double pr = dense_phase(sPh + 16, s, 1.0 - s) * INV_2PI_A;
+            sPr[q] = pr;
+            double zs, zc;
+            sincos_turns(reduce_turns(pr), zs, zc);
+            sZ[q] = make_double2(zc, -zs);  // exp(-i Phi_r)
             sSig[q] = (float)((t - sg.h_t) * inv_hw);
             float2 cg = make_float2(0.f, 0.f);
             if (i < hi) {
@@ -161,97 +186,97 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
         for (int st = 0; st < nsteps; st++) {
             const int i0 = st * 16;
             __syncthreads();
-            // u_r(t_i)
-            for (int e = tid; e < Nr * 16; e += NT) {
-                const int r = e >> 4, ii = e & 15;
-                const int2 mk = P.row_mk[r];
-                double sn, cs;
-                sincos_turns(reduce_turns((double)mk.x * sPp[i0 + ii] + (double)mk.y * sPt[i0 + ii]),
-                             sn, cs);
-                sU[e] = make_float2((float)cs, (float)(-sn));
-            }
-            // w_s = sigma^s * scale * conj(g)
-            for (int e = tid; e < A16_MOM * 16; e += NT) {
-                const int s = e >> 4, ii = e & 15;
-                const float sg1 = sSig[i0 + ii];
-                float pw = 1.f;
-                for (int k2 = 0; k2 < s; k2++) pw *= sg1;
-                const float2 cg = sCg[i0 + ii];
-                sW[e] = make_float2(pw * cg.x, pw * cg.y);
-            }
-            // v_n(t_i) * 256, split: four consecutive columns per thread
-            for (int e = tid; e < NBC * 4 * 16; e += NT) {
-                const int ii = e & 15, cg4 = e >> 4;
-                const double ph = sPr[i0 + ii];
-                double sn, cs, zs, zc;
-                sincos_turns(reduce_turns((double)(P.n_lo + 4 * cg4) * ph), sn, cs);
-                sincos_turns(reduce_turns(ph), zs, zc);
-                double vr = 256.0 * cs, vi = -256.0 * sn;
-                const double zr = zc, zi = -zs;
+            for (int slot = tid; slot < n_slots; slot += NT) {
+                if (slot < u_slots) {
+                    // A tile rows of complex row R = 7 r + s: a = sigma^s scale conj(g) u_r;
+                    // each (re, im) pair of a row is one 32-bit LDS store
+                    const int r = slot >> 4, ii = slot & 15;
+                    if (r < Nr) {
+                        const int2 mk = P.row_mk[r];
+                        double sn, cs;
+                        sincos_turns(reduce_turns((double)mk.x * sPp[i0 + ii] +
+                                                  (double)mk.y * sPt[i0 + ii]),
+                                     sn, cs);
+                        const float ux = (float)cs, uy = (float)(-sn);
+                        const float sg1 = sSig[i0 + ii];
+                        const float2 cgv = sCg[i0 + ii];
+                        float pw = 1.f;
 #pragma unroll
-                for (int t2 = 0; t2 < 4; t2++) {
-                    const int col = 4 * cg4 + t2;
-                    const bool in = col < P.Nc;
-                    _Float16 hr, lr, hi_, li_;
-                    split16(in ? f32_opaque((float)vr) : 0.f, hr, lr);
-                    split16(in ? f32_opaque((float)vi) : 0.f, hi_, li_);
-                    const int o = col * A16_VST + 2 * ii;
-                    typedef _Float16 h2v __attribute__((ext_vector_type(2)));
-                    *reinterpret_cast<h2v *>(sVh + o) = h2v{hr, hi_};
-                    *reinterpret_cast<h2v *>(sVl + o) = h2v{lr, li_};
-                    const double nr = vr * zr - vi * zi;
-                    const double ni = vr * zi + vi * zr;
-                    vr = nr;
-                    vi = ni;
+                        for (int sm = 0; sm < A16_MOM; sm++) {
+                            const float wx = pw * cgv.x, wy = pw * cgv.y;
+                            pw *= sg1;
+                            const v2f av = {wx * ux - wy * uy, wx * uy + wy * ux};
+                            v2h h, l;
+                            split_pair(av, h, l);
+                            const int o0 = (2 * (A16_MOM * r + sm)) * A16_AST + 2 * ii;
+                            const int o1 = o0 + A16_AST;
+                            *reinterpret_cast<v2h *>(sAh + o0) = v2h{h.x, -h.y};
+                            *reinterpret_cast<v2h *>(sAh + o1) = v2h{h.y, h.x};
+                            *reinterpret_cast<v2h *>(sAl + o0) = v2h{l.x, -l.y};
+                            *reinterpret_cast<v2h *>(sAl + o1) = v2h{l.y, l.x};
+                        }
+                    }
+                } else {
+                    // v_n(t_i) * 256, split, for A16_VG consecutive columns
+                    const int e = slot - u_slots;
+                    if (e < v_jobs) {
+                        const int ii = e & 15, grp = e >> 4;
+                        double sn, cs;
+                        sincos_turns(reduce_turns((double)(P.n_lo + A16_VG * grp) * sPr[i0 + ii]), sn,
+                                     cs);
+                        double vr = 256.0 * cs, vi = -256.0 * sn;
+                        const double2 z = sZ[i0 + ii];
+#pragma unroll
+                        for (int t2 = 0; t2 < A16_VG; t2++) {
+                            const int col = A16_VG * grp + t2;
+                            const bool in = col < P.Nc;
+                            const v2f xv = {in ? f32_opaque((float)vr) : 0.f,
+                                            in ? f32_opaque((float)vi) : 0.f};
+                            v2h h, l;
+                            split_pair(xv, h, l);
+                            const int o = col * A16_VST + 2 * ii;
+                            *reinterpret_cast<v2h *>(sVh + o) = h;
+                            *reinterpret_cast<v2h *>(sVl + o) = l;
+                            if (t2 + 1 < A16_VG) {
+                                const double nr = vr * z.x - vi * z.y;
+                                const double ni = vr * z.y + vi * z.x;
+                                vr = nr;
+                                vi = ni;
+                            }
+                        }
+                    }
                 }
             }
             __syncthreads();
-            // A tile: complex row R = 7 r + s, sample ii: a = w_s u_r.  A thread
-            // keeps its sample ii and walks the rows; each (re, im) pair of a
-            // row is one 32-bit LDS store.
-            {
-                const int ii = tid & 15;
-                for (int R = tid >> 4; R < A16_MOM * Nr; R += NT >> 4) {
-                    const int r = R / A16_MOM, s = R - A16_MOM * r;
-                    const float2 w = sW[s * 16 + ii];
-                    const float2 u = sU[r * 16 + ii];
-                    const float ar = w.x * u.x - w.y * u.y;
-                    const float ai = w.x * u.y + w.y * u.x;
-                    _Float16 hr, lr, hi_, li_;
-                    split16(ar, hr, lr);
-                    split16(ai, hi_, li_);
-                    const int o0 = (2 * R) * A16_AST + 2 * ii;
-                    const int o1 = o0 + A16_AST;
-                    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-                    *reinterpret_cast<h2 *>(sAh + o0) = h2{hr, -hi_};
-                    *reinterpret_cast<h2 *>(sAh + o1) = h2{hi_, hr};
-                    *reinterpret_cast<h2 *>(sAl + o0) = h2{lr, -li_};
-                    *reinterpret_cast<h2 *>(sAl + o1) = h2{li_, lr};
-                }
-            }
-            __syncthreads();
-            const v8h bh = *reinterpret_cast<const v8h *>(sVh + (16 * wv + li) * A16_VST + 8 * g);
-            const v8h bl = *reinterpret_cast<const v8h *>(sVl + (16 * wv + li) * A16_VST + 8 * g);
+            const v8h bh = *reinterpret_cast<const v8h *>(sVh + (16 * cb + li) * A16_VST + 8 * g);
+            const v8h bl = *reinterpret_cast<const v8h *>(sVl + (16 * cb + li) * A16_VST + 8 * g);
             const _Float16 *aH = sAh + li * A16_AST + 8 * g;
             const _Float16 *aL = sAl + li * A16_AST + 8 * g;
 #pragma unroll
-            for (int rb = 0; rb < RBM; rb++) {
-                const v8h ah = *reinterpret_cast<const v8h *>(aH + rb * 16 * A16_AST);
-                const v8h al = *reinterpret_cast<const v8h *>(aL + rb * 16 * A16_AST);
-                acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[rb], 0, 0, 0);
-                acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[rb], 0, 0, 0);
-                acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[rb], 0, 0, 0);
+            for (int k = 0; k < RBH; k++) {
+                const int rb = part + NH * k;
+                if (RBM % NH == 0 || rb < RBM) {
+                    const v8h ah = *reinterpret_cast<const v8h *>(aH + rb * 16 * A16_AST);
+                    const v8h al = *reinterpret_cast<const v8h *>(aL + rb * 16 * A16_AST);
+                    acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc[k], 0, 0, 0);
+                    acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc[k], 0, 0, 0);
+                    acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[k], 0, 0, 0);
+                }
             }
         }
     }
-    // chunk moments: D[16 rb + 4 g + v][16 wv + li] / (scale * 256)
+    // chunk moments: D[16 rb + 4 g + v][16 cb + li] / (scale * 256)
     const int ncol = NBC * 16;
     float *out = a.part + (long)c * RBM * 16 * ncol;
 #pragma unroll
-    for (int rb = 0; rb < RBM; rb++)
+    for (int k = 0; k < RBH; k++) {
+        const int rb = part + NH * k;
+        if (RBM % NH == 0 || rb < RBM) {
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-            out[(long)(16 * rb + 4 * g + v) * ncol + 16 * wv + li] = acc[rb][v] * inv_out;
+            for (int v = 0; v < 4; v++)
+                out[(long)(16 * rb + 4 * g + v) * ncol + 16 * cb + li] = acc[k][v] * inv_out;
+        }
+    }
 }
 
 // M_j = sum over the chunks of interval j (FP64, chunk order); zero for
@@ -398,13 +423,20 @@ int launch_adjoint16(const Adj16Args &a, int max_chunks, hipStream_t stream)
         set_error("mixed adjoint: tile of %zu LDS bytes / %d column blocks unsupported", lds, a.NBC);
         return FEW_ERR_UNSUPPORTED;
     }
+    const int nh = a16_nh(a.NBC);
+    const dim3 block(64 * a.NBC * nh);
     switch (a.RBM) {
 #define A16_CASE(X)                                                                             \
     case X:                                                                                     \
-        FEW_HIP(hipFuncSetAttribute((const void *)adjoint16_kernel<X>,                          \
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));     \
-        hipLaunchKernelGGL(adjoint16_kernel<X>, dim3(max_chunks), dim3(64 * a.NBC), lds, stream, \
-                           a);                                                                  \
+        if (nh == 2) {                                                                          \
+            FEW_HIP(hipFuncSetAttribute((const void *)adjoint16_kernel<X, 2>,                   \
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+            hipLaunchKernelGGL((adjoint16_kernel<X, 2>), dim3(max_chunks), block, lds, stream, a); \
+        } else {                                                                                \
+            FEW_HIP(hipFuncSetAttribute((const void *)adjoint16_kernel<X, 1>,                   \
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+            hipLaunchKernelGGL((adjoint16_kernel<X, 1>), dim3(max_chunks), block, lds, stream, a); \
+        }                                                                                       \
         break;
         A16_CASE(1) A16_CASE(2) A16_CASE(3) A16_CASE(4) A16_CASE(5) A16_CASE(6) A16_CASE(7)
         A16_CASE(8) A16_CASE(9) A16_CASE(10) A16_CASE(11) A16_CASE(12) A16_CASE(13) A16_CASE(14)
