@@ -44,9 +44,6 @@ typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 #define A16_VG 8               // v-table columns per job: one sincos, then the z recurrence
 #define INV_2PI_A 0.15915494309189533577
 
-typedef float v2f __attribute__((ext_vector_type(2)));
-typedef _Float16 v2h __attribute__((ext_vector_type(2)));
-
 struct A16Layout {
     size_t a_hi, a_lo, v_hi, v_lo, pp, pt, pr, sig, cg, z, ph, red, total;
 };
@@ -83,7 +80,7 @@ static inline int a16_nh(int NBC) { return NBC <= 4 ? 2 : 1; }
 __device__ __forceinline__ void split_pair(v2f x, v2h &h, v2h &l)
 {
     h = __builtin_convertvector(x, v2h);
-    l = __builtin_convertvector(x - __builtin_convertvector(h, v2f), v2h);
+    l = f16_split_lo(h, x);
 }
 
 // One workgroup = one chunk of samples.  Per 16-sample step every thread
@@ -172,7 +169,7 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
             const double pr = dense_phase(sPh + 16, s, 1.0 - s) * INV_2PI_A;
             sPr[q] = pr;
             double zs, zc;
-            sincos_turns(reduce_turns(pr), zs, zc);
+            sincos_turns_short(reduce_turns(pr), zs, zc);
             sZ[q] = make_double2(zc, -zs);  // exp(-i Phi_r)
             sSig[q] = (float)((t - sg.h_t) * inv_hw);
             float2 cg = make_float2(0.f, 0.f);
@@ -194,7 +191,7 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
                     if (r < Nr) {
                         const int2 mk = P.row_mk[r];
                         double sn, cs;
-                        sincos_turns(reduce_turns((double)mk.x * sPp[i0 + ii] +
+                        sincos_turns_short(reduce_turns((double)mk.x * sPp[i0 + ii] +
                                                   (double)mk.y * sPt[i0 + ii]),
                                      sn, cs);
                         const float ux = (float)cs, uy = (float)(-sn);
@@ -222,7 +219,7 @@ __global__ __launch_bounds__(512) void adjoint16_kernel(Adj16Args a)
                     if (e < v_jobs) {
                         const int ii = e & 15, grp = e >> 4;
                         double sn, cs;
-                        sincos_turns(reduce_turns((double)(P.n_lo + A16_VG * grp) * sPr[i0 + ii]), sn,
+                        sincos_turns_short(reduce_turns((double)(P.n_lo + A16_VG * grp) * sPr[i0 + ii]), sn,
                                      cs);
                         double vr = 256.0 * cs, vi = -256.0 * sn;
                         const double2 z = sZ[i0 + ii];

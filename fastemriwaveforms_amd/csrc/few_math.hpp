@@ -52,6 +52,40 @@ __host__ __device__ inline void sincos_poly(double x, int q, double &s, double &
     c = ((qi + 1) & 2) ? -c0 : c0;
 }
 
+// Shorter variant for the mixed-precision (FP16-split) paths, whose values
+// carry ~1e-7 relative error: Taylor series to x^11 and x^12 (truncation below
+// (pi/4)^13/13! ~ 7e-12), same quadrant handling.
+__host__ __device__ inline void sincos_poly_short(double x, int q, double &s, double &c)
+{
+    const double x2 = x * x;
+    double ps = -2.5052108385441718775e-08;           // -1/11!
+    ps = ps * x2 + 2.7557319223985890653e-06;         // 1/9!
+    ps = ps * x2 - 1.9841269841269841270e-04;         // -1/7!
+    ps = ps * x2 + 8.3333333333333333333e-03;         // 1/5!
+    ps = ps * x2 - 1.6666666666666666667e-01;         // -1/3!
+    const double sn = x + x * x2 * ps;
+    double pc = -2.0876756987868098979e-09;           // -1/12!
+    pc = pc * x2 + 2.7557319223985890653e-07;         // 1/10!
+    pc = pc * x2 - 2.4801587301587301587e-05;         // -1/8!
+    pc = pc * x2 + 1.3888888888888888889e-03;         // 1/6!
+    pc = pc * x2 - 4.1666666666666666667e-02;         // -1/4!
+    pc = pc * x2 + 0.5;
+    const double cs = 1.0 - x2 * pc;
+    const int qi = q & 3;
+    const bool swap = qi & 1;
+    const double s0 = swap ? cs : sn;
+    const double c0 = swap ? sn : cs;
+    s = (qi & 2) ? -s0 : s0;
+    c = ((qi + 1) & 2) ? -c0 : c0;
+}
+
+__host__ __device__ inline void sincos_turns_short(double f, double &s, double &c)
+{
+    const double q = rint(4.0 * f);
+    const double r = f - 0.25 * q;
+    sincos_poly_short(r * 6.283185307179586476925286766559, (int)q, s, c);
+}
+
 __host__ __device__ inline void sincos_turns(double f, double &s, double &c)
 {
     const double q = rint(4.0 * f);
@@ -87,6 +121,23 @@ __device__ __forceinline__ float f32_opaque(float x)
 {
     asm volatile("" : "+v"(x));
     return x;
+}
+
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef _Float16 v2h __attribute__((ext_vector_type(2)));
+
+// Low FP16 parts of a pair x split as h = fp16(x): fp16(x - h) per lane,
+// computed by v_fma_mix{lo,hi}_f16 (x - h is exact in FP32, so the one
+// rounding is the FP16 one, as in fp16((float)x - (float)h)); two
+// instructions instead of two conversions back, two subtractions and a pack.
+__device__ __forceinline__ v2h f16_split_lo(v2h h, v2f x)
+{
+    v2h r;
+    asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]\n\t"
+        "v_fma_mixhi_f16 %0, %1, -1.0, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+        : "=&v"(r)
+        : "v"(h), "v"(x.x), "v"(x.y));
+    return r;
 }
 
 }  // namespace few

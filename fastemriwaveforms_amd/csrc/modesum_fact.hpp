@@ -119,6 +119,16 @@ struct Fact16PrepArgs {
     FactPlanDev plan;
 };
 
+// Wave priority of the mixed16 sum kernels (modesum_fact16.hip): with prio
+// a wave raises its priority over its MFMA contraction, so the two waves
+// sharing a SIMD interleave one's MFMAs with the other's VALU work.
+struct F16Sched {
+    int prio;
+};
+// variant: 0 = 8 waves with priority, 4 = 4 waves with priority,
+// 8 = 8 waves without
+inline F16Sched fact16_sched(int variant) { return F16Sched{variant == 8 ? 0 : 1}; }
+
 struct Fact16Args {
     double2 *wave;
     const double *pcoef;
@@ -130,6 +140,7 @@ struct Fact16Args {
     long tile_bytes;
     int KB;
     FactPlanDev plan;
+    F16Sched sched;
 };
 
 int launch_fact16_prep(const Fact16PrepArgs &a, hipStream_t stream);

@@ -32,9 +32,7 @@
 namespace few {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
-typedef float v2f __attribute__((ext_vector_type(2)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
-typedef _Float16 v2h __attribute__((ext_vector_type(2)));
 
 // Halfs per 16 x 32 block of a tile plane.
 constexpr int F16_BLK = 512;
@@ -167,7 +165,7 @@ __device__ __forceinline__ void split_pack(const double (&vr)[4], const double (
     for (int t = 0; t < 4; t++) {
         const v2f x = {f32_opaque((float)vr[t]), f32_opaque((float)vi[t])};
         const v2h h = __builtin_convertvector(x, v2h);  // one packed conversion
-        const v2h l = __builtin_convertvector(x - __builtin_convertvector(h, v2f), v2h);
+        const v2h l = f16_split_lo(h, x);
         bh[2 * t] = h.x;
         bh[2 * t + 1] = h.y;
         bl[2 * t] = l.x;
@@ -238,43 +236,58 @@ __device__ __forceinline__ void f16_kstep(const _Float16 *aH, const _Float16 *aL
 }
 
 // One tile (tile q of the interval whose coefficient tile and phase
-// coefficients sit in LDS) of the FP16 split-operand factorised sum.
-template <int RB, int WAVES, bool ARITH>
-__device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const unsigned char *sTile,
-                                                 const double *sPh, int KB,
-                                                 const Segment &sg, int q, double dt,
-                                                 double2 *wave)
-{
-    constexpr int F16_TILE = f16_tile_samples(WAVES);
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wv = tid >> 6;
-    const int li = lane & 15;
-    const int g = lane >> 4;
-    const long plane = (long)KB * RB * F16_BLK;
-    const _Float16 *sHi = reinterpret_cast<const _Float16 *>(sTile);
-    const _Float16 *sLo = sHi + plane;
-    const float *sInv = reinterpret_cast<const float *>(sLo + plane);
-    const double inv_pw = 1.0 / sg.pw;
-    const double inv_hw = 1.0 / sg.hw;
+// coefficients sit in LDS) of the FP16 split-operand factorised sum runs in
+// three phases: setup (FP64 phases, v_n recurrence state, prefetch of the
+// output samples), contraction (MFMA), epilogue (row/power sum in FP64 and
+// the output update).
 
-    // ---- per-sample setup: Phi_r (FP64) and the v_n recurrence state
-    const int base_i = sg.start + q * F16_TILE + wv * (16 * F16_NB) + li;
+// v_n recurrence state of one wave between setup and contraction
+struct F16V {
     double vr[F16_NB][4], vi[F16_NB][4], zr16[F16_NB], zi16[F16_NB];
+};
+
+// First output sample of a lane in tile q of an interval.
+template <int WAVES>
+__device__ __forceinline__ int f16_base(const Segment &sg, int q)
+{
+    const int tid = threadIdx.x;
+    return sg.start + q * f16_tile_samples(WAVES) + (tid >> 6) * (16 * F16_NB) + (tid & 15);
+}
+
+// The output samples a tile adds into, fetched at setup so the HBM latency
+// hides behind the contraction (lane group 0 owns the update).
+__device__ __forceinline__ void f16_prefetch(const Segment &sg, int base_i, const double2 *wave,
+                                             double2 (&wold)[F16_NB])
+{
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int c = 0; c < F16_NB; c++) {
+        const int idx = base_i + c * 16;
+        wold[c] = (g == 0 && idx < sg.end) ? wave[idx] : make_double2(0.0, 0.0);
+    }
+}
+
+// Phi_r (FP64) and the v_n recurrence state: lane group g needs v_n for
+// n = n_lo + 16 kb + 4 g + t, t = 0..3, scaled by 256.
+__device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sPh, const Segment &sg,
+                                          int base_i, double dt, F16V &V)
+{
+    const int g = (threadIdx.x & 63) >> 4;
+    const double inv_pw = 1.0 / sg.pw;
 #pragma unroll
     for (int c = 0; c < F16_NB; c++) {
         const double t = dt * (double)(base_i + c * 16);
         const double s = (t - sg.pt) * inv_pw;
         const double ur = dense_phase(sPh + 16, s, 1.0 - s) * INV_2PI16;
-        // lane g needs v_n for n = n_lo + 16 kb + 4 g + t, t = 0..3 (scaled by 256)
         double sn, cs, zsn, zcs;
-        sincos_turns(reduce_turns((double)(P.n_lo + 4 * g) * ur), sn, cs);
-        sincos_turns(reduce_turns(ur), zsn, zcs);
+        sincos_turns_short(reduce_turns((double)(P.n_lo + 4 * g) * ur), sn, cs);
+        sincos_turns_short(reduce_turns(ur), zsn, zcs);
         const double zr = zcs, zi = -zsn;  // z = exp(-i Phi_r)
-        vr[c][0] = 256.0 * cs;
-        vi[c][0] = -256.0 * sn;
+        V.vr[c][0] = 256.0 * cs;
+        V.vi[c][0] = -256.0 * sn;
 #pragma unroll
-        for (int t2 = 1; t2 < 4; t2++) cmul(vr[c][t2 - 1], vi[c][t2 - 1], zr, zi, vr[c][t2], vi[c][t2]);
+        for (int t2 = 1; t2 < 4; t2++)
+            cmul(V.vr[c][t2 - 1], V.vi[c][t2 - 1], zr, zi, V.vr[c][t2], V.vi[c][t2]);
         double wr = zr, wi = zi, tr, ti;
 #pragma unroll
         for (int k2 = 0; k2 < 4; k2++) {  // z^16 by squaring
@@ -282,19 +295,51 @@ __device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const uns
             wr = tr;
             wi = ti;
         }
-        zr16[c] = wr;
-        zi16[c] = wi;
+        V.zr16[c] = wr;
+        V.zi16[c] = wi;
     }
+}
 
-    v4f acc[RB][F16_NB];
-    const int lo_lane = f16_lane_off(li, g);
+// n-contraction of one tile on MFMA: acc = W[(r, p), samples].
+template <int RB>
+__device__ __forceinline__ void f16_contract(const unsigned char *sTile, int KB, F16V &V,
+                                             v4f (&acc)[RB][F16_NB])
+{
+    const int lane = threadIdx.x & 63;
+    const long plane = (long)KB * RB * F16_BLK;
+    const _Float16 *sHi = reinterpret_cast<const _Float16 *>(sTile);
+    const _Float16 *sLo = sHi + plane;
+    const int lo_lane = f16_lane_off(lane & 15, lane >> 4);
     // kb = 0 starts the accumulators from zero (no separate clearing)
-    f16_kstep<RB, true>(sHi + lo_lane, sLo + lo_lane, KB > 1, vr, vi, zr16, zi16, acc);
-    for (int kb = 1; kb < KB; kb++)
+    f16_kstep<RB, true>(sHi + lo_lane, sLo + lo_lane, KB > 1, V.vr, V.vi, V.zr16, V.zi16, acc);
+    // two steps per trip: the v_n state alternates between two register sets
+    // instead of being copied back at every trip
+    int kb = 1;
+    for (; kb + 1 < KB; kb += 2) {
         f16_kstep<RB, false>(sHi + kb * RB * F16_BLK + lo_lane, sLo + kb * RB * F16_BLK + lo_lane,
-                             kb + 1 < KB, vr, vi, zr16, zi16, acc);
+                             true, V.vr, V.vi, V.zr16, V.zi16, acc);
+        f16_kstep<RB, false>(sHi + (kb + 1) * RB * F16_BLK + lo_lane,
+                             sLo + (kb + 1) * RB * F16_BLK + lo_lane, kb + 2 < KB, V.vr, V.vi,
+                             V.zr16, V.zi16, acc);
+    }
+    if (kb < KB)
+        f16_kstep<RB, false>(sHi + kb * RB * F16_BLK + lo_lane, sLo + kb * RB * F16_BLK + lo_lane,
+                             false, V.vr, V.vi, V.zr16, V.zi16, acc);
+}
 
-    // ---- epilogue: h = sum_{r,p} s^p u_r W / (256 sigma_R)
+// Epilogue: h = sum_{r,p} s^p u_r W / (256 sigma_R), added into the output.
+template <int RB, bool ARITH>
+__device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigned char *sTile,
+                                             int KB, const double *sPh, const Segment &sg,
+                                             int base_i, double dt,
+                                             const v4f (&acc)[RB][F16_NB],
+                                             double2 (&wold)[F16_NB], double2 *wave)
+{
+    const int g = (threadIdx.x & 63) >> 4;
+    const long plane = (long)KB * RB * F16_BLK;
+    const float *sInv = reinterpret_cast<const float *>(sTile + 4 * plane);
+    const double inv_pw = 1.0 / sg.pw;
+    const double inv_hw = 1.0 / sg.hw;
     const int f = g & 1;
     const int p0 = 2 * f;
 #pragma unroll
@@ -315,7 +360,7 @@ __device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const uns
             // sum_rb u_r z_rb = u_first * sum_rb e^rb z_rb (e = exp(-2 i Phi_phi)),
             // summed by Horner's rule in FP64.
             double sn, cs;
-            sincos_turns(reduce_turns(2.0 * uph), sn, cs);
+            sincos_turns_short(reduce_turns(2.0 * uph), sn, cs);
             const double e_re = cs, e_im = -sn;
 #pragma unroll
             for (int rb = RB - 1; rb >= 0; rb--) {
@@ -332,7 +377,7 @@ __device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const uns
                 him = ni;
             }
             const double m_first = (double)(P.m0 + (g >> 1));
-            sincos_turns(reduce_turns(m_first * uph + (double)P.k0 * uth), sn, cs);
+            sincos_turns_short(reduce_turns(m_first * uph + (double)P.k0 * uth), sn, cs);
             double nr, ni;
             cmul(cs, -sn, hre, him, nr, ni);
             hre = nr;
@@ -343,7 +388,7 @@ __device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const uns
                 const int r = 2 * rb + (g >> 1);
                 const int2 mk = r < P.Nr ? P.row_mk[r] : make_int2(0, 0);
                 double sn, cs;
-                sincos_turns(reduce_turns((double)mk.x * uph + (double)mk.y * uth), sn, cs);
+                sincos_turns_short(reduce_turns((double)mk.x * uph + (double)mk.y * uth), sn, cs);
                 const float ur = (float)cs, ui = (float)(-sn);
                 // rows r >= Nr hold zeros (W = 0, scale 1)
                 const int R0 = 8 * rb + 2 * g;
@@ -360,13 +405,28 @@ __device__ __forceinline__ void fact16_tile_body(const FactPlanDev &P, const uns
         him += __shfl_xor(him, 16);
         hre += __shfl_xor(hre, 32);
         him += __shfl_xor(him, 32);
-        if (g == 0 && idx < sg.end) {
-            double2 w = wave[idx];
-            w.x += hre;
-            w.y += him;
-            wave[idx] = w;
-        }
+        wold[c].x += hre;
+        wold[c].y += him;
     }
+    // stores after both columns: a store between the two would make the
+    // second column's wait on the prefetched samples also wait for it
+#pragma unroll
+    for (int c = 0; c < F16_NB; c++) {
+        const int idx = base_i + c * 16;
+        if (g == 0 && idx < sg.end) wave[idx] = wold[c];
+    }
+}
+
+// The contraction runs at raised wave priority: of the two waves sharing a
+// SIMD, the one contracting then issues its MFMAs first and the other one's
+// VALU work (setup, epilogue) fills the cycles between them.
+template <int RB>
+__device__ __forceinline__ void f16_contract_prio(const unsigned char *sTile, int KB, F16V &V,
+                                                  v4f (&acc)[RB][F16_NB], int prio)
+{
+    if (prio) __builtin_amdgcn_s_setprio(1);
+    f16_contract<RB>(sTile, KB, V, acc);
+    if (prio) __builtin_amdgcn_s_setprio(0);
 }
 
 // Single waveform, persistent: block b owns a contiguous range of the
@@ -388,6 +448,7 @@ void fact16_kernel(Fact16Args a)
     int jnext_start = a.prefix[j + 1];
     const long np = (long)a.nseg * 3;
     bool stale = true;
+    Segment sgj;
     for (int tile = t0; tile < t1; tile++) {
         while (tile >= jnext_start) {
             j++;
@@ -395,6 +456,7 @@ void fact16_kernel(Fact16Args a)
             stale = true;
         }
         if (stale) {
+            sgj = a.segs[j];
             __syncthreads();
             const uint4 *src = reinterpret_cast<const uint4 *>(a.tiles + (long)j * a.tile_bytes);
             uint4 *dst = reinterpret_cast<uint4 *>(sTile);
@@ -404,8 +466,14 @@ void fact16_kernel(Fact16Args a)
             __syncthreads();
             stale = false;
         }
-        fact16_tile_body<RB, WAVES, ARITH>(a.plan, sTile, sPh, a.KB, a.segs[j],
-                                           tile - a.prefix[j], a.dt, a.wave);
+        const int base_i = f16_base<WAVES>(sgj, tile - a.prefix[j]);
+        v4f acc[RB][F16_NB];
+        double2 wold[F16_NB];
+        F16V V;
+        f16_prefetch(sgj, base_i, a.wave, wold);
+        f16_setup(a.plan, sPh, sgj, base_i, a.dt, V);
+        f16_contract_prio<RB>(sTile, a.KB, V, acc, a.sched.prio);
+        f16_epilogue<RB, ARITH>(a.plan, sTile, a.KB, sPh, sgj, base_i, a.dt, acc, wold, a.wave);
     }
 }
 
@@ -413,7 +481,7 @@ void fact16_kernel(Fact16Args a)
 template <int RB, int WAVES, bool ARITH>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
 void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned char *tiles,
-                         long tile_bytes, int KB)
+                         long tile_bytes, int KB, F16Sched sched)
 {
     constexpr int NT = 64 * WAVES;
     extern __shared__ __attribute__((aligned(16))) unsigned char sTile[];
@@ -430,6 +498,7 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
     int j = upper_index(pref, d.L - 1, t0 - r.wt_prefix[b]);
     FactPlanDev P = batch_wave_plan(B, d);
     bool stale = true;
+    Segment sgj;
     for (int tile = t0; tile < t1; tile++) {
         while (tile >= r.wt_prefix[b + 1]) {  // next waveform with tiles
             b++;
@@ -445,6 +514,7 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
             stale = true;
         }
         if (stale) {
+            sgj = r.segs[d.seg_off + j];
             __syncthreads();
             const uint4 *src = reinterpret_cast<const uint4 *>(tiles + (long)(d.seg_off + j) * tile_bytes);
             uint4 *dst = reinterpret_cast<uint4 *>(sTile);
@@ -455,8 +525,15 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
             __syncthreads();
             stale = false;
         }
-        fact16_tile_body<RB, WAVES, ARITH>(P, sTile, sPh, KB, r.segs[d.seg_off + j],
-                                           lt - pref[j], d.dt, p.wave + d.wave_off / 2);
+        double2 *wave = p.wave + d.wave_off / 2;
+        const int base_i = f16_base<WAVES>(sgj, lt - pref[j]);
+        v4f acc[RB][F16_NB];
+        double2 wold[F16_NB];
+        F16V V;
+        f16_prefetch(sgj, base_i, wave, wold);
+        f16_setup(P, sPh, sgj, base_i, d.dt, V);
+        f16_contract_prio<RB>(sTile, KB, V, acc, sched.prio);
+        f16_epilogue<RB, ARITH>(P, sTile, KB, sPh, sgj, base_i, d.dt, acc, wold, wave);
     }
 }
 
@@ -512,9 +589,11 @@ static int launch_fact16_w(const Fact16Args &a, int RB, int blocks, size_t lds, 
     return FEW_OK;
 }
 
-int launch_fact16_sum(const Fact16Args &a, int RB, int max_blocks, int variant, hipStream_t stream)
+int launch_fact16_sum(const Fact16Args &a_in, int RB, int max_blocks, int variant, hipStream_t stream)
 {
     if (max_blocks <= 0) return FEW_OK;
+    Fact16Args a = a_in;
+    a.sched = fact16_sched(variant);
     const size_t lds = (size_t)a.tile_bytes + 24 * sizeof(double);
     if (lds > 160 * 1024) {
         set_error("mixed16 sum: coefficient tile of %zu bytes exceeds LDS", lds);
@@ -534,7 +613,7 @@ int launch_fact16_sum(const Fact16Args &a, int RB, int max_blocks, int variant, 
 template <int WAVES, bool ARITH>
 static int launch_batch_fact16_w(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
                                  const unsigned char *tiles, long tile_bytes, int KB,
-                                 int blocks, size_t lds, hipStream_t stream)
+                                 int blocks, size_t lds, F16Sched sched, hipStream_t stream)
 {
     switch (B.uni.RB) {
 #define FEW_CASEB(X)                                                                             \
@@ -542,7 +621,7 @@ static int launch_batch_fact16_w(const BatchDev &B, const BatchRunDev &r, const 
         FEW_HIP(hipFuncSetAttribute((const void *)batch_fact16_kernel<X, WAVES, ARITH>,          \
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
         hipLaunchKernelGGL((batch_fact16_kernel<X, WAVES, ARITH>), dim3(blocks),                \
-                           dim3(64 * WAVES), lds, stream, B, r, p, tiles, tile_bytes, KB);        \
+                           dim3(64 * WAVES), lds, stream, B, r, p, tiles, tile_bytes, KB, sched); \
         break;
         FEW_CASEB(1) FEW_CASEB(2) FEW_CASEB(3) FEW_CASEB(4) FEW_CASEB(5) FEW_CASEB(6)
         FEW_CASEB(7) FEW_CASEB(8) FEW_CASEB(9) FEW_CASEB(10) FEW_CASEB(11) FEW_CASEB(12)
@@ -576,11 +655,12 @@ int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs
     }
     const int blocks = device_cus16() * (lds <= 80 * 1024 ? 2 : 1);
     const bool arith = B.uni.rows_arith != 0;
+    const F16Sched sched = fact16_sched(variant);
     if (variant == 4)
-        return arith ? launch_batch_fact16_w<4, true>(B, r, p, tiles, tile_bytes, KB, blocks, lds, stream)
-                     : launch_batch_fact16_w<4, false>(B, r, p, tiles, tile_bytes, KB, blocks, lds, stream);
-    return arith ? launch_batch_fact16_w<8, true>(B, r, p, tiles, tile_bytes, KB, blocks, lds, stream)
-                 : launch_batch_fact16_w<8, false>(B, r, p, tiles, tile_bytes, KB, blocks, lds, stream);
+        return arith ? launch_batch_fact16_w<4, true>(B, r, p, tiles, tile_bytes, KB, blocks, lds, sched, stream)
+                     : launch_batch_fact16_w<4, false>(B, r, p, tiles, tile_bytes, KB, blocks, lds, sched, stream);
+    return arith ? launch_batch_fact16_w<8, true>(B, r, p, tiles, tile_bytes, KB, blocks, lds, sched, stream)
+                 : launch_batch_fact16_w<8, false>(B, r, p, tiles, tile_bytes, KB, blocks, lds, sched, stream);
 }
 
 }  // namespace few

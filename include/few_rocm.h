@@ -137,7 +137,9 @@ size_t few_plan_workspace_bytes(const void *plan, int init_len, int out_len, int
  * FEW_PRECISION_FP64 runs the reference arithmetic; FEW_PRECISION_MIXED the
  * factorised MFMA sum (rel-L2 ~1e-7 against the FP64 result).  `variant`
  * selects the tile shape (0 = default: 32 samples per wave, 2 waves per
- * SIMD; 4 = 64 samples per wave, 1 wave per SIMD).  `stage` is a bitmask:
+ * SIMD; 4 = 64 samples per wave, 1 wave per SIMD; for FEW_PRECISION_MIXED16
+ * 8 = the default shape without the raised wave priority over the MFMA
+ * phase).  `stage` is a bitmask:
  * 1 = prepare (interval planning, coefficient tiles), 2 = sum (the output
  * kernel), 3 = both; stage 2 reuses what the last stage-1 call on the same
  * workspace prepared.  Workspace:
