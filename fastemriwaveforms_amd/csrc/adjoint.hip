@@ -479,8 +479,7 @@ int launch_spline_adjoint(const SplineAdjArgs &a, hipStream_t stream)
         for (int g = 0; g < 3; g++)
             b.blk_prefix[g + 1] = b.blk_prefix[g] + (g < a.ngroups ? (a.grp[g].nsys + 63) / 64 : 0);
         const size_t lds = sizeof(double) * (size_t)a.L * (8 + 128);
-        FEW_HIP(hipFuncSetAttribute((const void *)spline_adjoint_lds_kernel,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        FEW_LDS(spline_adjoint_lds_kernel, (int)lds);
         hipLaunchKernelGGL(spline_adjoint_lds_kernel, dim3((unsigned)b.blk_prefix[a.ngroups]),
                            dim3(128), lds, stream, b);
         FEW_HIP(hipGetLastError());

@@ -426,12 +426,10 @@ int launch_adjoint16(const Adj16Args &a, int max_chunks, hipStream_t stream)
 #define A16_CASE(X)                                                                             \
     case X:                                                                                     \
         if (nh == 2) {                                                                          \
-            FEW_HIP(hipFuncSetAttribute((const void *)adjoint16_kernel<X, 2>,                   \
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+            FEW_LDS((adjoint16_kernel<X, 2>), lds);                                                \
             hipLaunchKernelGGL((adjoint16_kernel<X, 2>), dim3(max_chunks), block, lds, stream, a); \
         } else {                                                                                \
-            FEW_HIP(hipFuncSetAttribute((const void *)adjoint16_kernel<X, 1>,                   \
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+            FEW_LDS((adjoint16_kernel<X, 1>), lds);                                                \
             hipLaunchKernelGGL((adjoint16_kernel<X, 1>), dim3(max_chunks), block, lds, stream, a); \
         }                                                                                       \
         break;

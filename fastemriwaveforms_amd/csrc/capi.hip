@@ -44,6 +44,31 @@ int check_hip(hipError_t e, const char *what)
     return FEW_ERR_HIP;
 }
 
+int ensure_dynamic_lds(const void *kernel, int bytes)
+{
+    struct Entry {
+        const void *k;
+        int dev, bytes;
+    };
+    static std::mutex mu;
+    static Entry table[1024];
+    static int n = 0;
+    int dev = 0;
+    FEW_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    for (int i = 0; i < n; i++) {
+        if (table[i].k == kernel && table[i].dev == dev) {
+            if (table[i].bytes >= bytes) return FEW_OK;
+            FEW_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+            table[i].bytes = bytes;
+            return FEW_OK;
+        }
+    }
+    FEW_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    if (n < 1024) table[n++] = Entry{kernel, dev, bytes};
+    return FEW_OK;
+}
+
 // Library-owned device workspace for the synchronous drop-in entry points
 // (one per device, grown on demand, kept until few_release_workspace()).
 struct DeviceWorkspace {

@@ -25,6 +25,17 @@ int check_hip(hipError_t e, const char *what);
         if (e_ != hipSuccess) return ::few::check_hip(e_, #call); \
     } while (0)
 
+// hipFuncSetAttribute(kernel, MaxDynamicSharedMemorySize, bytes) on the
+// current device, skipped when that kernel already allows as many bytes
+// there (the attribute call costs microseconds of host time per launch).
+int ensure_dynamic_lds(const void *kernel, int bytes);
+
+#define FEW_LDS(kernel, bytes)                                                  \
+    do {                                                                        \
+        int rc_ = ::few::ensure_dynamic_lds((const void *)(kernel), (int)(bytes)); \
+        if (rc_) return rc_;                                                    \
+    } while (0)
+
 // One group of cubic-spline systems sharing a length; see spline.hip.
 struct SplineGroup {
     const double *y;     // knot values: y[s*y_sys + i*y_knot]

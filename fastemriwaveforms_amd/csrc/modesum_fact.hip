@@ -369,8 +369,7 @@ static int launch_fact_nb(const FactArgs &a, int RB, int blocks, size_t lds, hip
 #define FEW_CASE(X)                                                                            \
     case X:                                                                                    \
         if (lds > 65536)                                                                       \
-            FEW_HIP(hipFuncSetAttribute((const void *)fact_f32_kernel<X, NB>,                  \
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)); \
+            FEW_LDS((fact_f32_kernel<X, NB>), lds);                                            \
         hipLaunchKernelGGL((fact_f32_kernel<X, NB>), dim3(blocks), dim3(256), lds, stream, a); \
         break;
         FEW_CASE(1) FEW_CASE(2) FEW_CASE(3) FEW_CASE(4) FEW_CASE(5) FEW_CASE(6) FEW_CASE(7)

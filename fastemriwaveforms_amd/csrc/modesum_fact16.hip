@@ -572,8 +572,7 @@ static int launch_fact16_w(const Fact16Args &a, int RB, int blocks, size_t lds, 
     switch (RB) {
 #define FEW_CASE16(X)                                                                            \
     case X:                                                                                      \
-        FEW_HIP(hipFuncSetAttribute((const void *)fact16_kernel<X, WAVES, ARITH>,                \
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
+        FEW_LDS((fact16_kernel<X, WAVES, ARITH>), lds);                                         \
         hipLaunchKernelGGL((fact16_kernel<X, WAVES, ARITH>), dim3(blocks), dim3(64 * WAVES),    \
                            lds, stream, a);                                                      \
         break;
@@ -618,8 +617,7 @@ static int launch_batch_fact16_w(const BatchDev &B, const BatchRunDev &r, const 
     switch (B.uni.RB) {
 #define FEW_CASEB(X)                                                                             \
     case X:                                                                                      \
-        FEW_HIP(hipFuncSetAttribute((const void *)batch_fact16_kernel<X, WAVES, ARITH>,          \
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));      \
+        FEW_LDS((batch_fact16_kernel<X, WAVES, ARITH>), lds);                                   \
         hipLaunchKernelGGL((batch_fact16_kernel<X, WAVES, ARITH>), dim3(blocks),                \
                            dim3(64 * WAVES), lds, stream, B, r, p, tiles, tile_bytes, KB, sched); \
         break;

@@ -612,8 +612,7 @@ int launch_spline_build(const SplineGroup *groups, int ngroups, int length, doub
             if (groups[g].t_sys == 0) a.shared_lds = 1;
         if (a.shared_lds) {
             lds = sizeof(double) * (size_t)length * (64 * 2 + 6);
-            FEW_HIP(hipFuncSetAttribute((const void *)spline_build_kernel,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            FEW_LDS(spline_build_kernel, (int)lds);
         }
     }
     hipLaunchKernelGGL(spline_build_kernel, dim3(nblk), dim3(128), lds, stream, a);
