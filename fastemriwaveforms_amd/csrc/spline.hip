@@ -476,50 +476,170 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
     spline_system_core(G, s, a.length, a.col_prefix[gi] + s, a.nsys_total, sc);
 }
 
-// A 64-system block of a group on shared knots, two waves: wave 1 (one
-// lane) eliminates the shared matrix while wave 0 stages its 64 systems'
-// knot values in LDS and computes their right-hand sides; then wave 0 runs
-// the sweeps out of LDS (no global-memory latency in the recurrences).
+#define SPLINE_SWEEP_PF 8  // LDS values fetched ahead of the sweep recurrences
+
+// dgtsv's forward sweep of one right-hand side b with the recorded
+// elimination F, then the back substitution: slopes into b.  The same
+// operations as spline_solve_emit's first half; the operands of the next
+// SPLINE_SWEEP_PF steps are read from LDS before the dependent arithmetic.
+__device__ inline void spline_sweeps(int n, const double *F, double *bv, long bs)
+{
+#pragma clang fp contract(off)
+#define SB(i) bv[(long)(i) * bs]
+    const double *piv = F, *fact = F + n, *d = F + 2 * n, *du = F + 3 * n, *dl = F + 4 * n;
+    double cur = SB(0);
+    for (int i0 = 0; i0 < n - 1; i0 += SPLINE_SWEEP_PF) {
+        double nbv[SPLINE_SWEEP_PF], fv[SPLINE_SWEEP_PF], pv[SPLINE_SWEEP_PF];
+#pragma unroll
+        for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
+            const int i = min(i0 + u, n - 2);
+            nbv[u] = SB(i + 1);
+            fv[u] = fact[i];
+            pv[u] = piv[i];
+        }
+#pragma unroll
+        for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
+            const int i = i0 + u;
+            if (i < n - 1) {
+                const double nb = nbv[u];
+                if (pv[u] == 0.0) {
+                    SB(i) = cur;
+                    cur = nb - fv[u] * cur;
+                } else {
+                    SB(i) = nb;
+                    cur = cur - fv[u] * nb;
+                }
+            }
+        }
+    }
+    double x_next = cur / d[n - 1];
+    SB(n - 1) = x_next;
+    double x_next2 = 0.0;
+    for (int i0 = n - 2; i0 >= 0; i0 -= SPLINE_SWEEP_PF) {
+        double bvv[SPLINE_SWEEP_PF], duv[SPLINE_SWEEP_PF], dlv[SPLINE_SWEEP_PF], dv[SPLINE_SWEEP_PF];
+#pragma unroll
+        for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
+            const int ii = max(i0 - u, 0);
+            bvv[u] = SB(ii);
+            duv[u] = du[ii];
+            dlv[u] = dl[ii];
+            dv[u] = d[ii];
+        }
+#pragma unroll
+        for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
+            const int ii = i0 - u;
+            if (ii >= 0) {
+                double xi;
+                if (ii == n - 2)
+                    xi = (bvv[u] - duv[u] * x_next) / dv[u];
+                else
+                    xi = (bvv[u] - duv[u] * x_next - dlv[u] * x_next2) / dv[u];
+                SB(ii) = xi;
+                x_next2 = x_next;
+                x_next = xi;
+            }
+        }
+    }
+#undef SB
+}
+
+#define SPLINE_BLOCK 256  // threads of the shared-knot block (4 waves)
+
+// A 64-system block of a group on shared knots, four waves.  Work that is
+// parallel over (system, knot) is spread over the waves; only what is
+// sequential along the knots runs on a single wave:
+//   wave 3:     the shared matrix (prep_row's rows, one per lane), then
+//               dgtsv's elimination chain on lane 0;
+//   waves 0-2:  meanwhile each stages the knot values of a third of the rows
+//               (plus the neighbouring knots those rows read) and forms
+//               their right-hand sides, reading only what it staged itself;
+//   wave 0:     every system's sweeps (one lane per system) out of LDS;
+//   waves 0-3:  the coefficients of interval i on wave i % 4, and the stores.
 __device__ void spline_block_shared(const SplineArgs &a, int gi, int blk, double *lds)
 {
     const SplineGroup &G = a.g[gi];
     const int n = a.length;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     double *sx = lds, *F = sx + n, *sy = F + 5 * n, *sb = sy + (long)n * 64;
     __shared__ int s_info;
     const int s0 = blk * 64;
     const int nloc = min(64, G.nsys - s0);
-    for (int i = tid; i < n; i += 128) sx[i] = G.t[(long)i * G.t_knot];
+    for (int i = tid; i < n; i += SPLINE_BLOCK) sx[i] = G.t[(long)i * G.t_knot];
     __syncthreads();
-    if (tid >= 64) {
-        if (tid == 64) s_info = spline_factor(sx, n, F);
-    } else {
-        // lane tid stages the knot values of system s0 + tid, SPLINE_TILE_BATCH
-        // loads in flight at a time (clamped addresses, no branches); a lane
-        // reads back only its own column, so no barrier is needed before the
-        // right-hand-side pass
-        const double *ys = G.y + (long)(s0 + min(tid, nloc - 1)) * G.y_sys;
-        for (int i0 = 0; i0 < n; i0 += SPLINE_TILE_BATCH) {
-            double v[SPLINE_TILE_BATCH];
-#pragma unroll
-            for (int u = 0; u < SPLINE_TILE_BATCH; u++)
-                v[u] = ys[(long)min(i0 + u, n - 1) * G.y_knot];
-#pragma unroll
-            for (int u = 0; u < SPLINE_TILE_BATCH; u++)
-                if (i0 + u < n) sy[(long)(i0 + u) * 64 + tid] = tid < nloc ? v[u] : 0.0;
+    if (wv == 3) {
+        double *piv = F, *fact = F + n, *d = F + 2 * n, *du = F + 3 * n, *dl = F + 4 * n;
+        for (int r = lane; r < n; r += 64) {
+            const int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
+            double b_, u_, g_, l_;
+            prep_row(r, n, sx[i0], sx[i0 + 1], sx[i0 + 2], 0.0, 0.0, 0.0, b_, u_, g_, l_);
+            d[r] = g_;
+            if (r < n - 1) du[r] = u_;
+            if (r > 0) dl[r - 1] = l_;
         }
-        spline_rhs_pass(n, sx, sy + tid, 64, sb + tid, 64);
+        // the rows written by the other lanes of this wave, then the chain
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) s_info = dgtsv_factor(n, d, du, dl, piv, fact);
+    } else {
+        const int c = (n + 2) / 3;
+        const int r0 = wv * c, r1 = min(n, r0 + c);
+        if (r0 < r1) {
+            // knots [k0, k1] hold every value rows r0..r1-1 read
+            const int k0 = max(0, min(r0 - 1, n - 3)), k1 = min(n - 1, max(r1, 2));
+            const double *ys = G.y + (long)(s0 + min(lane, nloc - 1)) * G.y_sys;
+            for (int kb = k0; kb <= k1; kb += SPLINE_TILE_BATCH) {
+                double v[SPLINE_TILE_BATCH];
+#pragma unroll
+                for (int u = 0; u < SPLINE_TILE_BATCH; u++)
+                    v[u] = ys[(long)min(kb + u, k1) * G.y_knot];
+#pragma unroll
+                for (int u = 0; u < SPLINE_TILE_BATCH; u++)
+                    if (kb + u <= k1) sy[(long)(kb + u) * 64 + lane] = lane < nloc ? v[u] : 0.0;
+            }
+            // right-hand sides (prep_row's b); a lane reads back only its own column
+            for (int r = r0; r < r1; r++) {
+                const int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
+                double b_, u_, g_, l_;
+                prep_row(r, n, sx[i0], sx[i0 + 1], sx[i0 + 2], sy[(long)i0 * 64 + lane],
+                         sy[(long)(i0 + 1) * 64 + lane], sy[(long)(i0 + 2) * 64 + lane], b_, u_,
+                         g_, l_);
+                sb[(long)r * 64 + lane] = b_;
+            }
+        }
     }
     __syncthreads();
-    if (tid >= 64) return;
     if (s_info != 0) {  // singular: dgtsv's early return, handled system by system
-        spline_one_system(a, gi, s0 + tid);
+        if (wv == 0) spline_one_system(a, gi, s0 + lane);
         return;
     }
-    if (tid < nloc) spline_solve_emit(G, s0 + tid, n, sx, F, sy + tid, 64, sb + tid, 64);
+    if (wv == 0) spline_sweeps(n, F, sb + lane, 64);
+    __syncthreads();
+    if (lane >= nloc) return;
+    // coefficients of interval i on wave i % 4; the wave holding interval
+    // n - 2 also writes the final derivatives (fill_final_derivs,
+    // interpolate.cu:242-280)
+    const int s = s0 + lane;
+    double c1L = 0.0, c2L = 0.0, c3L = 0.0;
+#pragma unroll 2
+    for (int i = wv; i < n - 1; i += 4)
+        spline_emit(G, s, n, i, sb[(long)i * 64 + lane], sb[(long)(i + 1) * 64 + lane], sx[i],
+                    sx[i + 1], sy[(long)i * 64 + lane], sy[(long)(i + 1) * 64 + lane], c1L, c2L,
+                    c3L);
+    if (!G.dense && wv == (n - 2) % 4) {
+#pragma clang fp contract(off)
+        const double xx = sx[n - 1] - sx[n - 2];
+        const double x2 = xx * xx;
+        double *out = G.out;
+        const long o = (long)s * G.o_sys + (long)(n - 1) * G.o_knot;
+        out[o] = sy[(long)(n - 1) * 64 + lane];
+        out[G.o_coef + o] = c1L + 2 * c2L * xx + 3 * c3L * x2;
+        out[2 * G.o_coef + o] = (2. * c2L + 6. * c3L * xx) / 2.;
+        out[3 * G.o_coef + o] = c3L;
+    }
 }
 
-__global__ __launch_bounds__(128) void spline_build_kernel(SplineArgs a)
+__global__ __launch_bounds__(SPLINE_BLOCK) void spline_build_kernel(SplineArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) double spl_lds[];
     int gi = 0;
@@ -615,7 +735,7 @@ int launch_spline_build(const SplineGroup *groups, int ngroups, int length, doub
             FEW_LDS(spline_build_kernel, (int)lds);
         }
     }
-    hipLaunchKernelGGL(spline_build_kernel, dim3(nblk), dim3(128), lds, stream, a);
+    hipLaunchKernelGGL(spline_build_kernel, dim3(nblk), dim3(SPLINE_BLOCK), lds, stream, a);
     FEW_HIP(hipGetLastError());
     return FEW_OK;
 }
