@@ -28,7 +28,7 @@ namespace few {
 
 #define SPLINE_MAX_GROUPS 4
 #define SPLINE_PF 8  // prefetch distance (steps) of the sequential sweeps
-#define SPLINE_SHARED_MAX_L 150  // LDS path: (128 + 6) * L doubles per 64-system block
+#define SPLINE_SHARED_MAX_L 150  // LDS path: (2 * 32 + 6) * L doubles per 32-system block
 #define SPLINE_TILE_BATCH 24     // knot-value loads in flight per lane while staging a tile
 
 struct SplineArgs {
@@ -482,7 +482,7 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
 // elimination F, then the back substitution: slopes into b.  The same
 // operations as spline_solve_emit's first half; the operands of the next
 // SPLINE_SWEEP_PF steps are read from LDS before the dependent arithmetic.
-__device__ inline void spline_sweeps(int n, const double *F, double *bv, long bs)
+__host__ __device__ inline void spline_sweeps(int n, const double *F, double *bv, long bs)
 {
 #pragma clang fp contract(off)
 #define SB(i) bv[(long)(i) * bs]
@@ -492,7 +492,7 @@ __device__ inline void spline_sweeps(int n, const double *F, double *bv, long bs
         double nbv[SPLINE_SWEEP_PF], fv[SPLINE_SWEEP_PF], pv[SPLINE_SWEEP_PF];
 #pragma unroll
         for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
-            const int i = min(i0 + u, n - 2);
+            const int i = i0 + u < n - 2 ? i0 + u : n - 2;
             nbv[u] = SB(i + 1);
             fv[u] = fact[i];
             pv[u] = piv[i];
@@ -501,25 +501,26 @@ __device__ inline void spline_sweeps(int n, const double *F, double *bv, long bs
         for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
             const int i = i0 + u;
             if (i < n - 1) {
+                // no interchange: b[i] stays, b[i+1] -= f b[i];
+                // interchange:    (b[i], b[i+1]) = (b[i+1], b[i] - f b[i+1])
                 const double nb = nbv[u];
-                if (pv[u] == 0.0) {
-                    SB(i) = cur;
-                    cur = nb - fv[u] * cur;
-                } else {
-                    SB(i) = nb;
-                    cur = cur - fv[u] * nb;
-                }
+                const bool keep = pv[u] == 0.0;
+                SB(i) = keep ? cur : nb;
+                cur = keep ? nb - fv[u] * cur : cur - fv[u] * nb;
             }
         }
     }
     double x_next = cur / d[n - 1];
     SB(n - 1) = x_next;
-    double x_next2 = 0.0;
-    for (int i0 = n - 2; i0 >= 0; i0 -= SPLINE_SWEEP_PF) {
+    // row n - 2 has no second super-diagonal term
+    double x_next2 = x_next;
+    x_next = (SB(n - 2) - du[n - 2] * x_next) / d[n - 2];
+    SB(n - 2) = x_next;
+    for (int i0 = n - 3; i0 >= 0; i0 -= SPLINE_SWEEP_PF) {
         double bvv[SPLINE_SWEEP_PF], duv[SPLINE_SWEEP_PF], dlv[SPLINE_SWEEP_PF], dv[SPLINE_SWEEP_PF];
 #pragma unroll
         for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
-            const int ii = max(i0 - u, 0);
+            const int ii = i0 - u > 0 ? i0 - u : 0;
             bvv[u] = SB(ii);
             duv[u] = du[ii];
             dlv[u] = dl[ii];
@@ -529,11 +530,7 @@ __device__ inline void spline_sweeps(int n, const double *F, double *bv, long bs
         for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
             const int ii = i0 - u;
             if (ii >= 0) {
-                double xi;
-                if (ii == n - 2)
-                    xi = (bvv[u] - duv[u] * x_next) / dv[u];
-                else
-                    xi = (bvv[u] - duv[u] * x_next - dlv[u] * x_next2) / dv[u];
+                const double xi = (bvv[u] - duv[u] * x_next - dlv[u] * x_next2) / dv[u];
                 SB(ii) = xi;
                 x_next2 = x_next;
                 x_next = xi;
@@ -544,27 +541,31 @@ __device__ inline void spline_sweeps(int n, const double *F, double *bv, long bs
 }
 
 #define SPLINE_BLOCK 256  // threads of the shared-knot block (4 waves)
+#define SPLINE_SPB 32     // systems per shared-knot block (half a wave)
 
-// A 64-system block of a group on shared knots, four waves.  Work that is
-// parallel over (system, knot) is spread over the waves; only what is
-// sequential along the knots runs on a single wave:
+// A block of SPLINE_SPB systems of a group on shared knots, four waves; the
+// two halves of a wave take alternate knots / rows / intervals of the same
+// systems.  Work that is parallel over (system, knot) is spread over the
+// waves; only what is sequential along the knots runs on a single wave:
 //   wave 3:     the shared matrix (prep_row's rows, one per lane), then
-//               dgtsv's elimination chain on lane 0;
+//               dgtsv's elimination chain (every lane the same chain);
 //   waves 0-2:  meanwhile each stages the knot values of a third of the rows
 //               (plus the neighbouring knots those rows read) and forms
 //               their right-hand sides, reading only what it staged itself;
 //   wave 0:     every system's sweeps (one lane per system) out of LDS;
-//   waves 0-3:  the coefficients of interval i on wave i % 4, and the stores.
+//   waves 0-3:  the coefficients of the intervals and their stores.
 __device__ void spline_block_shared(const SplineArgs &a, int gi, int blk, double *lds)
 {
+    constexpr int S = SPLINE_SPB;
     const SplineGroup &G = a.g[gi];
     const int n = a.length;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6;
-    double *sx = lds, *F = sx + n, *sy = F + 5 * n, *sb = sy + (long)n * 64;
+    const int half = lane / S, ls = lane % S;
+    double *sx = lds, *F = sx + n, *sy = F + 5 * n, *sb = sy + (long)n * S;
     __shared__ int s_info;
-    const int s0 = blk * 64;
-    const int nloc = min(64, G.nsys - s0);
+    const int s0 = blk * S;
+    const int nloc = min(S, G.nsys - s0);
     for (int i = tid; i < n; i += SPLINE_BLOCK) sx[i] = G.t[(long)i * G.t_knot];
     __syncthreads();
     if (wv == 3) {
@@ -578,61 +579,66 @@ __device__ void spline_block_shared(const SplineArgs &a, int gi, int blk, double
             if (r > 0) dl[r - 1] = l_;
         }
         // the rows written by the other lanes of this wave, then the chain
+        // (identical on every lane: identical stores, no divergence)
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        if (lane == 0) s_info = dgtsv_factor(n, d, du, dl, piv, fact);
+        const int info = dgtsv_factor_ahead<8>(n, d, du, dl, piv, fact);
+        if (lane == 0) s_info = info;
     } else {
         const int c = (n + 2) / 3;
         const int r0 = wv * c, r1 = min(n, r0 + c);
         if (r0 < r1) {
-            // knots [k0, k1] hold every value rows r0..r1-1 read
+            // knots [k0, k1] hold every value rows r0..r1-1 read; the two
+            // halves stage alternate knots
             const int k0 = max(0, min(r0 - 1, n - 3)), k1 = min(n - 1, max(r1, 2));
-            const double *ys = G.y + (long)(s0 + min(lane, nloc - 1)) * G.y_sys;
-            for (int kb = k0; kb <= k1; kb += SPLINE_TILE_BATCH) {
+            const double *ys = G.y + (long)(s0 + min(ls, nloc - 1)) * G.y_sys;
+            for (int kb = k0 + half; kb <= k1; kb += 2 * SPLINE_TILE_BATCH) {
                 double v[SPLINE_TILE_BATCH];
 #pragma unroll
                 for (int u = 0; u < SPLINE_TILE_BATCH; u++)
-                    v[u] = ys[(long)min(kb + u, k1) * G.y_knot];
+                    v[u] = ys[(long)min(kb + 2 * u, k1) * G.y_knot];
 #pragma unroll
                 for (int u = 0; u < SPLINE_TILE_BATCH; u++)
-                    if (kb + u <= k1) sy[(long)(kb + u) * 64 + lane] = lane < nloc ? v[u] : 0.0;
+                    if (kb + 2 * u <= k1) sy[(long)(kb + 2 * u) * S + ls] = ls < nloc ? v[u] : 0.0;
             }
-            // right-hand sides (prep_row's b); a lane reads back only its own column
-            for (int r = r0; r < r1; r++) {
+            // the other half's knots, written by this wave's earlier LDS stores
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // right-hand sides (prep_row's b), alternate rows per half
+#pragma unroll 2
+            for (int r = r0 + half; r < r1; r += 2) {
                 const int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
                 double b_, u_, g_, l_;
-                prep_row(r, n, sx[i0], sx[i0 + 1], sx[i0 + 2], sy[(long)i0 * 64 + lane],
-                         sy[(long)(i0 + 1) * 64 + lane], sy[(long)(i0 + 2) * 64 + lane], b_, u_,
-                         g_, l_);
-                sb[(long)r * 64 + lane] = b_;
+                prep_row(r, n, sx[i0], sx[i0 + 1], sx[i0 + 2], sy[(long)i0 * S + ls],
+                         sy[(long)(i0 + 1) * S + ls], sy[(long)(i0 + 2) * S + ls], b_, u_, g_, l_);
+                sb[(long)r * S + ls] = b_;
             }
         }
     }
     __syncthreads();
     if (s_info != 0) {  // singular: dgtsv's early return, handled system by system
-        if (wv == 0) spline_one_system(a, gi, s0 + lane);
+        if (wv == 0 && half == 0) spline_one_system(a, gi, s0 + ls);
         return;
     }
-    if (wv == 0) spline_sweeps(n, F, sb + lane, 64);
+    if (wv == 0 && half == 0) spline_sweeps(n, F, sb + ls, S);
     __syncthreads();
-    if (lane >= nloc) return;
-    // coefficients of interval i on wave i % 4; the wave holding interval
-    // n - 2 also writes the final derivatives (fill_final_derivs,
-    // interpolate.cu:242-280)
-    const int s = s0 + lane;
+    if (ls >= nloc) return;
+    // coefficients of interval i on lane group i % 8 (wave, half); the group
+    // holding interval n - 2 also writes the final derivatives
+    // (fill_final_derivs, interpolate.cu:242-280)
+    const int grp = 2 * wv + half;
+    const int s = s0 + ls;
     double c1L = 0.0, c2L = 0.0, c3L = 0.0;
 #pragma unroll 2
-    for (int i = wv; i < n - 1; i += 4)
-        spline_emit(G, s, n, i, sb[(long)i * 64 + lane], sb[(long)(i + 1) * 64 + lane], sx[i],
-                    sx[i + 1], sy[(long)i * 64 + lane], sy[(long)(i + 1) * 64 + lane], c1L, c2L,
-                    c3L);
-    if (!G.dense && wv == (n - 2) % 4) {
-#pragma clang fp contract(off)
+    for (int i = grp; i < n - 1; i += 8)
+        spline_emit(G, s, n, i, sb[(long)i * S + ls], sb[(long)(i + 1) * S + ls], sx[i], sx[i + 1],
+                    sy[(long)i * S + ls], sy[(long)(i + 1) * S + ls], c1L, c2L, c3L);
+    if (!G.dense && grp == (n - 2) % 8) {
         const double xx = sx[n - 1] - sx[n - 2];
         const double x2 = xx * xx;
         double *out = G.out;
         const long o = (long)s * G.o_sys + (long)(n - 1) * G.o_knot;
-        out[o] = sy[(long)(n - 1) * 64 + lane];
+        out[o] = sy[(long)(n - 1) * S + ls];
         out[G.o_coef + o] = c1L + 2 * c2L * xx + 3 * c3L * x2;
         out[2 * G.o_coef + o] = (2. * c2L + 6. * c3L * xx) / 2.;
         out[3 * G.o_coef + o] = c3L;
@@ -724,17 +730,22 @@ int launch_spline_build(const SplineGroup *groups, int ngroups, int length, doub
     }
     SplineArgs a;
     fill_spline_args(groups, ngroups, length, scratch4, info_out, a);
-    const int nblk = a.blk_prefix[ngroups];
-    if (nblk == 0) return FEW_OK;
+    if (a.blk_prefix[ngroups] == 0) return FEW_OK;
     size_t lds = 0;
     if (length <= SPLINE_SHARED_MAX_L) {
         for (int g = 0; g < ngroups; g++)
             if (groups[g].t_sys == 0) a.shared_lds = 1;
         if (a.shared_lds) {
-            lds = sizeof(double) * (size_t)length * (64 * 2 + 6);
+            lds = sizeof(double) * (size_t)length * (SPLINE_SPB * 2 + 6);
             FEW_LDS(spline_build_kernel, (int)lds);
+            // groups on shared knots take SPLINE_SPB systems per block
+            for (int g = 0; g < ngroups; g++) {
+                const int per = groups[g].t_sys == 0 ? SPLINE_SPB : 64;
+                a.blk_prefix[g + 1] = a.blk_prefix[g] + (groups[g].nsys + per - 1) / per;
+            }
         }
     }
+    const int nblk = a.blk_prefix[ngroups];
     hipLaunchKernelGGL(spline_build_kernel, dim3(nblk), dim3(SPLINE_BLOCK), lds, stream, a);
     FEW_HIP(hipGetLastError());
     return FEW_OK;

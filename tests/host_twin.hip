@@ -320,6 +320,51 @@ void twin_fd_spa(const double *x, long n, double *out)
     }
 }
 
+// The shared-knot block's pieces against the plain routines, for knots x and
+// one array y (n knots): F_plain / F_ahead (5n each: piv | fact | d | du | dl)
+// from dgtsv_factor and dgtsv_factor_ahead on the same matrix, their INFOs;
+// slopes (n) from spline_sweeps on the right-hand side; c1 (n - 1) from
+// spline_system_shared (the slopes it emits).
+void twin_spline_block_pieces(const double *x, const double *y, int n, double *F_plain,
+                              double *F_ahead, int *info, double *slopes, double *c1)
+{
+    info[0] = spline_factor(x, n, F_plain);
+    for (int r = 0; r < n; r++) {
+        const int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
+        double b_, u_, g_, l_;
+        prep_row(r, n, x[i0], x[i0 + 1], x[i0 + 2], 0.0, 0.0, 0.0, b_, u_, g_, l_);
+        F_ahead[2 * n + r] = g_;
+        if (r < n - 1) F_ahead[3 * n + r] = u_;
+        if (r > 0) F_ahead[4 * n + r - 1] = l_;
+    }
+    info[1] = dgtsv_factor_ahead<8>(n, F_ahead + 2 * n, F_ahead + 3 * n, F_ahead + 4 * n, F_ahead,
+                                    F_ahead + n);
+    spline_rhs_pass(n, x, y, 1, slopes, 1);
+    if (info[1] == 0) spline_sweeps(n, F_ahead, slopes, 1);
+    std::vector<double> out(4 * (size_t)n), b(n);
+    SplineGroup G{y, 0, 1, x, 0, 1, out.data(), n, 0, 1, 1, 0};
+    if (info[0] == 0) spline_system_shared(G, 0, n, x, F_plain, y, 1, b.data(), 1);
+    for (int i = 0; i < n - 1; i++) c1[i] = out[n + i];
+}
+
+// dgtsv_factor and dgtsv_factor_ahead<8> on copies of one matrix:
+// out[k] = piv | fact | d | du | dl (5n) of variant k, info[k].
+void twin_dgtsv_pair(int n, const double *d, const double *du, const double *dl, double *out,
+                     int *info)
+{
+    for (int k = 0; k < 2; k++) {
+        double *F = out + 5 * (long)n * k;
+        for (int i = 0; i < 5 * n; i++) F[i] = 0.0;
+        for (int i = 0; i < n; i++) F[2 * n + i] = d[i];
+        for (int i = 0; i < n - 1; i++) {
+            F[3 * n + i] = du[i];
+            F[4 * n + i] = dl[i];
+        }
+        info[k] = k == 0 ? dgtsv_factor(n, F + 2 * n, F + 3 * n, F + 4 * n, F, F + n)
+                         : dgtsv_factor_ahead<8>(n, F + 2 * n, F + 3 * n, F + 4 * n, F, F + n);
+    }
+}
+
 void twin_sincos_turns(const double *f, double *s, double *c, long n)
 {
     for (long i = 0; i < n; i++) sincos_turns(reduce_turns(f[i]), s[i], c[i]);

@@ -223,3 +223,60 @@ def test_twin_sincos_cw(tw):
     s2, c2 = np.zeros_like(big), np.zeros_like(big)
     tw.twin_sincos_cw(_p(big), _p(s2), _p(c2), _p(ok2), len(big))
     assert not ok2.any()
+
+
+@pytest.mark.parametrize("n", [3, 4, 5, 9, 17, 120, 150])
+def test_twin_spline_block_pieces_bitwise(tw, n):
+    """The shared-knot block's read-ahead, branch-free elimination and sweeps
+    (dgtsv_factor_ahead, spline_sweeps) against dgtsv_factor and the plain
+    sweep of spline_system_shared, bit for bit, on knots whose spacings force
+    row interchanges at random rows."""
+    tw.twin_spline_block_pieces.argtypes = [P, P, ctypes.c_int, P, P, P, P, P]
+    rng = np.random.default_rng(n)
+    swaps = 0
+    for trial in range(20):
+        h = rng.uniform(0.05, 1.0, n - 1)
+        # occasional very uneven neighbours make |dl| > |d| (interchanges)
+        h[rng.random(n - 1) < 0.3] *= rng.uniform(20.0, 200.0)
+        x = np.concatenate([[0.0], np.cumsum(h)])
+        y = rng.normal(size=n)
+        Fp = np.zeros(5 * n)
+        Fa = np.zeros(5 * n)
+        info = np.zeros(2, np.int32)
+        slopes = np.zeros(n)
+        c1 = np.zeros(n - 1)
+        tw.twin_spline_block_pieces(_p(x), _p(y), n, _p(Fp), _p(Fa), _p(info), _p(slopes), _p(c1))
+        assert info[0] == info[1]
+        if info[0] != 0:
+            continue
+        assert np.array_equal(Fp[:2 * n - 1], Fa[:2 * n - 1])          # piv, fact
+        assert np.array_equal(Fp[2 * n:3 * n], Fa[2 * n:3 * n])        # d
+        assert np.array_equal(Fp[3 * n:4 * n - 1], Fa[3 * n:4 * n - 1])  # du
+        assert np.array_equal(Fp[4 * n:5 * n - 2], Fa[4 * n:5 * n - 2])  # dl (fill-in)
+        assert np.array_equal(slopes[:n - 1], c1)
+        swaps += int(np.sum(Fa[:n - 1] == 1.0))
+    assert n < 4 or swaps > 0   # the interchange branch was exercised
+
+
+def test_twin_factor_ahead_interchanges_and_singular(tw):
+    """Hand-built matrices: interchanges at every step, and dgtsv's early
+    return on an exactly singular leading block (same INFO, same stored rows)."""
+    tw.twin_dgtsv_pair.argtypes = [ctypes.c_int, P, P, P, P, P]
+    n = 12
+    rng = np.random.default_rng(7)
+    d = rng.uniform(0.01, 0.1, n)
+    du = rng.uniform(0.5, 1.0, n - 1)
+    dl = rng.uniform(5.0, 10.0, n - 1)   # |dl| > |d|: interchange at every step
+    out = np.zeros((2, 5 * n))
+    info = np.zeros(2, np.int32)
+    tw.twin_dgtsv_pair(n, _p(d), _p(du), _p(dl), _p(out), _p(info))
+    assert info[0] == info[1] == 0
+    assert np.all(out[0, :n - 1] == 1.0)
+    assert np.array_equal(out[0], out[1])
+    d2 = d.copy()
+    d2[0] = 0.0
+    dl2 = dl.copy()
+    dl2[0] = 0.0   # no interchange possible and a zero pivot: INFO = 1
+    tw.twin_dgtsv_pair(n, _p(d2), _p(du), _p(dl2), _p(out), _p(info))
+    assert info[0] == info[1] == 1
+    assert np.array_equal(out[0], out[1])
