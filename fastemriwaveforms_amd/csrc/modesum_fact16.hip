@@ -158,7 +158,12 @@ int launch_fact16_prep(const Fact16PrepArgs &a, hipStream_t stream)
 }
 
 // ------------------------------------------------------------- sum kernel
-__device__ __forceinline__ void split_pack(const double (&vr)[4], const double (&vi)[4], v8h &bh,
+// v_n recurrence state type of the K loop: FP32 (the operands are split to
+// FP16 hi/lo pairs, ~2^-22 relative, right after; three FP32 steps by z^16,
+// itself rounded from FP64, add a few 2^-24)
+typedef float f16v_t;
+
+__device__ __forceinline__ void split_pack(const f16v_t (&vr)[4], const f16v_t (&vi)[4], v8h &bh,
                                            v8h &bl)
 {
 #pragma unroll
@@ -173,8 +178,8 @@ __device__ __forceinline__ void split_pack(const double (&vr)[4], const double (
     }
 }
 
-__device__ __forceinline__ void cmul(double ar, double ai, double br, double bi, double &cr,
-                                     double &ci)
+template <typename T>
+__device__ __forceinline__ void cmul(T ar, T ai, T br, T bi, T &cr, T &ci)
 {
     cr = ar * br - ai * bi;
     ci = ar * bi + ai * br;
@@ -191,9 +196,9 @@ constexpr int f16_tile_samples(int waves) { return 16 * F16_NB * waves; }
 // block column at aH / aL, and the three split products per row block.
 template <int RB, bool FIRST>
 __device__ __forceinline__ void f16_kstep(const _Float16 *aH, const _Float16 *aL, bool advance,
-                                          double (&vr)[F16_NB][4], double (&vi)[F16_NB][4],
-                                          const double (&zr16)[F16_NB],
-                                          const double (&zi16)[F16_NB], v4f (&acc)[RB][F16_NB])
+                                          f16v_t (&vr)[F16_NB][4], f16v_t (&vi)[F16_NB][4],
+                                          const f16v_t (&zr16)[F16_NB],
+                                          const f16v_t (&zi16)[F16_NB], v4f (&acc)[RB][F16_NB])
 {
     v8h bh[F16_NB], bl[F16_NB];
 #pragma unroll
@@ -203,7 +208,7 @@ __device__ __forceinline__ void f16_kstep(const _Float16 *aH, const _Float16 *aL
         for (int c = 0; c < F16_NB; c++)
 #pragma unroll
             for (int t2 = 0; t2 < 4; t2++) {
-                double nr, ni;
+                f16v_t nr, ni;
                 cmul(vr[c][t2], vi[c][t2], zr16[c], zi16[c], nr, ni);
                 vr[c][t2] = nr;
                 vi[c][t2] = ni;
@@ -243,7 +248,7 @@ __device__ __forceinline__ void f16_kstep(const _Float16 *aH, const _Float16 *aL
 
 // v_n recurrence state of one wave between setup and contraction
 struct F16V {
-    double vr[F16_NB][4], vi[F16_NB][4], zr16[F16_NB], zi16[F16_NB];
+    f16v_t vr[F16_NB][4], vi[F16_NB][4], zr16[F16_NB], zi16[F16_NB];
 };
 
 // First output sample of a lane in tile q of an interval.
@@ -283,11 +288,18 @@ __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sP
         sincos_turns_short(reduce_turns((double)(P.n_lo + 4 * g) * ur), sn, cs);
         sincos_turns_short(reduce_turns(ur), zsn, zcs);
         const double zr = zcs, zi = -zsn;  // z = exp(-i Phi_r)
-        V.vr[c][0] = 256.0 * cs;
-        V.vi[c][0] = -256.0 * sn;
+        double vr = 256.0 * cs, vi = -256.0 * sn;
+        V.vr[c][0] = (f16v_t)vr;
+        V.vi[c][0] = (f16v_t)vi;
 #pragma unroll
-        for (int t2 = 1; t2 < 4; t2++)
-            cmul(V.vr[c][t2 - 1], V.vi[c][t2 - 1], zr, zi, V.vr[c][t2], V.vi[c][t2]);
+        for (int t2 = 1; t2 < 4; t2++) {
+            double nr, ni;
+            cmul(vr, vi, zr, zi, nr, ni);
+            vr = nr;
+            vi = ni;
+            V.vr[c][t2] = (f16v_t)vr;
+            V.vi[c][t2] = (f16v_t)vi;
+        }
         double wr = zr, wi = zi, tr, ti;
 #pragma unroll
         for (int k2 = 0; k2 < 4; k2++) {  // z^16 by squaring
@@ -295,8 +307,8 @@ __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sP
             wr = tr;
             wi = ti;
         }
-        V.zr16[c] = wr;
-        V.zi16[c] = wi;
+        V.zr16[c] = (f16v_t)wr;
+        V.zi16[c] = (f16v_t)wi;
     }
 }
 
