@@ -249,6 +249,9 @@ __device__ __forceinline__ void f16_kstep(const _Float16 *aH, const _Float16 *aL
 // v_n recurrence state of one wave between setup and contraction
 struct F16V {
     f16v_t vr[F16_NB][4], vi[F16_NB][4], zr16[F16_NB], zi16[F16_NB];
+    // arithmetic rows: the sample phasor this lane group evaluated at setup
+    // (sin, cos), handed to the other lane groups of the sample in the epilogue
+    double ps[F16_NB], pc[F16_NB];
 };
 
 // First output sample of a lane in tile q of an interval.
@@ -274,19 +277,43 @@ __device__ __forceinline__ void f16_prefetch(const Segment &sg, int base_i, cons
 
 // Phi_r (FP64) and the v_n recurrence state: lane group g needs v_n for
 // n = n_lo + 16 kb + 4 g + t, t = 0..3, scaled by 256.
+template <bool ARITH>
 __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sPh, const Segment &sg,
                                           int base_i, double dt, F16V &V)
 {
-    const int g = (threadIdx.x & 63) >> 4;
+    const int lane = threadIdx.x & 63, g = lane >> 4, li = lane & 15;
     const double inv_pw = 1.0 / sg.pw;
 #pragma unroll
     for (int c = 0; c < F16_NB; c++) {
         const double t = dt * (double)(base_i + c * 16);
         const double s = (t - sg.pt) * inv_pw;
-        const double ur = dense_phase(sPh + 16, s, 1.0 - s) * INV_2PI16;
-        double sn, cs, zsn, zcs;
+        double ur, zsn, zcs;
+        if constexpr (ARITH) {
+            // The four lane groups of a sample share its phases and phasors:
+            // group g evaluates one of them and the others read it (ds_bpermute)
+            //   g = 0:     Phi_r             -> z = exp(-i Phi_r)
+            //   g = 1:     Phi_phi           -> e = exp(-2 i Phi_phi)      (epilogue)
+            //   g = 2, 3:  Phi_phi, Phi_theta -> u = exp(-i((m0 + g - 2) Phi_phi
+            //                                               + k0 Phi_theta)) (epilogue)
+            const double p1 = dense_phase(sPh + (g == 0 ? 16 : 0), s, 1.0 - s) * INV_2PI16;
+            const double p2 = P.has_k ? dense_phase(sPh + 8, s, 1.0 - s) * INV_2PI16 : 0.0;
+            const double arg = g == 0 ? p1
+                             : g == 1 ? 2.0 * p1
+                                      : (double)(P.m0 + g - 2) * p1 + (double)P.k0 * p2;
+            double sn, cs;
+            sincos_turns_short(reduce_turns(arg), sn, cs);
+            V.ps[c] = sn;
+            V.pc[c] = cs;
+            ur = __shfl(p1, li);
+            zsn = __shfl(sn, li);
+            zcs = __shfl(cs, li);
+        } else {
+            ur = dense_phase(sPh + 16, s, 1.0 - s) * INV_2PI16;
+            sincos_turns_short(reduce_turns(ur), zsn, zcs);
+        }
+        // lane group g needs v_n for n = n_lo + 16 kb + 4 g + t, t = 0..3 (scaled by 256)
+        double sn, cs;
         sincos_turns_short(reduce_turns((double)(P.n_lo + 4 * g) * ur), sn, cs);
-        sincos_turns_short(reduce_turns(ur), zsn, zcs);
         const double zr = zcs, zi = -zsn;  // z = exp(-i Phi_r)
         double vr = 256.0 * cs, vi = -256.0 * sn;
         V.vr[c][0] = (f16v_t)vr;
@@ -344,10 +371,11 @@ template <int RB, bool ARITH>
 __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigned char *sTile,
                                              int KB, const double *sPh, const Segment &sg,
                                              int base_i, double dt,
-                                             const v4f (&acc)[RB][F16_NB],
+                                             const v4f (&acc)[RB][F16_NB], const F16V &V,
                                              double2 (&wold)[F16_NB], double2 *wave)
 {
     const int g = (threadIdx.x & 63) >> 4;
+    const int li = threadIdx.x & 15;
     const long plane = (long)KB * RB * F16_BLK;
     const float *sInv = reinterpret_cast<const float *>(sTile + 4 * plane);
     const double inv_pw = 1.0 / sg.pw;
@@ -359,9 +387,6 @@ __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigne
         // sample-level quantities recomputed here rather than held across the K loop
         const int idx = base_i + c * 16;
         const double t = dt * (double)idx;
-        const double s = (t - sg.pt) * inv_pw;
-        const double uph = dense_phase(sPh, s, 1.0 - s) * INV_2PI16;
-        const double uth = P.has_k ? dense_phase(sPh + 8, s, 1.0 - s) * INV_2PI16 : 0.0;
         const float s1f = (float)((t - sg.h_t) * inv_hw);
         const float s2f = s1f * s1f;
         const float spa = (p0 == 0 ? 1.f : s2f) * (1.0f / 256.0f);
@@ -371,9 +396,8 @@ __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigne
             // Rows of one lane are r = 2 rb + (g >> 1) with m_r = m_first + 2 rb:
             // sum_rb u_r z_rb = u_first * sum_rb e^rb z_rb (e = exp(-2 i Phi_phi)),
             // summed by Horner's rule in FP64.
-            double sn, cs;
-            sincos_turns_short(reduce_turns(2.0 * uph), sn, cs);
-            const double e_re = cs, e_im = -sn;
+            // e from lane group 1, u_first from group 2 + (g >> 1) (set up at setup)
+            const double e_re = __shfl(V.pc[c], li + 16), e_im = -__shfl(V.ps[c], li + 16);
 #pragma unroll
             for (int rb = RB - 1; rb >= 0; rb--) {
                 // rows r >= Nr hold zeros (W = 0, scale 1)
@@ -388,13 +412,16 @@ __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigne
                 hre = nr;
                 him = ni;
             }
-            const double m_first = (double)(P.m0 + (g >> 1));
-            sincos_turns_short(reduce_turns(m_first * uph + (double)P.k0 * uth), sn, cs);
+            const int ul = li + 32 + 16 * (g >> 1);
+            const double u_re = __shfl(V.pc[c], ul), u_im = -__shfl(V.ps[c], ul);
             double nr, ni;
-            cmul(cs, -sn, hre, him, nr, ni);
+            cmul(u_re, u_im, hre, him, nr, ni);
             hre = nr;
             him = ni;
         } else {
+            const double s = (t - sg.pt) * inv_pw;
+            const double uph = dense_phase(sPh, s, 1.0 - s) * INV_2PI16;
+            const double uth = P.has_k ? dense_phase(sPh + 8, s, 1.0 - s) * INV_2PI16 : 0.0;
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
                 const int r = 2 * rb + (g >> 1);
@@ -483,9 +510,9 @@ void fact16_kernel(Fact16Args a)
         double2 wold[F16_NB];
         F16V V;
         f16_prefetch(sgj, base_i, a.wave, wold);
-        f16_setup(a.plan, sPh, sgj, base_i, a.dt, V);
+        f16_setup<ARITH>(a.plan, sPh, sgj, base_i, a.dt, V);
         f16_contract_prio<RB>(sTile, a.KB, V, acc, a.sched.prio);
-        f16_epilogue<RB, ARITH>(a.plan, sTile, a.KB, sPh, sgj, base_i, a.dt, acc, wold, a.wave);
+        f16_epilogue<RB, ARITH>(a.plan, sTile, a.KB, sPh, sgj, base_i, a.dt, acc, V, wold, a.wave);
     }
 }
 
@@ -543,9 +570,9 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
         double2 wold[F16_NB];
         F16V V;
         f16_prefetch(sgj, base_i, wave, wold);
-        f16_setup(P, sPh, sgj, base_i, d.dt, V);
+        f16_setup<ARITH>(P, sPh, sgj, base_i, d.dt, V);
         f16_contract_prio<RB>(sTile, KB, V, acc, sched.prio);
-        f16_epilogue<RB, ARITH>(P, sTile, KB, sPh, sgj, base_i, d.dt, acc, wold, wave);
+        f16_epilogue<RB, ARITH>(P, sTile, KB, sPh, sgj, base_i, d.dt, acc, V, wold, wave);
     }
 }
 
