@@ -12,6 +12,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -163,8 +164,27 @@ static int check_mode_args(const void *wave, const double *interp, const double 
     return FEW_OK;
 }
 
+// Zero fill of a complex128 range (a kernel rather than hipMemsetAsync, so
+// the call stays capturable in a graph).
+__global__ __launch_bounds__(256) void zero_complex_kernel(double2 *w, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        w[i] = make_double2(0.0, 0.0);
+}
+
+static int launch_zero_complex(double2 *w, long n, hipStream_t stream)
+{
+    if (n <= 0) return FEW_OK;
+    const long blocks = std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(zero_complex_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, w, n);
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
 // Plans (device) and runs one waveform's mode sum on `stream`; h_t and pt
-// are device pointers.
+// are device pointers.  stage: 1 = plan (+ coefficient tiles), 2 = sum,
+// 4 (with 2) = overwrite: the output is written rather than added to, and
+// samples no interval covers are zeroed.
 static int run_waveform(double2 *wave, const double *interp, const double *pt, const double *pcoef,
                         const int *m, const int *k, const int *n, int L, int out_len, int K,
                         const double2 *ylm, double dt, const double *h_t, int precision, char *ws,
@@ -214,6 +234,8 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
         if (!(stage & 2)) return FEW_OK;
         Fact16Args fa{wave, pcoef, segs, prefix, L - 1, dt, tiles, tb, fact16_kb(ph),
                       plan->dev};
+        fa.overwrite = (stage & 4) ? 1 : 0;
+        fa.out_len = out_len;
         return launch_fact16_sum(fa, ph.RB, (int)max_blocks, variant, stream);
     }
     if (precision == FEW_PRECISION_MIXED) {
@@ -248,6 +270,10 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
             if (rc) return rc;
         }
         if (!(stage & 2)) return FEW_OK;
+        if (stage & 4) {
+            int rc = launch_zero_complex(wave, out_len, stream);
+            if (rc) return rc;
+        }
         FactArgs fa{wave, pcoef, segs, prefix, L - 1, dt, C, c_stride, plan->dev};
         return launch_fact_sum(fa, ph, (int)max_blocks, variant, stream);
     }
@@ -262,6 +288,10 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
         if (rc) return rc;
     }
     if (!(stage & 2)) return FEW_OK;
+    if (stage & 4) {
+        int rc = launch_zero_complex(wave, out_len, stream);
+        if (rc) return rc;
+    }
     DirectArgs a;
     a.wave = wave;
     a.interp = interp;
@@ -590,8 +620,9 @@ int few_get_waveform_plan_async(void *d_waveform, const double *interp_array,
         set_error("get_waveform_plan_async: null device time array or workspace");
         return FEW_ERR_ARG;
     }
-    if (stage < 1 || stage > 3) {
-        set_error("get_waveform_plan_async: stage must be 1 (prepare), 2 (sum) or 3 (both)");
+    if (stage < 1 || stage > 7 || (stage == 4 || stage == 5)) {
+        set_error("get_waveform_plan_async: stage must be 1 (prepare), 2 (sum) or 3 (both), "
+                  "plus 4 (overwrite) with the sum");
         return FEW_ERR_ARG;
     }
     return run_waveform(static_cast<double2 *>(d_waveform), interp_array, d_phase_interp_t,

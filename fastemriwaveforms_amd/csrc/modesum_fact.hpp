@@ -141,6 +141,8 @@ struct Fact16Args {
     int KB;
     FactPlanDev plan;
     F16Sched sched;
+    int overwrite = 0;  // write h instead of adding it; zero the samples after the last interval
+    int out_len = 0;
 };
 
 int launch_fact16_prep(const Fact16PrepArgs &a, hipStream_t stream);

@@ -265,13 +265,13 @@ __device__ __forceinline__ int f16_base(const Segment &sg, int q)
 // The output samples a tile adds into, fetched at setup so the HBM latency
 // hides behind the contraction (lane group 0 owns the update).
 __device__ __forceinline__ void f16_prefetch(const Segment &sg, int base_i, const double2 *wave,
-                                             double2 (&wold)[F16_NB])
+                                             double2 (&wold)[F16_NB], bool overwrite = false)
 {
     const int g = (threadIdx.x & 63) >> 4;
 #pragma unroll
     for (int c = 0; c < F16_NB; c++) {
         const int idx = base_i + c * 16;
-        wold[c] = (g == 0 && idx < sg.end) ? wave[idx] : make_double2(0.0, 0.0);
+        wold[c] = (!overwrite && g == 0 && idx < sg.end) ? wave[idx] : make_double2(0.0, 0.0);
     }
 }
 
@@ -482,6 +482,12 @@ void fact16_kernel(Fact16Args a)
     const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
     const int t0 = (int)blockIdx.x * per;
     const int t1 = min(total, t0 + per);
+    if (a.overwrite) {
+        // samples after the last interval: zero (the intervals cover [0, end))
+        const long end = a.nseg > 0 ? a.segs[a.nseg - 1].end : 0;
+        for (long i = end + (long)blockIdx.x * NT + tid; i < a.out_len; i += (long)gridDim.x * NT)
+            a.wave[i] = make_double2(0.0, 0.0);
+    }
     if (t0 >= t1) return;
     int j = upper_index(a.prefix, a.nseg, t0);
     int jnext_start = a.prefix[j + 1];
@@ -509,7 +515,7 @@ void fact16_kernel(Fact16Args a)
         v4f acc[RB][F16_NB];
         double2 wold[F16_NB];
         F16V V;
-        f16_prefetch(sgj, base_i, a.wave, wold);
+        f16_prefetch(sgj, base_i, a.wave, wold, a.overwrite != 0);
         f16_setup<ARITH>(a.plan, sPh, sgj, base_i, a.dt, V);
         f16_contract_prio<RB>(sTile, a.KB, V, acc, a.sched.prio);
         f16_epilogue<RB, ARITH>(a.plan, sTile, a.KB, sPh, sgj, base_i, a.dt, acc, V, wold, a.wave);

@@ -95,8 +95,12 @@ class ModeSumProblem:
             self.scratch.data_ptr(), _stream())
         check(rc, "prepare_modes")
 
-    def mode_sum(self, precision: str = "fp64", stage: int = 3) -> None:
-        """stage: 1 = plan intervals + coefficient tiles, 2 = the summation kernel, 3 = both."""
+    def mode_sum(self, precision: str = "fp64", stage: int = 3, overwrite: bool = False) -> None:
+        """stage: 1 = plan intervals + coefficient tiles, 2 = the summation kernel, 3 = both.
+        overwrite (with the summation): write the waveform instead of adding to it -- every
+        sample is written, 0 after the last interval -- so no zeroing pass is needed."""
+        if overwrite and stage & 2:
+            stage |= 4
         rc = lib().few_get_waveform_plan_async(
             self.wave.data_ptr(), self.interp.data_ptr(), self.t.data_ptr(), self.pcoef.data_ptr(),
             self.plan.handle, self.m.data_ptr(), self.k.data_ptr(), self.n.data_ptr(), self.L,
@@ -106,10 +110,9 @@ class ModeSumProblem:
         check(rc, "get_waveform")
 
     def run(self, precision: str = "fp64") -> torch.Tensor:
-        """One full step: zero the output, build splines, plan, sum."""
-        self.wave.zero_()
+        """One full step: build splines, plan, sum (writing every output sample)."""
         self.build_splines()
-        self.mode_sum(precision)
+        self.mode_sum(precision, overwrite=True)
         return self.wave
 
 

@@ -142,7 +142,10 @@ size_t few_plan_workspace_bytes(const void *plan, int init_len, int out_len, int
  * phase).  `stage` is a bitmask:
  * 1 = prepare (interval planning, coefficient tiles), 2 = sum (the output
  * kernel), 3 = both; stage 2 reuses what the last stage-1 call on the same
- * workspace prepared.  Workspace:
+ * workspace prepared.  Adding 4 to a stage with the sum (6, 7) overwrites
+ * instead of accumulating: every output sample is written (h where an
+ * interval covers it, 0 after the last one), so the caller need not zero
+ * d_waveform first.  Workspace:
  * few_plan_workspace_bytes() bytes, caller-owned, one call in flight at a
  * time.  No host synchronisation, no allocation: capturable in a hipGraph.
  */

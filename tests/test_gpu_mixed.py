@@ -101,3 +101,27 @@ def test_mixed_accumulates_and_edges(gpu, prec):
     h = (p.wave - init).cpu().numpy()
     ref = oracle_waveform(inp)
     assert oracle.rel_l2(h, ref) < MIXED_TOL
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed16"])
+def test_overwrite_equals_zero_then_accumulate(gpu, prec):
+    """stage flag 4 (overwrite): every output sample written -- h inside the
+    intervals, 0 in the tail after the last knot -- bit for bit what zeroing
+    the buffer and accumulating gives, whatever the buffer held before."""
+    import torch
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.pipeline import ModeSumProblem
+    inp = synthetic.small_input(seed=29, L=14, K=30, T_years=0.002)
+    inp.num_pts += 3000  # samples after the last knot
+    p = ModeSumProblem(inp)
+    p.build_splines()
+    p.wave.zero_()
+    p.mode_sum(prec)
+    ref = p.wave.clone()
+    tail = ref[-2000:]
+    assert torch.count_nonzero(tail) == 0
+    gen = torch.Generator(device=p.wave.device).manual_seed(5)
+    junk = torch.randn(inp.num_pts, 2, dtype=torch.float64, device=p.wave.device, generator=gen)
+    p.wave.copy_(torch.view_as_complex(junk))
+    p.mode_sum(prec, stage=2, overwrite=True)
+    assert torch.equal(p.wave, ref)
