@@ -124,6 +124,84 @@ __host__ __device__ inline double aak_jn(int n, double x)
     return x < 0.0 && n % 2 == 1 ? -ans : ans;
 }
 
+// One step j of Miller's backward recurrence (aak_jn's x <= n branch) for
+// the chains LO..4 of aak_jn5: each chain's operations and their order are
+// aak_jn's (update, renormalisation, the sum over even terms -- j odd, as m
+// is even -- and the capture at j = order).
+template <int LO>
+__host__ __device__ inline void aak_miller_step(int j, double tox, int nu0, double (&bj)[5],
+                                                double (&bjp)[5], double (&sum)[5],
+                                                double (&ans)[5])
+{
+    const double jt = j * tox;
+    bool big = false;
+#pragma unroll
+    for (int i = LO; i < 5; i++) {
+        const double bjm = jt * bj[i] - bjp[i];
+        bjp[i] = bj[i];
+        bj[i] = bjm;
+        big = big || fabs(bjm) > 1.0e10;
+    }
+    if (big) {
+#pragma unroll
+        for (int i = LO; i < 5; i++) {
+            if (fabs(bj[i]) > 1.0e10) {
+                bj[i] *= 1.0e-10;
+                bjp[i] *= 1.0e-10;
+                ans[i] *= 1.0e-10;
+                sum[i] *= 1.0e-10;
+            }
+        }
+    }
+    if (j & 1) {
+#pragma unroll
+        for (int i = LO; i < 5; i++) sum[i] += bj[i];
+    }
+#pragma unroll
+    for (int i = LO; i < 5; i++)
+        if (j == nu0 + i) ans[i] = bjp[i];
+}
+
+// aak_jn(nu0 + i, x), i = 0..4, for x >= 0 (the five Bessel values of one
+// harmonic), bit for bit.  Miller's recurrences of the five orders run in
+// lockstep (independent chains, instruction-level parallelism; the start
+// m(order) grows with the order, so chain i joins at j = m_i); orders 0 and
+// 1, x = 0 and x > order take aak_jn's other branches (the lockstep chain of
+// such an order runs and is discarded).
+__host__ __device__ inline void aak_jn5(int nu0, double x, double (&out)[5])
+{
+    const double ax = fabs(x);
+    const double tox = 2.0 / ax;
+    int m[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const int nu = nu0 + i;
+        m[i] = nu >= 2 ? 2 * ((nu + (int)sqrt(40.0 * nu)) / 2) : 0;
+    }
+    double bj[5], bjp[5], sum[5], ans[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        bj[i] = 1.0;
+        bjp[i] = ans[i] = sum[i] = 0.0;
+    }
+    int j = m[4];
+    for (; j > m[3]; j--) aak_miller_step<4>(j, tox, nu0, bj, bjp, sum, ans);
+    for (; j > m[2]; j--) aak_miller_step<3>(j, tox, nu0, bj, bjp, sum, ans);
+    for (; j > m[1]; j--) aak_miller_step<2>(j, tox, nu0, bj, bjp, sum, ans);
+    for (; j > m[0]; j--) aak_miller_step<1>(j, tox, nu0, bj, bjp, sum, ans);
+    for (; j > 0; j--) aak_miller_step<0>(j, tox, nu0, bj, bjp, sum, ans);
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        const int nu = nu0 + i;
+        if (nu >= 2 && ax != 0.0 && !(ax > (double)nu)) {
+            const double sm = 2.0 * sum[i] - bj[i];
+            out[i] = ans[i] / sm;
+        } else {
+            out[i] = aak_jn(nu, x);
+        }
+    }
+}
+
 // gpuAAK.cu:136-176: rotation coefficients source frame -> wave frame
 __host__ __device__ inline void aak_rot_coeff(double *rot, double iota, double theta_S, double phi_S,
                                               double theta_K, double phi_K, double alpha)
@@ -280,11 +358,10 @@ __host__ __device__ inline double2 aak_sample(const AakArgs &a, const double (*s
             nPhi = n * Phi;
         }
         const double ne = n * e;
-        const double J0 = (n == 1) ? -1.0 * aak_j1(ne) : aak_jn(n - 2, ne);
-        const double J1 = aak_jn(n - 1, ne);
-        const double J2 = aak_jn(n, ne);
-        const double J3 = aak_jn(n + 1, ne);
-        const double J4 = aak_jn(n + 2, ne);
+        double Jv[5];
+        aak_jn5(n - 2, ne, Jv);
+        const double J0 = (n == 1) ? -1.0 * aak_j1(ne) : Jv[0];
+        const double J1 = Jv[1], J2 = Jv[2], J3 = Jv[3], J4 = Jv[4];
         double sn, cn;
         sincos(nPhi, &sn, &cn);
         const double aa = -n * Amp * (J0 - 2. * e * J1 + 2. / n * J2 + 2. * e * J3 - J4) * cn;
