@@ -13,8 +13,9 @@ The headline path is the mixed-precision factorised sum (FP64 phases and
 spline build; the n-contraction on FP16 MFMA with hi/lo split operands and
 FP32 accumulation, or FP32 MFMA when the mode set's tile does not fit LDS);
 the plain FP64 GPU sum (the reference's arithmetic) is timed beside it as the
-denominator of the speed-up.  Further legs: the 256-waveform uneven batch, the
-differentiation (forward + backward), the frequency-domain sum.  The oracle
+denominator of the speed-up.  Further legs: config 2 (2 yr at 5 s, 2000 modes
+with n to 55), the 256-waveform uneven batch, the differentiation (forward +
+backward), the frequency-domain sum, AAK, interp2D.  The oracle
 (oracle/, test infrastructure) runs only in the CPU leg: as the CPU baseline and
 as the checker of values the GPU legs kept.  Prints ONE JSON line.
 """
@@ -83,7 +84,7 @@ def cpu_baseline(inp, seconds_target: float = 12.0):
                       "restating interpolate.cu's CPU branch (-O2 -ffp-contract=off), 1 thread"}
 
 
-def cpu_checks(inp, head, batch_chk, fd_chk, aak_chk=None, amp_chk=None):
+def cpu_checks(inp, head, batch_chk, fd_chk, aak_chk=None, amp_chk=None, mixed2_chk=None):
     """The CPU leg's checker role: the oracle's values at the samples / bins the
     GPU legs kept, compared with them (rel-L2), and the oracle's frequency-domain
     sum timed on the same check (1 thread)."""
@@ -100,6 +101,9 @@ def cpu_checks(inp, head, batch_chk, fd_chk, aak_chk=None, amp_chk=None):
         errs = [oracle.rel_l2(vals, oracle_waveform(x, idx, nthreads=nth))
                 for x, idx, vals in batch_chk]
         res["batch_max_of_8"] = float(max(errs))
+    if mixed2_chk is not None:
+        x, idx, vals = mixed2_chk
+        res["mixed2"] = oracle.rel_l2(vals, oracle_waveform(x, idx, nthreads=nth))
     if fd_chk is not None:
         c = fd_chk
         t0 = time.perf_counter()
@@ -233,6 +237,41 @@ def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256):
         log(f"[bench] batch fp64 control: {ms64:.1f} ms/step, speed-up {ms64 / ms_graph:.2f}x, "
             f"mixed16 rel-L2 vs fp64 (all samples) {float(diff):.3e}")
     del bp, h16, g
+    torch.cuda.empty_cache()
+    return out, chk
+
+
+def mixed2_leg(dev, steps: int = 4):
+    """Config 2, mixed precision at larger scale: T = 2 yr, dt = 5 s (12,623,260 samples),
+    200 knots, 2000 of the 12,987 modes (|m| <= l <= 10, n in [-55, 55]); the mixed16 step
+    (same stages as the headline) timed, every sample compared with the plain FP64 GPU path
+    (rel-L2 and mismatch), spread samples kept for the oracle in the CPU leg."""
+    import torch
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.pipeline import ModeSumProblem
+    inp = synthetic.make_workload("mixed2")
+    prob = ModeSumProblem(inp, dev)
+    prec = "mixed16" if prob.plan.supported16 else "mixed"
+    K, N = inp.K, inp.num_pts
+    ms_step, ms = time_path(prob, prec, steps, 1)
+    h = prob.wave.cpu().numpy()
+    ms64_step, ms64 = time_path(prob, "fp64", 1, 0)
+    h64 = prob.wave.cpu().numpy()
+    fa, fb = np.fft.fft(h), np.fft.fft(h64)
+    overlap = np.vdot(fa, fb) / np.sqrt(np.vdot(fa, fa) * np.vdot(fb, fb))
+    out = {"workload": f"config 2: T=2 yr, dt=5 s, N={N} samples, L={inp.L} knots, K={K} modes "
+                       f"(n in [-55, 55]), plan rows={prob.plan.rows} cols={prob.plan.cols}",
+           "precision": prec, "ms_per_step": ms_step, "kernel_ms": ms,
+           "mode_samples_per_s": K * N / (ms_step * 1e-3),
+           "fp64_ms_per_step": ms64_step, "speedup_vs_fp64": ms64_step / ms_step,
+           "rel_l2_vs_fp64_all_samples": float(np.linalg.norm(h - h64) / np.linalg.norm(h64)),
+           "mismatch_vs_fp64": float(1.0 - overlap.real)}
+    log(f"[bench] mixed2 {prec}: step {ms_step:.3f} ms ({out['mode_samples_per_s']:.4g} "
+        f"mode-samples/s), fp64 {ms64_step:.1f} ms; rel-L2 vs fp64 "
+        f"{out['rel_l2_vs_fp64_all_samples']:.3e}, mismatch {out['mismatch_vs_fp64']:.3e}")
+    idx = spread_indices(N, count=600, seed=9)
+    chk = (inp, idx, h[idx].copy())
+    del prob, h, h64, fa, fb
     torch.cuda.empty_cache()
     return out, chk
 
@@ -594,6 +633,8 @@ def main():
                     help="skip the differentiation leg (forward + backward, FD check)")
     ap.add_argument("--no-fd", action="store_true",
                     help="skip the frequency-domain sum leg")
+    ap.add_argument("--no-mixed2", action="store_true",
+                    help="skip the config-2 leg (2 yr at 5 s, 200 knots, 2000 modes, n to 55)")
     ap.add_argument("--no-aak", action="store_true",
                     help="skip the AAK and interp2D legs")
     ap.add_argument("--fd-coords", type=int, default=64,
@@ -694,6 +735,10 @@ def main():
         batch, batch_chk = batch_leg(dev, max(2, args.steps // 2), 1, not args.no_fp64,
                                      args.batch_count)
 
+    mixed2 = mixed2_chk = None
+    if not args.no_mixed2:
+        mixed2, mixed2_chk = mixed2_leg(dev)
+
     fd = fd_chk = None
     if not args.no_fd:
         fd, fd_chk = fd_leg(dev)
@@ -711,12 +756,15 @@ def main():
     err = None
     if not args.no_cpu:
         log("[bench] CPU leg: oracle checks and baseline (1 thread)")
-        chk = cpu_checks(inp, head_chk, batch_chk, fd_chk, aak_chk, amp_chk)
+        chk = cpu_checks(inp, head_chk, batch_chk, fd_chk, aak_chk, amp_chk, mixed2_chk)
         err = chk["headline"]
         log(f"[bench] rel-L2 vs oracle ({len(idx)} samples) {err:.3e}")
         if fp64 is not None:
             ref = chk["headline_ref"]
             fp64["rel_l2_vs_oracle"] = float(np.linalg.norm(fp64_chk - ref) / np.linalg.norm(ref))
+        if mixed2 is not None:
+            mixed2["rel_l2_vs_oracle_600_samples"] = chk["mixed2"]
+            log(f"[bench] mixed2 rel-L2 vs oracle (600 samples) {chk['mixed2']:.3e}")
         if batch is not None:
             batch["rel_l2_vs_oracle_max_of_8"] = chk["batch_max_of_8"]
             log(f"[bench] batch rel-L2 (8 waveforms vs oracle) max {chk['batch_max_of_8']:.3e}")
@@ -762,6 +810,7 @@ def main():
         "fp64_plain_gpu": fp64,
         "speedup_vs_fp64_plain_gpu": (fp64["ms_per_step"] / ms_step) if fp64 else None,
         "batch": batch,
+        "mixed2": mixed2,
         "adjoint": adjoint,
         "fd": fd,
         "aak": aak,

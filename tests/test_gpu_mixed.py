@@ -125,3 +125,24 @@ def test_overwrite_equals_zero_then_accumulate(gpu, prec):
     p.wave.copy_(torch.view_as_complex(junk))
     p.mode_sum(prec, stage=2, overwrite=True)
     assert torch.equal(p.wave, ref)
+
+
+def test_mixed16_config2_full_size(gpu):
+    """Config 2 at its real size (T = 2 yr at dt = 5 s: 12,623,260 samples; 200
+    knots; 2000 modes with n in [-55, 55]): the split-FP16 sum against the FP64
+    GPU path on every sample and against the oracle on spread samples."""
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.pipeline import ModeSumProblem
+    inp = synthetic.make_workload("mixed2")
+    assert inp.num_pts == 12623260 and inp.L == 200 and inp.K == 2000
+    p = ModeSumProblem(inp)
+    assert p.plan.supported16
+    h64 = p.run("fp64").cpu().numpy()
+    hm = p.run("mixed16").cpu().numpy()
+    err = oracle.rel_l2(hm, h64)
+    idx = spread_indices(inp.num_pts, count=400)
+    ref = oracle_waveform(inp, idx)
+    err_o = oracle.rel_l2(hm[idx], ref)
+    err_64 = oracle.rel_l2(h64[idx], ref)
+    print(f"config 2 mixed16: rel-L2 vs fp64 GPU {err:.3e}, vs oracle {err_o:.3e}; fp64 vs oracle {err_64:.3e}")
+    assert err < MIXED_TOL and err_o < MIXED_TOL and err_64 < 1e-12
