@@ -264,3 +264,36 @@ def test_direct_mode_sum(gpu):
     ref = w1 + w2
     assert len(h) == N
     assert oracle.rel_l2(h, ref) < 1e-13
+
+
+@pytest.mark.parametrize("length", [3, 4, 5, 9, 33, 120, 150, 151])
+def test_spline_shared_knots_bitwise(gpu, length):
+    """The shared-knot block path of the spline build (t_sys = 0: the matrix
+    eliminated once per block, 32 systems per block, coefficients spread over
+    four waves) at small, odd and limit lengths (150 is the last LDS length,
+    151 takes the per-lane path) and system counts that leave partial blocks:
+    bit for bit the oracle's interpolate_arrays on the tiled knots.  3 knots
+    is singular (LAPACK's early return; the block falls back per system)."""
+    import torch
+    from fastemriwaveforms_amd._lib import lib
+    from fastemriwaveforms_amd.pipeline import _stream
+    rng = np.random.default_rng(length)
+    for nsys in (1, 31, 33, 70):
+        h = rng.uniform(0.2, 1.0, length - 1)
+        h[rng.random(length - 1) < 0.2] *= 40.0  # uneven spacing: row interchanges
+        t = np.concatenate([[0.0], np.cumsum(h)]) * 1e4
+        y = rng.normal(size=(nsys, length))
+        dev = torch.device("cuda", 0)
+        t_d = torch.from_numpy(t).to(dev)
+        y_d = torch.from_numpy(y).to(dev)
+        out = torch.zeros((4, nsys, length), dtype=torch.float64, device=dev)
+        scratch = torch.zeros(4 * length * nsys, dtype=torch.float64, device=dev)
+        rc = lib().few_spline_build_async(y_d.data_ptr(), length, 1, t_d.data_ptr(), 0, 1,
+                                          out.data_ptr(), nsys * length, length, 1, nsys, length,
+                                          0, scratch.data_ptr(), _stream())
+        assert rc == 0, lib().few_last_error()
+        torch.cuda.synchronize()
+        ref, _, info = oracle.interpolate_arrays(np.tile(t, (nsys, 1)), y)
+        assert info == (3 if length == 3 else 0)
+        got = out.cpu().numpy()
+        assert np.array_equal(got, ref), (nsys, length)
