@@ -221,33 +221,150 @@ __global__ __launch_bounds__(64) void tri_factor_kernel(const double *t, int L, 
     }
 }
 
-// One workgroup: 64 systems of group gi on the shared knots, with the knots,
-// the elimination and both right-hand-side tiles (scratch and knot-value
-// gradients) in LDS; the arithmetic is spline_adjoint_rhs's.  Two waves:
-// wave 1 (one lane) eliminates the transposed matrix while wave 0 runs the
-// first pass, which needs only the knots.
-__global__ __launch_bounds__(128) void spline_adjoint_lds_kernel(SplineAdjArgs a)
+// Row r of tri_factor_transposed's matrix (d, dl, du) and of its fill_B
+// weights (al, be): the same expressions, one row.
+__device__ inline void tri_row_transposed(const double *x, int L, int r, double *F)
 {
+#pragma clang fp contract(off)
+    double *d = F + 2L * L, *du = F + 3L * L, *dl = F + 4L * L;
+    double *al = F + 5L * L, *be = F + 6L * L;
+    double sub = 0.0, sup = 0.0, dg;
+    if (r == L - 1) {
+        dg = x[L - 2] - x[L - 3];
+    } else if (r == 0) {
+        dg = x[2] - x[1];
+        sub = x[2] - x[0];
+    } else {
+        dg = 2 * ((x[r] - x[r - 1]) + (x[r + 1] - x[r]));
+        sub = x[r] - x[r - 1];
+    }
+    if (r < L - 1) sup = (r + 1 == L - 1) ? x[L - 1] - x[L - 3] : x[r + 2] - x[r + 1];
+    d[r] = dg;
+    dl[r] = sub;
+    du[r] = sup;
+    if (r == 0) {
+        const double dx1 = x[1] - x[0], dx2 = x[2] - x[1], dd = x[2] - x[0];
+        al[0] = (dx1 + 2 * dd) * dx2 / (dx1 * dd);
+        be[0] = dx1 * dx1 / (dx2 * dd);
+    } else if (r == L - 1) {
+        const double dx1 = x[L - 2] - x[L - 3], dx2 = x[L - 1] - x[L - 2];
+        const double dd = x[L - 1] - x[L - 3];
+        al[L - 1] = dx2 * dx2 / (dx1 * dd);
+        be[L - 1] = (2 * dd + dx2) * dx1 / (dx2 * dd);
+    } else {
+        const double dx1 = x[r] - x[r - 1], dx2 = x[r + 1] - x[r];
+        al[r] = 3.0 * dx2 / dx1;
+        be[r] = 3.0 * dx1 / dx2;
+    }
+}
+
+// spline_adjoint_pass1's work for interval i of one system (the same
+// expressions): z part of S(i), the pending z it hands to S(i+1), q.
+struct AdjIv {
+    double zpart, pend, q, g0;
+};
+__device__ inline AdjIv adj_interval(const double *x, int L, const double *g, long gp, long gk,
+                                     int i)
+{
+#pragma clang fp contract(off)
+#define G(p, j) g[(long)(p) * gp + (long)(j) * gk]
+    double g0 = G(0, i), g1 = G(1, i), g2 = G(2, i), g3 = G(3, i);
+    const double dx = x[i + 1] - x[i];
+    if (i == L - 2) {
+        const double h = x[L - 1] - x[L - 2];
+        const double f1 = G(1, L - 1), f2 = G(2, L - 1), f3 = G(3, L - 1);
+        g1 += f1;
+        g2 += 2.0 * h * f1 + f2;
+        g3 += 3.0 * h * h * f1 + 3.0 * h * f2 + f3;
+    }
+#undef G
+    const double gs = 3.0 * g2 / dx - 2.0 * g3 / (dx * dx);
+    AdjIv v;
+    v.zpart = g1 - 2.0 * g2 / dx + g3 / (dx * dx);
+    v.pend = 0.0 + (-g2 / dx + g3 / (dx * dx));
+    v.q = gs / dx;
+    v.g0 = g0;
+    return v;
+}
+
+#define ADJ_SPB 32  // systems per shared-knot block (half a wave)
+
+// One workgroup: ADJ_SPB systems of group gi on the shared knots, four
+// waves, the knots, the elimination and both right-hand-side tiles
+// (scratch z / lambda and the knot-value gradients) in LDS; the arithmetic is
+// spline_adjoint_rhs's, bit for bit.  What is parallel over (system, knot)
+// is spread over the waves; only the sweeps are sequential along the knots:
+//   wave 3:    A^T's rows and fill_B weights (one row per lane), then dgtsv's
+//              elimination chain (every lane the same chain);
+//   waves 0-2: meanwhile the first pass, one (system, interval) per lane
+//              (each lane recomputes the previous interval's carries);
+//   wave 0:    the sweeps of A^T lambda = z, one system per lane;
+//   waves 0-3: fill_B transposed, one (system, knot) per lane: the rows
+//              touching the knot in ascending order, as the sequential pass
+//              adds them.
+__global__ __launch_bounds__(256) void spline_adjoint_lds_kernel(SplineAdjArgs a)
+{
+    constexpr int S = ADJ_SPB;
     extern __shared__ __attribute__((aligned(16))) double adj_lds[];
     int gi = 0;
     while (gi + 1 < a.ngroups && (int)blockIdx.x >= a.blk_prefix[gi + 1]) gi++;
     const SplineAdjGroup &G = a.grp[gi];
     const int L = a.L, tid = threadIdx.x;
-    const long s0 = (long)((int)blockIdx.x - a.blk_prefix[gi]) * 64;
-    double *sx = adj_lds, *F = sx + L, *sS = F + 7L * L, *sG = sS + 64L * L;
-    for (int i = tid; i < L; i += 128) sx[i] = a.t[i];
+    const int lane = tid & 63, wv = tid >> 6, half = lane / S, ls = lane % S;
+    const long s0 = (long)((int)blockIdx.x - a.blk_prefix[gi]) * S;
+    const int nloc = (int)min((long)S, G.nsys - s0);
+    double *sx = adj_lds, *F = sx + L, *sS = F + 7L * L, *sG = sS + (long)S * L;
+    for (int i = tid; i < L; i += 256) sx[i] = a.t[i];
     __syncthreads();
-    const long s = s0 + tid;
-    if (tid >= 64) {
-        if (tid == 64) tri_factor_transposed(sx, L, F);
-    } else if (s < G.nsys) {
-        spline_adjoint_pass1(sx, L, G.g + s * G.g_sys, G.g_p, G.g_knot, sG + tid, 64, sS + tid, 64);
+    if (wv == 3) {
+        for (int r = lane; r < L; r += 64) tri_row_transposed(sx, L, r, F);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        (void)dgtsv_factor_ahead<8>(L, F + 2L * L, F + 3L * L, F + 4L * L, F, F + L);
+    } else if (ls < nloc) {
+        const double *g = G.g + (s0 + ls) * G.g_sys;
+        // interval i on lane group 2 wv + half of waves 0-2 (6 groups)
+        for (int i = 2 * wv + half; i < L - 1; i += 6) {
+            const AdjIv v = adj_interval(sx, L, g, G.g_p, G.g_knot, i);
+            double pend_prev = 0.0, q_prev = 0.0;
+            if (i > 0) {
+                const AdjIv w = adj_interval(sx, L, g, G.g_p, G.g_knot, i - 1);
+                pend_prev = w.pend;
+                q_prev = w.q;
+            }
+            sS[(long)i * S + ls] = pend_prev + v.zpart;
+            sG[(long)i * S + ls] = (i > 0 ? v.g0 + q_prev : v.g0) - v.q;
+            if (i == L - 2) {
+                sS[(long)(L - 1) * S + ls] = v.pend;
+                sG[(long)(L - 1) * S + ls] = g[(long)(L - 1) * G.g_knot] + v.q;
+            }
+        }
     }
     __syncthreads();
-    if (tid >= 64 || s >= G.nsys) return;
-    spline_adjoint_solve(sx, L, F, sG + tid, 64, sS + tid, 64);
-    double *gy = G.gy + s * G.gy_sys;
-    for (int i = 0; i < L; i++) gy[(long)i * G.gy_knot] = sG[(long)i * 64 + tid];
+    if (wv == 0 && half == 0) dgtsv_sweeps(L, F, sS + ls, S);
+    __syncthreads();
+    if (ls >= nloc) return;
+    {
+#pragma clang fp contract(off)
+        const double *al = F + 5L * L, *be = F + 6L * L;
+        double *gy = G.gy + (s0 + ls) * G.gy_sys;
+        for (int k = 2 * wv + half; k < L; k += 8) {
+            double y = sG[(long)k * S + ls];
+            const int r0 = k - 2 > 0 ? k - 2 : 0, r1 = k + 2 < L - 1 ? k + 2 : L - 1;
+            for (int r = r0; r <= r1; r++) {
+                const int st = (r == 0) ? 0 : (r == L - 1 ? L - 3 : r - 1);
+                const int pos = k - st;
+                const double lam = sS[(long)r * S + ls];
+                if (pos == 0)
+                    y -= al[r] * lam;
+                else if (pos == 1)
+                    y += (al[r] - be[r]) * lam;
+                else if (pos == 2)
+                    y += be[r] * lam;
+            }
+            gy[(long)k * G.gy_knot] = y;
+        }
+    }
 }
 
 __global__ __launch_bounds__(64) void spline_adjoint_kernel(SplineAdjArgs a)
@@ -477,11 +594,12 @@ int launch_spline_adjoint(const SplineAdjArgs &a, hipStream_t stream)
         SplineAdjArgs b = a;
         b.blk_prefix[0] = 0;
         for (int g = 0; g < 3; g++)
-            b.blk_prefix[g + 1] = b.blk_prefix[g] + (g < a.ngroups ? (a.grp[g].nsys + 63) / 64 : 0);
-        const size_t lds = sizeof(double) * (size_t)a.L * (8 + 128);
+            b.blk_prefix[g + 1] =
+                b.blk_prefix[g] + (g < a.ngroups ? (a.grp[g].nsys + ADJ_SPB - 1) / ADJ_SPB : 0);
+        const size_t lds = sizeof(double) * (size_t)a.L * (8 + 2 * ADJ_SPB);
         FEW_LDS(spline_adjoint_lds_kernel, (int)lds);
         hipLaunchKernelGGL(spline_adjoint_lds_kernel, dim3((unsigned)b.blk_prefix[a.ngroups]),
-                           dim3(128), lds, stream, b);
+                           dim3(256), lds, stream, b);
         FEW_HIP(hipGetLastError());
         return FEW_OK;
     }

@@ -69,8 +69,8 @@ struct SplineAdjArgs {
     int blk_prefix[4];        // first workgroup of each group (LDS path)
 };
 
-// Knot counts up to this run the LDS path: knots, elimination and two L x 64
-// tiles of one workgroup's right-hand sides stay in LDS (8 L (8 + 128) bytes).
+// Knot counts up to this run the LDS path: knots, elimination and two L x 32
+// tiles of one workgroup's right-hand sides stay in LDS (8 L (8 + 64) bytes).
 #define ADJ_SHARED_MAX_L 150
 #define ADJ_PF 8               // prefetch distance (intervals) of the gradient loads
 

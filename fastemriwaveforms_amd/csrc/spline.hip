@@ -476,70 +476,6 @@ __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
     spline_system_core(G, s, a.length, a.col_prefix[gi] + s, a.nsys_total, sc);
 }
 
-#define SPLINE_SWEEP_PF 8  // LDS values fetched ahead of the sweep recurrences
-
-// dgtsv's forward sweep of one right-hand side b with the recorded
-// elimination F, then the back substitution: slopes into b.  The same
-// operations as spline_solve_emit's first half; the operands of the next
-// SPLINE_SWEEP_PF steps are read from LDS before the dependent arithmetic.
-__host__ __device__ inline void spline_sweeps(int n, const double *F, double *bv, long bs)
-{
-#pragma clang fp contract(off)
-#define SB(i) bv[(long)(i) * bs]
-    const double *piv = F, *fact = F + n, *d = F + 2 * n, *du = F + 3 * n, *dl = F + 4 * n;
-    double cur = SB(0);
-    for (int i0 = 0; i0 < n - 1; i0 += SPLINE_SWEEP_PF) {
-        double nbv[SPLINE_SWEEP_PF], fv[SPLINE_SWEEP_PF], pv[SPLINE_SWEEP_PF];
-#pragma unroll
-        for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
-            const int i = i0 + u < n - 2 ? i0 + u : n - 2;
-            nbv[u] = SB(i + 1);
-            fv[u] = fact[i];
-            pv[u] = piv[i];
-        }
-#pragma unroll
-        for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
-            const int i = i0 + u;
-            if (i < n - 1) {
-                // no interchange: b[i] stays, b[i+1] -= f b[i];
-                // interchange:    (b[i], b[i+1]) = (b[i+1], b[i] - f b[i+1])
-                const double nb = nbv[u];
-                const bool keep = pv[u] == 0.0;
-                SB(i) = keep ? cur : nb;
-                cur = keep ? nb - fv[u] * cur : cur - fv[u] * nb;
-            }
-        }
-    }
-    double x_next = cur / d[n - 1];
-    SB(n - 1) = x_next;
-    // row n - 2 has no second super-diagonal term
-    double x_next2 = x_next;
-    x_next = (SB(n - 2) - du[n - 2] * x_next) / d[n - 2];
-    SB(n - 2) = x_next;
-    for (int i0 = n - 3; i0 >= 0; i0 -= SPLINE_SWEEP_PF) {
-        double bvv[SPLINE_SWEEP_PF], duv[SPLINE_SWEEP_PF], dlv[SPLINE_SWEEP_PF], dv[SPLINE_SWEEP_PF];
-#pragma unroll
-        for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
-            const int ii = i0 - u > 0 ? i0 - u : 0;
-            bvv[u] = SB(ii);
-            duv[u] = du[ii];
-            dlv[u] = dl[ii];
-            dv[u] = d[ii];
-        }
-#pragma unroll
-        for (int u = 0; u < SPLINE_SWEEP_PF; u++) {
-            const int ii = i0 - u;
-            if (ii >= 0) {
-                const double xi = (bvv[u] - duv[u] * x_next - dlv[u] * x_next2) / dv[u];
-                SB(ii) = xi;
-                x_next2 = x_next;
-                x_next = xi;
-            }
-        }
-    }
-#undef SB
-}
-
 #define SPLINE_BLOCK 256  // threads of the shared-knot block (4 waves)
 #define SPLINE_SPB 32     // systems per shared-knot block (half a wave)
 
@@ -620,7 +556,7 @@ __device__ void spline_block_shared(const SplineArgs &a, int gi, int blk, double
         if (wv == 0 && half == 0) spline_one_system(a, gi, s0 + ls);
         return;
     }
-    if (wv == 0 && half == 0) spline_sweeps(n, F, sb + ls, S);
+    if (wv == 0 && half == 0) dgtsv_sweeps(n, F, sb + ls, S);
     __syncthreads();
     if (ls >= nloc) return;
     // coefficients of interval i on lane group i % 8 (wave, half); the group

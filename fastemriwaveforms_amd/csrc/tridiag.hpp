@@ -157,4 +157,69 @@ __host__ __device__ inline int dgtsv_factor_ahead(int n, double *d, double *du, 
     return 0;
 }
 
+#define DGTSV_SWEEP_PF 8  // values fetched ahead of the sweep recurrences
+
+// dgtsv's forward sweep of one right-hand side b (stride bs) with the
+// elimination F = piv | fact | d | du | dl recorded by dgtsv_factor, then the
+// back substitution: the solution into b.  dgtsv's operations in its order;
+// the operands of the next DGTSV_SWEEP_PF steps are read before the dependent
+// arithmetic and each pivot choice is a select (no branch on the chain).
+__host__ __device__ inline void dgtsv_sweeps(int n, const double *F, double *bv, long bs)
+{
+#pragma clang fp contract(off)
+#define SB(i) bv[(long)(i) * bs]
+    const double *piv = F, *fact = F + n, *d = F + 2 * n, *du = F + 3 * n, *dl = F + 4 * n;
+    double cur = SB(0);
+    for (int i0 = 0; i0 < n - 1; i0 += DGTSV_SWEEP_PF) {
+        double nbv[DGTSV_SWEEP_PF], fv[DGTSV_SWEEP_PF], pv[DGTSV_SWEEP_PF];
+#pragma unroll
+        for (int u = 0; u < DGTSV_SWEEP_PF; u++) {
+            const int i = i0 + u < n - 2 ? i0 + u : n - 2;
+            nbv[u] = SB(i + 1);
+            fv[u] = fact[i];
+            pv[u] = piv[i];
+        }
+#pragma unroll
+        for (int u = 0; u < DGTSV_SWEEP_PF; u++) {
+            const int i = i0 + u;
+            if (i < n - 1) {
+                // no interchange: b[i] stays, b[i+1] -= f b[i];
+                // interchange:    (b[i], b[i+1]) = (b[i+1], b[i] - f b[i+1])
+                const double nb = nbv[u];
+                const bool keep = pv[u] == 0.0;
+                SB(i) = keep ? cur : nb;
+                cur = keep ? nb - fv[u] * cur : cur - fv[u] * nb;
+            }
+        }
+    }
+    double x_next = cur / d[n - 1];
+    SB(n - 1) = x_next;
+    // row n - 2 has no second super-diagonal term
+    double x_next2 = x_next;
+    x_next = (SB(n - 2) - du[n - 2] * x_next) / d[n - 2];
+    SB(n - 2) = x_next;
+    for (int i0 = n - 3; i0 >= 0; i0 -= DGTSV_SWEEP_PF) {
+        double bvv[DGTSV_SWEEP_PF], duv[DGTSV_SWEEP_PF], dlv[DGTSV_SWEEP_PF], dv[DGTSV_SWEEP_PF];
+#pragma unroll
+        for (int u = 0; u < DGTSV_SWEEP_PF; u++) {
+            const int ii = i0 - u > 0 ? i0 - u : 0;
+            bvv[u] = SB(ii);
+            duv[u] = du[ii];
+            dlv[u] = dl[ii];
+            dv[u] = d[ii];
+        }
+#pragma unroll
+        for (int u = 0; u < DGTSV_SWEEP_PF; u++) {
+            const int ii = i0 - u;
+            if (ii >= 0) {
+                const double xi = (bvv[u] - duv[u] * x_next - dlv[u] * x_next2) / dv[u];
+                SB(ii) = xi;
+                x_next2 = x_next;
+                x_next = xi;
+            }
+        }
+    }
+#undef SB
+}
+
 }  // namespace few

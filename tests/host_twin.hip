@@ -323,7 +323,7 @@ void twin_fd_spa(const double *x, long n, double *out)
 // The shared-knot block's pieces against the plain routines, for knots x and
 // one array y (n knots): F_plain / F_ahead (5n each: piv | fact | d | du | dl)
 // from dgtsv_factor and dgtsv_factor_ahead on the same matrix, their INFOs;
-// slopes (n) from spline_sweeps on the right-hand side; c1 (n - 1) from
+// slopes (n) from dgtsv_sweeps on the right-hand side; c1 (n - 1) from
 // spline_system_shared (the slopes it emits).
 void twin_spline_block_pieces(const double *x, const double *y, int n, double *F_plain,
                               double *F_ahead, int *info, double *slopes, double *c1)
@@ -340,7 +340,7 @@ void twin_spline_block_pieces(const double *x, const double *y, int n, double *F
     info[1] = dgtsv_factor_ahead<8>(n, F_ahead + 2 * n, F_ahead + 3 * n, F_ahead + 4 * n, F_ahead,
                                     F_ahead + n);
     spline_rhs_pass(n, x, y, 1, slopes, 1);
-    if (info[1] == 0) spline_sweeps(n, F_ahead, slopes, 1);
+    if (info[1] == 0) dgtsv_sweeps(n, F_ahead, slopes, 1);
     std::vector<double> out(4 * (size_t)n), b(n);
     SplineGroup G{y, 0, 1, x, 0, 1, out.data(), n, 0, 1, 1, 0};
     if (info[0] == 0) spline_system_shared(G, 0, n, x, F_plain, y, 1, b.data(), 1);

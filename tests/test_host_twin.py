@@ -228,7 +228,7 @@ def test_twin_sincos_cw(tw):
 @pytest.mark.parametrize("n", [3, 4, 5, 9, 17, 120, 150])
 def test_twin_spline_block_pieces_bitwise(tw, n):
     """The shared-knot block's read-ahead, branch-free elimination and sweeps
-    (dgtsv_factor_ahead, spline_sweeps) against dgtsv_factor and the plain
+    (dgtsv_factor_ahead, dgtsv_sweeps) against dgtsv_factor and the plain
     sweep of spline_system_shared, bit for bit, on knots whose spacings force
     row interchanges at random rows."""
     tw.twin_spline_block_pieces.argtypes = [P, P, ctypes.c_int, P, P, P, P, P]
