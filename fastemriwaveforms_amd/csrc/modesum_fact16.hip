@@ -168,7 +168,7 @@ __device__ __forceinline__ void split_pack(const f16v_t (&vr)[4], const f16v_t (
 {
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        const v2f x = {f32_opaque((float)vr[t]), f32_opaque((float)vi[t])};
+        const v2f x = {(float)vr[t], (float)vi[t]};
         const v2h h = __builtin_convertvector(x, v2h);  // one packed conversion
         const v2h l = f16_split_lo(h, x);
         bh[2 * t] = h.x;
