@@ -142,7 +142,8 @@ size_t few_plan_workspace_bytes(const void *plan, int init_len, int out_len, int
  * phase).  `stage` is a bitmask:
  * 1 = prepare (interval planning, coefficient tiles), 2 = sum (the output
  * kernel), 3 = both; stage 2 reuses what the last stage-1 call on the same
- * workspace prepared.  Adding 4 to a stage with the sum (6, 7) overwrites
+ * workspace prepared, and is refused (FEW_ERR_ARG) unless that call had the
+ * same plan, precision, variant, init_len and out_len.  Adding 4 to a stage with the sum (6, 7) overwrites
  * instead of accumulating: every output sample is written (h where an
  * interval covers it, 0 after the last one), so the caller need not zero
  * d_waveform first.  Workspace:

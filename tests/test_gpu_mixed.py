@@ -6,6 +6,7 @@ factorised sum is also compared with the FP64 GPU path on every sample.
 
 import numpy as np
 import pytest
+import torch
 
 import oracle
 from helpers import oracle_waveform, spread_indices
@@ -146,3 +147,32 @@ def test_mixed16_config2_full_size(gpu):
     err_64 = oracle.rel_l2(h64[idx], ref)
     print(f"config 2 mixed16: rel-L2 vs fp64 GPU {err:.3e}, vs oracle {err_o:.3e}; fp64 vs oracle {err_64:.3e}")
     assert err < MIXED_TOL and err_o < MIXED_TOL and err_64 < 1e-12
+
+
+def test_stage2_must_match_stage1(gpu):
+    """The interval plan and coefficient tiles a stage-1 call leaves in the
+    workspace depend on the precision and the variant's tile shape: a stage-2
+    call that does not match is refused instead of summing with a stale plan."""
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd._lib import FewRocmError
+    from fastemriwaveforms_amd.pipeline import ModeSumProblem
+    inp = synthetic.small_input(seed=44, L=16, K=40, T_years=0.004)
+    p = ModeSumProblem(inp)
+    p.build_splines()
+    p.variant = 0
+    p.mode_sum("mixed16", stage=1)
+    p.mode_sum("mixed16", stage=2, overwrite=True)
+    ref = p.wave.clone()
+    p.variant = 4
+    with pytest.raises(FewRocmError, match="prepared"):
+        p.mode_sum("mixed16", stage=2, overwrite=True)
+    with pytest.raises(FewRocmError, match="prepared"):
+        p.mode_sum("mixed", stage=2, overwrite=True)
+    p.mode_sum("mixed16", stage=1)
+    p.mode_sum("mixed16", stage=2, overwrite=True)
+    assert torch.equal(p.wave, ref)
+    # another problem's plan on this workspace
+    q = ModeSumProblem(inp)
+    q.ws = p.ws
+    with pytest.raises(FewRocmError, match="another mode plan"):
+        q.mode_sum("mixed16", stage=2, overwrite=True)
