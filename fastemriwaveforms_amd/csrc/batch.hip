@@ -291,14 +291,18 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
                   reinterpret_cast<int *>(w + L.wt)};
     BatchPtrs p{t, teuk, phase_knots, ylm, interp, pcoef, reinterpret_cast<double *>(w + L.region),
                 static_cast<double2 *>(wave)};
-    int rc;
+    const int tile = precision == FEW_PRECISION_FP64 ? direct_tile(FEW_PRECISION_FP64)
+                                                     : fact16_tile(variant);
+    const PrepKey key{workspace, batch, precision, tile, 0, 0};
+    int rc = prep_stage(stage & 2, stage & 4, key, "batch_run");
+    if (rc) return rc;
     if (stage & 1) {
         rc = launch_batch_splines(bp.dev, p, s);
         if (rc) return rc;
     }
     if (precision == FEW_PRECISION_FP64) {
         if (stage & 2) {
-            rc = launch_batch_plan(bp.dev, r, p, direct_tile(FEW_PRECISION_FP64), s);
+            rc = launch_batch_plan(bp.dev, r, p, tile, s);
             if (rc) return rc;
         }
         if (stage & 4) {
@@ -310,7 +314,6 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
         }
         return FEW_OK;
     }
-    const int tile = fact16_tile(variant);
     if (stage & 2) {
         rc = launch_batch_plan(bp.dev, r, p, tile, s);
         if (rc) return rc;

@@ -181,16 +181,8 @@ static int launch_zero_complex(double2 *w, long n, hipStream_t stream)
     return FEW_OK;
 }
 
-// What the last stage-1 call prepared in each workspace: the interval plan and
-// the coefficient tiles depend on the precision, the samples per tile (the
-// variant), the knots and the output length.  A stage-2 call that does not
-// match (another variant, precision or size) would sum with a stale plan, so it
-// is refused.  Keyed by workspace address; bounded, cleared when full.
-struct PrepKey {
-    const void *ws;
-    const void *plan;
-    int precision, tile, L, out_len;
-};
+// What the last preparing call left in each workspace (see few_common.hpp).
+// Keyed by workspace address; bounded, cleared when full.
 static std::mutex g_prep_mu;
 static std::vector<PrepKey> g_prep;
 
@@ -206,7 +198,7 @@ static void prep_record(const PrepKey &k)
     g_prep.push_back(k);
 }
 
-static int prep_check(const PrepKey &k)
+static int prep_check(const PrepKey &k, const char *what)
 {
     std::lock_guard<std::mutex> lk(g_prep_mu);
     for (const PrepKey &e : g_prep)
@@ -214,26 +206,25 @@ static int prep_check(const PrepKey &k)
             if (e.plan == k.plan && e.precision == k.precision && e.tile == k.tile &&
                 e.L == k.L && e.out_len == k.out_len)
                 return FEW_OK;
-            set_error("get_waveform: stage 2 with precision %d, %d samples per tile, L %d, "
-                      "%d samples%s; the workspace was prepared (stage 1) with precision %d, "
-                      "%d samples per tile, L %d, %d samples",
-                      k.precision, k.tile, k.L, k.out_len,
-                      e.plan == k.plan ? "" : " and another mode plan", e.precision, e.tile, e.L,
+            set_error("%s: sum with precision %d, %d samples per tile, L %d, %d samples%s; the "
+                      "workspace was prepared with precision %d, %d samples per tile, L %d, "
+                      "%d samples",
+                      what, k.precision, k.tile, k.L, k.out_len,
+                      e.plan == k.plan ? "" : " and another plan", e.precision, e.tile, e.L,
                       e.out_len);
             return FEW_ERR_ARG;
         }
-    set_error("get_waveform: stage 2 on a workspace no stage-1 call prepared");
+    set_error("%s: sum on a workspace no preparing call prepared", what);
     return FEW_ERR_ARG;
 }
 
-// Stage 1 records what it prepares; stage 2 alone checks it.
-static int prep_stage(int stage, const PrepKey &k)
+int prep_stage(bool prepares, bool sums, const PrepKey &k, const char *what)
 {
-    if (stage & 1) {
+    if (prepares) {
         prep_record(k);
         return FEW_OK;
     }
-    return (stage & 2) ? prep_check(k) : FEW_OK;
+    return sums ? prep_check(k, what) : FEW_OK;
 }
 
 // Plans (device) and runs one waveform's mode sum on `stream`; h_t and pt
@@ -277,7 +268,8 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
             set_error("get_waveform: output too long (%d samples)", out_len);
             return FEW_ERR_LIMIT;
         }
-        if (int rc = prep_stage(stage, PrepKey{ws, plan, precision, tile, L, out_len})) return rc;
+        const PrepKey key{ws, plan, precision, tile, L, out_len};
+        if (int rc = prep_stage(stage & 1, stage & 2, key, "get_waveform")) return rc;
         if (stage & 1) {
             PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
             int rc = launch_plan(p, stream);
@@ -317,7 +309,8 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
             set_error("get_waveform: output too long (%d samples)", out_len);
             return FEW_ERR_LIMIT;
         }
-        if (int rc = prep_stage(stage, PrepKey{ws, plan, precision, tile, L, out_len})) return rc;
+        const PrepKey key{ws, plan, precision, tile, L, out_len};
+        if (int rc = prep_stage(stage & 1, stage & 2, key, "get_waveform")) return rc;
         if (stage & 1) {
             PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
             int rc = launch_plan(p, stream);
@@ -339,7 +332,8 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
         return FEW_ERR_UNSUPPORTED;
     }
     const int tile = direct_tile(precision);
-    if (int rc = prep_stage(stage, PrepKey{ws, plan, precision, tile, L, out_len})) return rc;
+    const PrepKey key{ws, plan, precision, tile, L, out_len};
+    if (int rc = prep_stage(stage & 1, stage & 2, key, "get_waveform")) return rc;
     if (stage & 1) {
         PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
         int rc = launch_plan(p, stream);

@@ -30,6 +30,18 @@ int check_hip(hipError_t e, const char *what);
 // there (the attribute call costs microseconds of host time per launch).
 int ensure_dynamic_lds(const void *kernel, int bytes);
 
+// What a preparing call (interval plan, coefficient tiles) left in a caller's
+// workspace: it depends on the plan, the precision, the samples per tile (the
+// variant) and the sizes.  prep_stage records it when the call prepares and,
+// for a call that only sums, refuses (FEW_ERR_ARG) a workspace prepared
+// differently: the sum would run with a stale plan.
+struct PrepKey {
+    const void *ws;
+    const void *plan;
+    int precision, tile, L, out_len;
+};
+int prep_stage(bool prepares, bool sums, const PrepKey &k, const char *what);
+
 #define FEW_LDS(kernel, bytes)                                                  \
     do {                                                                        \
         int rc_ = ::few::ensure_dynamic_lds((const void *)(kernel), (int)(bytes)); \

@@ -15,6 +15,8 @@ be captured in a HIP graph.
 
 from __future__ import annotations
 
+import sys
+
 import numpy as np
 import torch
 
@@ -35,7 +37,8 @@ class ModePlan:
         self.k = np.ascontiguousarray(k, np.int32)
         self.n = np.ascontiguousarray(n, np.int32)
         self.K = len(self.m)
-        self.handle = lib().few_mode_plan_create(self.m.ctypes.data, self.k.ctypes.data,
+        self._lib = lib()  # held for __del__, which may run after module teardown
+        self.handle = self._lib.few_mode_plan_create(self.m.ctypes.data, self.k.ctypes.data,
                                                  self.n.ctypes.data, self.K)
         if not self.handle:
             check(1, "mode_plan_create")
@@ -49,11 +52,12 @@ class ModePlan:
     def workspace_bytes(self, L, N, precision) -> int:
         return int(lib().few_plan_workspace_bytes(self.handle, L, N, PRECISION_CODES[precision]))
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
+        # at interpreter exit the process releases the device memory itself
         h = getattr(self, "handle", None)
-        if h:
-            lib().few_mode_plan_destroy(h)
-            self.handle = None
+        if h and not _finalizing():
+            self._lib.few_mode_plan_destroy(h)
+        self.handle = None
 
 
 def _stream() -> int:
@@ -143,9 +147,10 @@ class BatchProblem:
         m = np.ascontiguousarray(np.concatenate([x.m for x in inputs]), np.int32)
         k = np.ascontiguousarray(np.concatenate([x.k for x in inputs]), np.int32)
         n = np.ascontiguousarray(np.concatenate([x.n for x in inputs]), np.int32)
-        self.handle = lib().few_batch_create(B, L.ctypes.data, K.ctypes.data, N.ctypes.data,
-                                             dt.ctypes.data, m.ctypes.data, k.ctypes.data,
-                                             n.ctypes.data)
+        self._lib = lib()
+        self.handle = self._lib.few_batch_create(B, L.ctypes.data, K.ctypes.data, N.ctypes.data,
+                                                 dt.ctypes.data, m.ctypes.data, k.ctypes.data,
+                                                 n.ctypes.data)
         if not self.handle:
             check(1, "batch_create")
         totals = np.zeros(10, np.int64)
@@ -203,8 +208,8 @@ class BatchProblem:
             self.run(precision)
         return g
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         h = getattr(self, "handle", None)
-        if h:
-            lib().few_batch_destroy(h)
-            self.handle = None
+        if h and not _finalizing():
+            self._lib.few_batch_destroy(h)
+        self.handle = None

@@ -100,3 +100,24 @@ def test_single_autotune(gpu, tmp_path, monkeypatch):
     h = p.run(best["precision"]).cpu().numpy()
     ref = oracle_waveform(inp)
     assert oracle.rel_l2(h, ref) < 1e-6
+
+
+def test_batch_sum_must_match_plan(gpu):
+    """The batch's interval plan depends on the precision and the variant's tile
+    shape: a sum-only call with another variant or precision is refused."""
+    import torch
+    from fastemriwaveforms_amd._lib import FewRocmError
+    from fastemriwaveforms_amd.pipeline import (BATCH_STAGE_PLAN, BATCH_STAGE_SPLINES,
+                                                BATCH_STAGE_SUM, BatchProblem)
+    bp = BatchProblem(_small_batch(seed=9, count=3))
+    bp.variant = 0
+    ref = bp.run("mixed16").clone()
+    bp.variant = 4
+    with pytest.raises(FewRocmError, match="prepared"):
+        bp.stages("mixed16", BATCH_STAGE_SUM)
+    with pytest.raises(FewRocmError, match="prepared"):
+        bp.stages("fp64", BATCH_STAGE_SUM)
+    bp.wave.zero_()
+    bp.stages("mixed16", BATCH_STAGE_SPLINES | BATCH_STAGE_PLAN)
+    bp.stages("mixed16", BATCH_STAGE_SUM)
+    assert torch.equal(bp.wave, ref)

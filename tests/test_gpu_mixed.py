@@ -174,5 +174,5 @@ def test_stage2_must_match_stage1(gpu):
     # another problem's plan on this workspace
     q = ModeSumProblem(inp)
     q.ws = p.ws
-    with pytest.raises(FewRocmError, match="another mode plan"):
+    with pytest.raises(FewRocmError, match="another plan"):
         q.mode_sum("mixed16", stage=2, overwrite=True)
