@@ -466,8 +466,12 @@ __global__ __launch_bounds__(FD_SCAN_BLOCK) void fd_scan3_kernel(const long *loc
 
 // Persistent walk over the flattened bins: workgroup b owns a contiguous
 // range; each thread strides through it, reloading the item only when its
-// next bin belongs to another (mode, segment).
-__global__ __launch_bounds__(FD_THREADS) void fd_sum_kernel(FdArgs a, long nitems)
+// next bin belongs to another (mode, segment).  Capped at 256 registers, two
+// waves per SIMD: the per-bin chains of FP64 libm calls are latency-bound,
+// and a second wave hides more of that than the few spills it costs
+// (measured on full1-sized FD sums: 133 ms against 164 ms at one wave).
+__global__ __launch_bounds__(FD_THREADS) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void fd_sum_kernel(FdArgs a, long nitems)
 {
     const long total = a.prefix[nitems];
     const long per = (total + gridDim.x - 1) / gridDim.x;
