@@ -412,7 +412,12 @@ __global__ __launch_bounds__(64) void aak_plan_kernel(AakArgs a)
 
 #define AAK_THREADS 128
 
-__global__ __launch_bounds__(AAK_THREADS) void aak_sum_kernel(AakArgs a)
+// Three waves per SIMD (at most 168 registers, a few spills): the per-sample
+// Bessel recurrences and sincos chains are FP64-latency-bound, and the third
+// wave hides more of that than the spills cost (full year at 10 s, 30
+// harmonics: 7.1 ms against 8.2 ms at two waves; four waves, 7.5 ms).
+__global__ __launch_bounds__(AAK_THREADS) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void aak_sum_kernel(AakArgs a)
 {
     const int npts = a.plan[a.L];
     const int nseg = npts - 1;
