@@ -119,6 +119,20 @@ class ModeSumProblem:
         self.mode_sum(precision, overwrite=True)
         return self.wave
 
+    def capture(self, precision: str = "mixed16") -> "torch.cuda.CUDAGraph":
+        """Capture :meth:`run` in a HIP graph (warm-up run first, on a side stream):
+        a replay is one step with no host work per launch."""
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self.run(precision)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run(precision)
+        return g
+
 
 BATCH_STAGE_SPLINES = 1
 BATCH_STAGE_PLAN = 2

@@ -176,3 +176,22 @@ def test_stage2_must_match_stage1(gpu):
     q.ws = p.ws
     with pytest.raises(FewRocmError, match="another plan"):
         q.mode_sum("mixed16", stage=2, overwrite=True)
+
+
+def test_single_step_graph_replays(gpu):
+    """The captured step (spline build, plan + tiles, sum writing every sample)
+    replayed several times, with eager work and synchronisations in between:
+    every replay equals the eager step bit for bit."""
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.pipeline import ModeSumProblem
+    inp = synthetic.small_input(seed=47, L=30, K=200, T_years=0.01)
+    p = ModeSumProblem(inp)
+    ref = p.run("mixed16").clone()
+    g = p.capture("mixed16")
+    for rep in range(4):
+        p.wave.fill_(float(rep + 1))
+        p.mode_sum("fp64", stage=3)           # eager work between the replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(p.wave, ref), f"replay {rep}"
