@@ -670,8 +670,8 @@ def main():
                          "kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: there is no multi-GPU path in this build (the sum "
-              "shards by time with no exchange; see DESIGN.md). Run with --gpus 1.",
+        print("bench.py measures one GPU: there is no multi-GPU path (the sum would shard "
+              "by output time ranges with no exchange). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 

@@ -328,7 +328,7 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
         return launch_fact_sum(fa, ph, (int)max_blocks, variant, stream);
     }
     if (precision != FEW_PRECISION_FP64) {
-        set_error("get_waveform: precision %d is not available in this build", precision);
+        set_error("get_waveform: precision %d is not supported", precision);
         return FEW_ERR_UNSUPPORTED;
     }
     const int tile = direct_tile(precision);
