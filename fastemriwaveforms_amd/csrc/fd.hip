@@ -563,7 +563,11 @@ int launch_fd_sum(FdArgs a, void *workspace, size_t workspace_bytes, hipStream_t
     int dev = 0, ncu = 256;
     FEW_HIP(hipGetDevice(&dev));
     FEW_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    hipLaunchKernelGGL(fd_sum_kernel, dim3((unsigned)(ncu * 8)), dim3(FD_THREADS), 0, stream, a,
+    // 64 workgroups per CU, each a contiguous bin range: short ranges balance
+    // the uneven per-bin cost (roots in or out of the segment, the SPA branch)
+    // better than a few long ones (full1-sized FD sum: 124 ms at 64 per CU,
+    // 133 ms at 8, 138 ms at 2)
+    hipLaunchKernelGGL(fd_sum_kernel, dim3((unsigned)(ncu * 64)), dim3(FD_THREADS), 0, stream, a,
                        nitems);
     FEW_HIP(hipGetLastError());
     return FEW_OK;
