@@ -293,6 +293,17 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
                 static_cast<double2 *>(wave)};
     const int tile = precision == FEW_PRECISION_FP64 ? direct_tile(FEW_PRECISION_FP64)
                                                      : fact16_tile(variant);
+    if (precision == FEW_PRECISION_MIXED16 && (stage & 1) && !(stage & 2)) {
+        // the spline scratch shares the region of the coefficient tiles: what
+        // an earlier stage 2 prepared is gone, so a sum needs stage 2 again
+        if (stage & 4) {
+            set_error("batch_run: stage 1 overwrites the mixed16 coefficient tiles; run stage 2 "
+                      "before the sum");
+            return FEW_ERR_ARG;
+        }
+        const PrepKey spent{workspace, batch, -1, 0, 0, 0};
+        (void)prep_stage(true, false, spent, "batch_run");
+    }
     const PrepKey key{workspace, batch, precision, tile, 0, 0};
     int rc = prep_stage(stage & 2, stage & 4, key, "batch_run");
     if (rc) return rc;

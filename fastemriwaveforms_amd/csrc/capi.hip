@@ -237,7 +237,11 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
                         size_t ws_bytes, hipStream_t stream, const FactPlan *plan = nullptr,
                         int variant = 0, int stage = 3)
 {
-    if (out_len == 0 || K == 0) return FEW_OK;
+    if (out_len == 0) return FEW_OK;
+    if (K == 0) {
+        // no modes: h = 0, which the overwriting sum must still write
+        return ((stage & 2) && (stage & 4)) ? launch_zero_complex(wave, out_len, stream) : FEW_OK;
+    }
     const PlanLayout pl = plan_layout(L);
     if (ws_bytes < pl.total) {
         set_error("get_waveform: workspace of %zu bytes, need %zu", ws_bytes, pl.total);

@@ -50,15 +50,8 @@ __device__ void plan_segments_block(const PlanArgs &a)
         int nt = 0;
         if (j < nseg) {
             Segment sg;
-            int st = 0, en = 0;
-            if (j < used - 1) {
-                st = plan_start(a.h_t, j, T, a.dt, a.out_len);
-                en = plan_start(a.h_t, j + 1, T, a.dt, a.out_len);
-                if (j + 1 == used - 1 && used < L) en = a.out_len;
-                st = st < 0 ? 0 : (st > a.out_len ? a.out_len : st);
-                en = en < 0 ? 0 : (en > a.out_len ? a.out_len : en);
-                if (en < st) en = st;
-            }
+            int st, en;
+            plan_interval(a.h_t, j, used, L, T, a.dt, a.out_len, &st, &en);
             sg.start = st;
             sg.end = en;
             sg.h_t = a.h_t[j];

@@ -48,6 +48,28 @@ __host__ __device__ inline int plan_start(const double *h_t, int j, double T, do
     return (t < T) ? (int)ceil(t / dt) : out_len;
 }
 
+// Sample range [*st, *en) of interval j, given `used` (the find_start_inds
+// stop: the first i >= 1 with h_t[i] >= T gives used = i + 1, else L).  The
+// last used interval runs to out_len when the rule stopped early
+// (interpolate.cu:711-715).  Intervals at or after used - 1 own no samples
+// and are parked at out_len, so the covered samples are [0, en) of the last
+// interval in every case.
+__host__ __device__ inline void plan_interval(const double *h_t, int j, int used, int L, double T,
+                                              double dt, int out_len, int *st, int *en)
+{
+    int s = out_len, e = out_len;
+    if (j < used - 1) {
+        s = plan_start(h_t, j, T, dt, out_len);
+        e = plan_start(h_t, j + 1, T, dt, out_len);
+        if (j + 1 == used - 1 && used < L) e = out_len;
+        s = s < 0 ? 0 : (s > out_len ? out_len : s);
+        e = e < 0 ? 0 : (e > out_len ? out_len : e);
+        if (e < s) e = s;
+    }
+    *st = s;
+    *en = e;
+}
+
 struct SampleState {
     double x, x2, x3;     // x = t - h_t[j] and its powers (amplitude cubic)
     double Pp, Pt, Pr;    // Phi_phi, Phi_theta, Phi_r at t

@@ -103,19 +103,7 @@ int twin_plan(const double *h_t, int L, double dt, int out_len, int *starts, int
             used = i + 1;
             break;
         }
-    for (int j = 0; j < L - 1; j++) {
-        int st = 0, en = 0;
-        if (j < used - 1) {
-            st = plan_start(h_t, j, T, dt, out_len);
-            en = plan_start(h_t, j + 1, T, dt, out_len);
-            if (j + 1 == used - 1 && used < L) en = out_len;
-            st = st < 0 ? 0 : (st > out_len ? out_len : st);
-            en = en < 0 ? 0 : (en > out_len ? out_len : en);
-            if (en < st) en = st;
-        }
-        starts[j] = st;
-        ends[j] = en;
-    }
+    for (int j = 0; j < L - 1; j++) plan_interval(h_t, j, used, L, T, dt, out_len, starts + j, ends + j);
     return used;
 }
 

@@ -127,7 +127,10 @@ def test_twin_plan_matches_find_start_inds(tw, out_delta):
     assert used == rused
     for j in range(rused - 1):
         assert (st[j], en[j]) == (ref[j], ref[j + 1])
-    assert np.all(st[rused - 1:inp.L - 1] == en[rused - 1:inp.L - 1])  # no samples after the stop
+    # intervals after the stop own no samples and sit at out_len, so the last
+    # interval's end is where the covered samples end
+    assert np.all(st[rused - 1:inp.L - 1] == N) and np.all(en[rused - 1:inp.L - 1] == N)
+    assert en[inp.L - 2] == (N if rused < inp.L else ref[inp.L - 1])
 
 
 def test_twin_direct_fp64_vs_oracle(tw):
