@@ -15,7 +15,6 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libfew_rocm.so")
 
 FEW_PRECISION_FP64 = 0
-FEW_PRECISION_FP64_FACT = 1
 FEW_PRECISION_MIXED = 2
 FEW_PRECISION_MIXED16 = 3
 
