@@ -263,7 +263,7 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
                         void *wave, int precision, int variant, int stage, void *workspace,
                         size_t workspace_bytes, void *stream)
 {
-    if (!batch || !t || !teuk || !phase_knots || !ylm || !interp || !pcoef || !wave || !workspace) {
+    if (!batch || !t || !teuk || !ylm || !interp || !pcoef || !wave || !workspace) {
         set_error("batch_run: null argument");
         return FEW_ERR_ARG;
     }

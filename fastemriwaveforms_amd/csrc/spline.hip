@@ -617,7 +617,8 @@ void fill_spline_args(const SplineGroup *groups, int ngroups, int length, double
 }
 
 // Batched build: every system of every waveform (2K amplitude arrays, then
-// the 3 phase arrays in dense-output form), one lane per system.
+// the 3 phase arrays in dense-output form unless the phases come as dense
+// output already, p.pk == nullptr), one lane per system.
 __global__ __launch_bounds__(64) void batch_spline_kernel(BatchDev B, BatchPtrs p)
 {
     const long sg = (long)blockIdx.x * 64 + threadIdx.x;
@@ -638,6 +639,7 @@ __global__ __launch_bounds__(64) void batch_spline_kernel(BatchDev B, BatchPtrs 
         G = SplineGroup{teuk + 1, 2, 2 * K, t, 0, 1, interp + K, L * 2 * K, 1, 2 * K, (int)K, 0};
         s = (int)(ls - K);
     } else {
+        if (!p.pk) return;  // dense-output phases handed over in pcoef: nothing to build
         G = SplineGroup{p.pk + d.pk_off, L, 1, t, 0, 1, p.pcoef + d.pc_off, 0, 0, 0, 3, 1};
         s = (int)(ls - 2 * K);
     }

@@ -81,6 +81,11 @@ class ModeSumProblem:
         L, K = self.L, self.K
         self.interp = torch.empty((4, L, 2 * K), dtype=f64, device=dev)
         self.pcoef = torch.empty((8, L - 1, 3), dtype=f64, device=dev)
+        # phases handed over as dense output (the trajectory's DOPR853 coefficients) are an
+        # input: only the amplitudes are splined per step
+        self.dense_input = inp.dense is not None
+        if self.dense_input:
+            self.pcoef.copy_(torch.from_numpy(np.ascontiguousarray(inp.dense, np.float64)))
         self.scratch = torch.empty(4 * L * (2 * K + 3), dtype=f64, device=dev)
         self.wave = torch.zeros(self.N, dtype=torch.complex128, device=dev)
         self.plan = ModePlan(inp.m, inp.k, inp.n)
@@ -93,10 +98,11 @@ class ModeSumProblem:
         return self.K * self.N
 
     def build_splines(self) -> None:
+        pk = None if self.dense_input else self.phase_knots.data_ptr()
+        pc = None if self.dense_input else self.pcoef.data_ptr()
         rc = lib().few_prepare_modes_async(
-            torch.view_as_real(self.teuk).data_ptr(), self.phase_knots.data_ptr(),
-            self.t.data_ptr(), self.L, self.K, self.interp.data_ptr(), self.pcoef.data_ptr(),
-            self.scratch.data_ptr(), _stream())
+            torch.view_as_real(self.teuk).data_ptr(), pk, self.t.data_ptr(), self.L, self.K,
+            self.interp.data_ptr(), pc, self.scratch.data_ptr(), _stream())
         check(rc, "prepare_modes")
 
     def mode_sum(self, precision: str = "fp64", stage: int = 3, overwrite: bool = False) -> None:
@@ -175,17 +181,28 @@ class BatchProblem:
         self.supported16 = bool(totals[9] & 2)
         self.mode_samples = int(np.sum(K.astype(np.int64) * N))
         self.samples = int(np.sum(N))
+        # phases as dense output (every waveform) are an input in pcoef; else splined per step
+        dense = [x.dense is not None for x in inputs]
+        if any(dense) and not all(dense):
+            raise ValueError("a batch takes dense-output phases for every waveform or for none")
+        self.dense_input = all(dense) and B > 0
         # pack host buffers (doubles) once, then one copy each to HBM
         host = [np.zeros(int(totals[i]), np.float64) for i in range(4)]
+        pc_host = np.zeros(int(totals[5]), np.float64) if self.dense_input else None
         for b, x in enumerate(inputs):
             o = offsets[b]
             host[0][o[0]:o[0] + x.L] = x.t
             host[1][o[1]:o[1] + 2 * x.L * x.K] = np.ascontiguousarray(x.teuk).view(np.float64).ravel()
             host[2][o[2]:o[2] + 3 * x.L] = np.ascontiguousarray(x.phase_knots).ravel()
             host[3][o[3]:o[3] + 4 * x.K] = np.ascontiguousarray(x.ylms).view(np.float64)
+            if self.dense_input:
+                pc_host[o[5]:o[5] + 24 * (x.L - 1)] = np.ascontiguousarray(x.dense).ravel()
         self.t, self.teuk, self.pk, self.ylm = (torch.from_numpy(h).to(dev) for h in host)
         self.interp = torch.empty(int(totals[4]), dtype=torch.float64, device=dev)
-        self.pcoef = torch.empty(int(totals[5]), dtype=torch.float64, device=dev)
+        if self.dense_input:
+            self.pcoef = torch.from_numpy(pc_host).to(dev)
+        else:
+            self.pcoef = torch.empty(int(totals[5]), dtype=torch.float64, device=dev)
         self.wave = torch.zeros(int(totals[6]) // 2, dtype=torch.complex128, device=dev)
         nbytes = int(lib().few_batch_workspace_bytes(self.handle, PRECISION_CODES["mixed16"]))
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
@@ -197,7 +214,8 @@ class BatchProblem:
 
     def stages(self, precision: str = "mixed16", stage: int = 7) -> None:
         rc = lib().few_batch_run_async(
-            self.handle, self.t.data_ptr(), self.teuk.data_ptr(), self.pk.data_ptr(),
+            self.handle, self.t.data_ptr(), self.teuk.data_ptr(),
+            None if self.dense_input else self.pk.data_ptr(),
             self.ylm.data_ptr(), self.interp.data_ptr(), self.pcoef.data_ptr(),
             self.wave.data_ptr(), PRECISION_CODES[precision], self.variant, stage,
             self.ws.data_ptr(), self.ws.numel(), _stream())

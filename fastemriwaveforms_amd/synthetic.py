@@ -50,6 +50,10 @@ class ModeSumInput:
     num_pts: int
     name: str = ""
     freqs: np.ndarray | None = None  # (3, L) f_phi, f_theta, f_r [Hz] at the knots
+    # (8, L-1, 3) dense-output phase coefficients handed over as they are (the
+    # trajectory's DOPR853 output); None: the phases are cubic splines of
+    # phase_knots, converted to that layout with rcont5..8 = 0
+    dense: np.ndarray | None = None
 
     @property
     def L(self) -> int:
@@ -65,13 +69,19 @@ class ModeSumInput:
 
 
 def mode_set(kind: str):
-    """(l, m, n) arrays of a mode set."""
-    ls, ms, ns = [], [], []
+    """(l, m, n) arrays of a mode set (k = 0), or (l, m, k, n) for the k sets ("k1_20")."""
+    ls, ms, ks, ns = [], [], [], []
     if kind == "m_nonneg_30":      # SchwarzschildEccentric(lmax=10, nmax=30): 63 * 61 = 3843
         for l in range(2, 11):
             for m in range(0, l + 1):
                 for n in range(-30, 31):
                     ls.append(l); ms.append(m); ns.append(n)
+    elif kind == "k1_20":          # l = 2..4, m = 0..l, k = -1..1, n = -20..20: 12 * 3 * 41 = 1476
+        for l in range(2, 5):
+            for m in range(0, l + 1):
+                for kk in (-1, 0, 1):
+                    for n in range(-20, 21):
+                        ls.append(l); ms.append(m); ks.append(kk); ns.append(n)
     elif kind == "m_all_55":       # 117 (l, m) pairs with |m| <= l, times 111 n = 12,987
         for l in range(2, 11):
             for m in range(-l, l + 1):
@@ -79,7 +89,10 @@ def mode_set(kind: str):
                     ls.append(l); ms.append(m); ns.append(n)
     else:
         raise ValueError(kind)
-    return (np.asarray(ls, np.int32), np.asarray(ms, np.int32), np.asarray(ns, np.int32))
+    out = [np.asarray(ls, np.int32), np.asarray(ms, np.int32), np.asarray(ns, np.int32)]
+    if ks:
+        out.insert(2, np.asarray(ks, np.int32))
+    return tuple(out)
 
 
 def num_points(t_first: float, t_last: float, T_years: float, dt: float) -> int:
@@ -109,18 +122,90 @@ def freq_phi(t: np.ndarray, Ts: float, chirp: str = "smooth") -> np.ndarray:
     return 1e-3 + 2e-3 * (3 * tau ** 2 - 2 * tau ** 3)
 
 
+def inspiral_phase_terms(kappa: float, kappa_theta: float):
+    """Frequency terms (f [Hz], a) of a quasi-Newtonian inspiral for the three
+    phases: each term contributes f u^(a-1) to the frequency, u = 1 - tau/1.25,
+    tau = t/T (f_phi rises from 1e-3 to 1.83e-3 Hz as u^(-3/8), the leading-order
+    chirp law).  Phi_theta and Phi_r follow Phi_phi with ratios kappa_theta,
+    kappa plus slowly drifting terms, so none of the three is a polynomial."""
+    return ([(1e-3, 5.0 / 8.0)],
+            [(kappa_theta * 1e-3, 5.0 / 8.0), (2e-5, 7.0 / 8.0)],
+            [(kappa * 1e-3, 5.0 / 8.0), (-1e-5, 3.0 / 4.0)])
+
+
+INSPIRAL_TC = 1.25  # coalescence at 1.25 T: u = 1 - t / (1.25 T) stays >= 0.2
+
+
+def inspiral_phase(t: np.ndarray, Ts: float, terms) -> tuple[np.ndarray, np.ndarray]:
+    """Phase sum_terms 2 pi f Ts tc / a (1 - u^a) and its rate 2 pi sum f u^(a-1)."""
+    tc = INSPIRAL_TC
+    u = 1.0 - np.asarray(t, np.float64) / (tc * Ts)
+    ph = np.zeros_like(u)
+    rate = np.zeros_like(u)
+    for f, a in terms:
+        ph += 2.0 * np.pi * f * Ts * tc / a * (1.0 - u ** a)
+        rate += 2.0 * np.pi * f * u ** (a - 1.0)
+    return ph, rate
+
+
+def dense_output(t: np.ndarray, fn, nodes: int = 24) -> np.ndarray:
+    """DOPR853-style dense-output coefficients (8, L-1) of a smooth function on
+    the knots t, for get_waveform's nested form
+    P(s) = r1 + s (r2 + s1 (r3 + s (r4 + s1 (r5 + s (r6 + s1 (r7 + s r8)))))).
+
+    r1..r4 are DOP853's own (trajectory/dopr853.py:631-642): the Hermite cubic
+    of the values and slopes at both ends, rcont3 = h y'_0 - (y_1 - y_0),
+    rcont4 = (y_1 - y_0) - h y'_1 - rcont3.  DOP853 builds r5..r8 from its
+    extra stages; here they are the least-squares fit of the remainder
+    y - Hermite = s^2 s1^2 (r5 + s r6 + s s1 r7 + s^2 s1 r8) on Chebyshev
+    nodes, which has the same structure (no change of the end values and
+    slopes) and order.  fn(t) returns (y, y')."""
+    t = np.asarray(t, np.float64)
+    h = np.diff(t)
+    y, dy = fn(t)
+    out = np.zeros((8, len(h)))
+    out[0] = y[:-1]
+    out[1] = y[1:] - y[:-1]
+    out[2] = h * dy[:-1] - out[1]
+    out[3] = out[1] - h * dy[1:] - out[2]
+    k = np.arange(nodes)
+    sn = 0.5 - 0.5 * np.cos(np.pi * (k + 0.5) / nodes)          # (nodes,) in (0, 1)
+    s1 = 1.0 - sn
+    w = sn ** 2 * s1 ** 2
+    basis = np.stack([w, w * sn, w * sn * s1, w * sn ** 2 * s1], axis=1)   # (nodes, 4)
+    tt = t[:-1, None] + sn[None, :] * h[:, None]
+    yv, _ = fn(tt.ravel())
+    r = out[:, :, None]
+    herm = r[0] + sn * (r[1] + s1 * (r[2] + sn * r[3]))
+    resid = yv.reshape(tt.shape) - herm                                  # (L-1, nodes)
+    coef, *_ = np.linalg.lstsq(basis, resid.T, rcond=None)               # (4, L-1)
+    out[4:] = coef
+    return out
+
+
 def make_input(T_years: float, dt: float, L: int, mode_kind: str, keep, seed: int,
-               minus_m_ylm: bool = True, name: str = "", chirp: str = "smooth") -> ModeSumInput:
+               minus_m_ylm: bool = True, name: str = "", chirp: str = "smooth",
+               phases: str = "spline") -> ModeSumInput:
+    """phases="spline": Phi_phi from the chirp law ``chirp`` (a quartic), Phi_r =
+    kappa Phi_phi, Phi_theta = 0, all splined by the build (rcont5..8 = 0).
+    phases="dopr": the non-polynomial inspiral of :func:`inspiral_phase_terms`
+    on all three phases (Phi_theta != 0), handed over as DOPR853-style dense
+    output with non-zero rcont5..8 (``dense``)."""
     rng = np.random.default_rng(seed)
     Ts = T_years * YRSID_SI
     interior = np.sort(rng.uniform(0.0, 1.0, L - 2)) * Ts
     t = np.concatenate([[0.0], interior, [Ts]])
-    l_all, m_all, n_all = mode_set(mode_kind)
+    ms_all = mode_set(mode_kind)
+    if len(ms_all) == 4:
+        l_all, m_all, k_all, n_all = ms_all
+    else:
+        l_all, m_all, n_all = ms_all
+        k_all = np.zeros_like(m_all)
     if keep is not None and keep < len(l_all):
         sel = np.sort(rng.choice(len(l_all), size=keep, replace=False))
     else:
         sel = np.arange(len(l_all))
-    l, m, n = l_all[sel], m_all[sel], n_all[sel]
+    l, m, kk, n = l_all[sel], m_all[sel], k_all[sel], n_all[sel]
     K = len(l)
     tau = t / Ts
     base = (rng.normal(size=K) + 1j * rng.normal(size=K)) / np.sqrt(2.0)
@@ -133,16 +218,29 @@ def make_input(T_years: float, dt: float, L: int, mode_kind: str, keep, seed: in
     if not minus_m_ylm:
         ym = np.zeros(K, dtype=np.complex128)
     kappa = rng.uniform(0.8, 0.95)
-    Pphi = phase_phi(t, Ts, chirp)
-    phase_knots = np.stack([Pphi, np.zeros_like(Pphi), kappa * Pphi])
-    fphi = freq_phi(t, Ts, chirp)
-    freqs = np.stack([fphi, np.zeros_like(fphi), kappa * fphi])
+    dense = None
+    if phases == "dopr":
+        kappa_theta = rng.uniform(0.9, 0.99)
+        terms = inspiral_phase_terms(kappa, kappa_theta)
+        vals, rates = zip(*(inspiral_phase(t, Ts, tm) for tm in terms))
+        phase_knots = np.stack(vals)
+        freqs = np.stack(rates) / (2.0 * np.pi)
+        dense = np.stack([dense_output(t, lambda x, tm=tm: inspiral_phase(x, Ts, tm))
+                          for tm in terms], axis=-1)                     # (8, L-1, 3)
+    elif phases == "spline":
+        Pphi = phase_phi(t, Ts, chirp)
+        phase_knots = np.stack([Pphi, np.zeros_like(Pphi), kappa * Pphi])
+        fphi = freq_phi(t, Ts, chirp)
+        freqs = np.stack([fphi, np.zeros_like(fphi), kappa * fphi])
+    else:
+        raise ValueError(phases)
     return ModeSumInput(
         t=t, teuk=np.ascontiguousarray(teuk.astype(np.complex128)),
         ylms=np.concatenate([yp, ym]).astype(np.complex128),
         phase_knots=np.ascontiguousarray(phase_knots), l=l, m=m,
-        k=np.zeros(K, np.int32), n=n, dt=float(dt), T_years=float(T_years),
-        num_pts=num_points(t[0], t[-1], T_years, dt), name=name, freqs=freqs)
+        k=np.ascontiguousarray(kk, np.int32), n=n, dt=float(dt), T_years=float(T_years),
+        num_pts=num_points(t[0], t[-1], T_years, dt), name=name, freqs=freqs,
+        dense=None if dense is None else np.ascontiguousarray(dense))
 
 
 def make_workload(name: str, seed: int | None = None) -> ModeSumInput:

@@ -177,7 +177,10 @@ int few_get_waveform_plan_async(void *d_waveform, const double *interp_array,
  * reference arithmetic) or FEW_PRECISION_MIXED16; stage is a mask: 1 =
  * splines of every waveform, 2 = interval plan (+ coefficient tiles for
  * MIXED16), 4 = the mode sum.  Stage 1 overwrites the coefficient tiles, so
- * run 2 after it.  Workspace: few_batch_workspace_bytes(), caller-owned.
+ * run 2 after it.  phase_knots may be NULL: pcoef then holds the caller's
+ * dense-output phase coefficients (the trajectory's DOPR853 output,
+ * src/few/trajectory/inspiral.py:143-147), which stage 1 leaves as they are.
+ * Workspace: few_batch_workspace_bytes(), caller-owned.
  * No host synchronisation and no allocation: capturable in a hipGraph.
  */
 void *few_batch_create(int B, const int *h_L, const int *h_K, const int *h_N, const double *h_dt,
@@ -393,6 +396,7 @@ int few_aak_get_waveform(void *waveform, double *interp_array, double M_phys, do
 
 /* Frees the library-owned workspaces of the synchronous drop-in entries. */
 int few_release_workspace(void);
+
 
 #ifdef __cplusplus
 }

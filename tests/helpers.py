@@ -9,13 +9,16 @@ def oracle_coefficients(inp):
     """Spline coefficients in the layouts get_waveform reads, via the oracle.
 
     Amplitudes: y_all = [Re A; Im A] (interpolatedmodesum.py:326-335) splined
-    and transposed to (4, L, 2K) (:99-101).  Phases: knot values splined and
-    converted to the (8, L-1, 3) dense-output layout.
+    and transposed to (4, L, 2K) (:99-101).  Phases: the input's dense output
+    as it is, or knot values splined and converted to the (8, L-1, 3)
+    dense-output layout.
     """
     y_all = np.concatenate([inp.teuk.T.real, inp.teuk.T.imag])
     interp, _, info = oracle.interpolate_arrays(inp.t, y_all)
     assert info == 0
     interp_tl = np.ascontiguousarray(np.transpose(interp, (0, 2, 1)))
+    if inp.dense is not None:
+        return interp_tl, np.ascontiguousarray(inp.dense)
     pc, _, info = oracle.interpolate_arrays(inp.t, inp.phase_knots)
     assert info == 0
     pcoef = oracle.cubic_to_dense(pc, inp.t)
@@ -56,9 +59,12 @@ def oracle_fd_inputs(inp):
                             f_r[None]])
     interp, _, info = oracle.interpolate_arrays(inp.t, y_all)
     assert info == 0
-    pc, _, info = oracle.interpolate_arrays(inp.t, inp.phase_knots)
-    assert info == 0
-    dense = oracle.cubic_to_dense(pc, inp.t)           # (8, L-1, 3)
+    if inp.dense is not None:
+        dense = inp.dense
+    else:
+        pc, _, info = oracle.interpolate_arrays(inp.t, inp.phase_knots)
+        assert info == 0
+        dense = oracle.cubic_to_dense(pc, inp.t)       # (8, L-1, 3)
     return np.ascontiguousarray(interp), np.ascontiguousarray(np.transpose(dense, (0, 2, 1)))
 
 
@@ -77,24 +83,41 @@ def oracle_fd(inp, frequency, f_arr_given=False, include_minus_m=True, separate_
     return wave, (start, end, zero)
 
 
-def aak_input(T_years=0.05, L=40, seed=9, e0=0.4, e1=0.1, Y0=0.7, dt=10.0):
+def aak_input(T_years=0.05, L=40, seed=9, e0=0.4, e1=0.1, Y0=0.7, dt=10.0, dopr=False):
     """Synthetic sparse AAK trajectory: knot times and the (L-1, 6, 8) dense-output
     coefficients of (p, e, Y, Phi_phi, Phi_theta, Phi_r) the trajectory would hand
-    to AAKSummation (cubic splines of smooth knot values, converted to the
-    dense form), plus source parameters."""
+    to AAKSummation, plus source parameters.  dopr=False: cubic splines of smooth
+    knot values converted to the dense form (rcont5..8 = 0); dopr=True: DOPR853-style
+    dense output (non-zero rcont5..8) of non-polynomial p, e, Y and inspiral phases."""
     from fastemriwaveforms_amd import synthetic
     rng = np.random.default_rng(seed)
     Ts = T_years * synthetic.YRSID_SI
     t = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 1.0, L - 2)) * Ts, [Ts]])
-    tau = t / Ts
-    p = 12.0 - 4.0 * tau
-    e = e0 + (e1 - e0) * tau ** 2
-    Y = Y0 + 0.05 * tau
-    Pphi = synthetic.phase_phi(t, Ts, "rising")
-    knots = np.stack([p, e, Y, Pphi, 0.97 * Pphi, 0.83 * Pphi])
-    pc, _, info = oracle.interpolate_arrays(t, knots)
-    assert info == 0
-    dense = oracle.cubic_to_dense(pc, t)                      # (8, L-1, 6)
+    if dopr:
+        def orbit(q):
+            def fn(x):
+                tau = x / Ts
+                if q == 0:
+                    return 12.0 - 4.0 * tau - 0.2 * np.sin(2 * tau), (-4.0 - 0.4 * np.cos(2 * tau)) / Ts
+                if q == 1:
+                    return (e0 + (e1 - e0) * tau ** 2 + 0.01 * np.sin(3 * tau),
+                            (2 * (e1 - e0) * tau + 0.03 * np.cos(3 * tau)) / Ts)
+                return Y0 + 0.05 * tau + 0.01 * np.sin(tau), (0.05 + 0.01 * np.cos(tau)) / Ts
+            return fn
+        terms = synthetic.inspiral_phase_terms(0.83, 0.97)
+        fns = [orbit(0), orbit(1), orbit(2)] + [
+            (lambda x, tm=tm: synthetic.inspiral_phase(x, Ts, tm)) for tm in terms]
+        dense = np.stack([synthetic.dense_output(t, f) for f in fns], axis=-1)  # (8, L-1, 6)
+    else:
+        tau = t / Ts
+        p = 12.0 - 4.0 * tau
+        e = e0 + (e1 - e0) * tau ** 2
+        Y = Y0 + 0.05 * tau
+        Pphi = synthetic.phase_phi(t, Ts, "rising")
+        knots = np.stack([p, e, Y, Pphi, 0.97 * Pphi, 0.83 * Pphi])
+        pc, _, info = oracle.interpolate_arrays(t, knots)
+        assert info == 0
+        dense = oracle.cubic_to_dense(pc, t)                      # (8, L-1, 6)
     coeffs = np.ascontiguousarray(np.transpose(dense, (1, 2, 0)))  # (L-1, 6, 8)
     params = dict(M=1e6, S=0.5, mu=10.0, qS=0.9, phiS=1.3, qK=0.6, phiK=2.1, dist=1.0)
     n_out = synthetic.num_points(t[0], t[-1], T_years, dt)
@@ -106,3 +129,31 @@ def oracle_aak(t, coeffs, params, n_out, nmodes=20, mich=False, dt=10.0):
     dense = np.ascontiguousarray(np.transpose(coeffs, (2, 0, 1)))   # (8, L-1, 6)
     return oracle.aak_waveform(dense, P["M"], P["S"], P["mu"], P["qS"], P["phiS"], P["qK"],
                                P["phiK"], P["dist"], nmodes, mich, t, n_out, dt)
+
+
+def dense_basis_monomials():
+    """(8, 8) matrix B with P(s) = sum_c r_c sum_j B[c, j] s^j for the nested
+    dense-output form of interpolate.cu:642 / dopr853.py:794-803, built by
+    polynomial products of s and s1 = 1 - s (numpy.polynomial), independently
+    of the nested evaluation: the basis functions are 1, s, s s1, s^2 s1,
+    s^2 s1^2, s^3 s1^2, s^3 s1^3, s^4 s1^3."""
+    from numpy.polynomial import polynomial as Pn
+    s = np.array([0.0, 1.0])
+    s1 = np.array([1.0, -1.0])
+    powers = [(0, 0), (1, 0), (1, 1), (2, 1), (2, 2), (3, 2), (3, 3), (4, 3)]
+    B = np.zeros((8, 8))
+    for c, (a, b) in enumerate(powers):
+        poly = np.array([1.0])
+        for _ in range(a):
+            poly = Pn.polymul(poly, s)
+        for _ in range(b):
+            poly = Pn.polymul(poly, s1)
+        B[c, :len(poly)] = poly
+    return B
+
+
+def dense_eval_monomial(r, s):
+    """Dense-output polynomial at s from its monomial expansion: r (8, ...) -> P."""
+    from numpy.polynomial import polynomial as Pn
+    mono = np.tensordot(dense_basis_monomials().T, r, axes=1)    # (8, ...) monomial coefficients
+    return Pn.polyval(s, mono, tensor=False) if mono.ndim > 1 else Pn.polyval(s, mono)

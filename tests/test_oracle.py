@@ -12,7 +12,7 @@ from scipy.interpolate import CubicSpline
 from scipy.linalg import lapack
 
 import oracle
-from helpers import oracle_coefficients, oracle_waveform
+from helpers import dense_basis_monomials, dense_eval_monomial, oracle_coefficients, oracle_waveform
 
 
 def test_dgtsv_bitwise_vs_lapack():
@@ -150,6 +150,8 @@ def test_dense_output_identity():
 
 
 def _brute(inp):
+    """Dense per-sample sum with every phase from the monomial expansion of the
+    8-coefficient dense output (all eight terms)."""
     interp_tl, pcoef = oracle_coefficients(inp)
     N, K = inp.num_pts, inp.K
     tt = np.arange(N) * inp.dt
@@ -157,19 +159,84 @@ def _brute(inp):
     x = tt - inp.t[j]
     A = sum((interp_tl[p][j, :K] + 1j * interp_tl[p][j, K:]) * x[:, None] ** p for p in range(4))
     s = (tt - inp.t[j]) / (inp.t[j + 1] - inp.t[j])
-    ph = [pcoef[0, j, q] + s * (pcoef[1, j, q] + (1 - s) * (pcoef[2, j, q] + s * pcoef[3, j, q]))
-          for q in range(3)]
+    ph = [dense_eval_monomial(pcoef[:, j, q], s) for q in range(3)]
     phase = inp.m[None] * ph[0][:, None] + inp.k[None] * ph[1][:, None] + inp.n[None] * ph[2][:, None]
     P = A * np.exp(-1j * phase)
     return (P * inp.ylms[:K]).sum(1) + ((inp.m != 0) * np.conj(P) * inp.ylms[K:]).sum(1)
 
 
-def test_mode_sum_vs_brute_force():
-    """DirectModeSum pattern (directmodesum.py:91-126) on dense spline values."""
+@pytest.mark.parametrize("phases,modes", [("spline", "m_nonneg_30"), ("dopr", "m_nonneg_30"),
+                                          ("dopr", "k1_20")])
+def test_mode_sum_vs_brute_force(phases, modes):
+    """DirectModeSum pattern (directmodesum.py:91-126) on dense spline values; with
+    DOPR853-style phases (non-zero rcont5..8, Phi_theta != 0) and k != 0 modes."""
     from fastemriwaveforms_amd import synthetic
     for seed in (2, 9):
-        inp = synthetic.small_input(seed=seed, L=10, K=15, T_years=0.001)
+        inp = synthetic.make_input(0.001, 10.0, 10, modes, 15, seed, phases=phases)
+        if phases == "dopr":
+            assert np.all(np.abs(inp.dense[4:]) > 0) and np.any(inp.phase_knots[1] != 0)
+        if modes == "k1_20":
+            assert np.any(inp.k != 0)
         np.testing.assert_allclose(oracle_waveform(inp), _brute(inp), rtol=1e-10, atol=1e-13)
+
+
+def test_dense_output_form_pinned():
+    """The oracle's 8-coefficient phase evaluation (interpolate.cu:642) against the
+    monomial expansion of its basis (polynomial products, no nested form), with
+    random non-zero r1..r8 on all three phases; each phase reaches the sum
+    through a mode of its own (m, k or n = 1), so a dropped, swapped or
+    mis-indexed coefficient or phase shows as a phase error."""
+    rng = np.random.default_rng(12)
+    L, N, dt = 5, 100, 10.0
+    t = np.array([0.0, 137.0, 400.0, 655.0, 1000.0])
+    scale = np.array([30.0, 20.0, 5.0, 3.0, 2.0, 1.0, 0.5, 0.3])
+    dense = np.ascontiguousarray(rng.normal(size=(8, L - 1, 3)) * scale[:, None, None])
+    teuk = np.ones((L, 1), np.complex128)
+    interp, _, _ = oracle.interpolate_arrays(t, np.concatenate([teuk.T.real, teuk.T.imag]))
+    interp_tl = np.ascontiguousarray(np.transpose(interp, (0, 2, 1)))
+    tt = np.arange(N) * dt
+    j = np.searchsorted(t, tt, side="right") - 1
+    s = (tt - t[j]) / (t[j + 1] - t[j])
+    for q, (m, k, n) in enumerate([(1, 0, 0), (0, 1, 0), (0, 0, 1)]):
+        h = oracle.get_waveform(interp_tl, t, dense, [m], [k], [n], L, N, 1,
+                                np.array([1.0 + 0j, 0.0]), dt, t)
+        P = dense_eval_monomial(dense[:, j, q], s)
+        np.testing.assert_allclose(np.abs(h), 1.0, atol=1e-14)
+        np.testing.assert_allclose(np.angle(h * np.exp(1j * P)), 0.0, atol=1e-12)
+    # the expansion itself: the DOP853 end conditions (dopr853.py:631-642)
+    from numpy.polynomial import polynomial as Pn
+    r = rng.normal(size=8)
+    mono = dense_basis_monomials().T @ r
+    d1 = Pn.polyder(mono)
+    np.testing.assert_allclose([Pn.polyval(0.0, mono), Pn.polyval(1.0, mono)],
+                               [r[0], r[0] + r[1]], rtol=1e-14)
+    np.testing.assert_allclose([Pn.polyval(0.0, d1), Pn.polyval(1.0, d1)],
+                               [r[1] + r[2], r[1] - r[2] - r[3]], rtol=1e-13, atol=1e-14)
+
+
+def test_synthetic_dense_output():
+    """The DOPR853-style generator: the first four coefficients are the Hermite
+    cubic (end values and slopes exact), the eight together follow the
+    non-polynomial inspiral phase to 1e-11 of its value inside every interval
+    of a 120-knot year, and rcont5..8 are far from zero."""
+    from fastemriwaveforms_amd import synthetic
+    rng = np.random.default_rng(0)
+    Ts = synthetic.YRSID_SI
+    t = np.concatenate([[0.0], np.sort(rng.uniform(0, 1, 118)) * Ts, [Ts]])
+    h = np.diff(t)
+    for terms in synthetic.inspiral_phase_terms(0.87, 0.95):
+        fn = lambda x: synthetic.inspiral_phase(x, Ts, terms)  # noqa: E731
+        r = synthetic.dense_output(t, fn)
+        y, dy = fn(t)
+        np.testing.assert_allclose(r[0] + r[1], y[1:], rtol=1e-15)
+        np.testing.assert_allclose(r[1] + r[2], h * dy[:-1], rtol=1e-12)
+        np.testing.assert_allclose(r[1] - r[2] - r[3], h * dy[1:], rtol=1e-12)
+        assert np.median(np.abs(r[4])) > 1e-6 and np.median(np.abs(r[7])) > 1e-12
+        j = rng.integers(0, len(h), 400)
+        sv = rng.uniform(0, 1, 400)
+        P = dense_eval_monomial(r[:, j], sv)
+        exact, _ = fn(t[j] + sv * h[j])
+        np.testing.assert_allclose(P, exact, rtol=1e-11)
 
 
 def test_mode_sum_closed_form_geometric():
