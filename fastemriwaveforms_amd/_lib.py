@@ -18,6 +18,9 @@ FEW_PRECISION_FP64 = 0
 FEW_PRECISION_MIXED = 2
 FEW_PRECISION_MIXED16 = 3
 
+# size of the static device array behind few_get_waveform_plan_noalloc
+FEW_NOALLOC_SCRATCH_BYTES = 32 << 20
+
 _lib = None
 
 _d = ctypes.c_void_p  # every pointer travels as an untyped address
@@ -82,6 +85,8 @@ def _declare(lib: ctypes.CDLL) -> None:
                                    c_int),
         "few_aak_get_waveform": ([_d, _d] + [c_double] * 8 + [c_int, c_int, c_int, c_int,
                                                                c_double, _d, _d], c_int),
+        "few_get_waveform_plan_noalloc": ([_d, _d, _d, _d, _d, _d, _d, _d, c_int, c_int, c_int,
+                                           _d, c_double, _d, c_int, c_int, c_int, _d], c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -126,4 +131,5 @@ def exported_symbols() -> list[str]:
         "few_get_waveform_generic_fd_async", "few_get_waveform_generic_fd",
         "few_interp2d_workspace_bytes", "few_interp2d_async", "few_interp2D_wrap",
         "few_aak_workspace_bytes", "few_aak_waveform_async", "few_aak_get_waveform",
+        "few_get_waveform_plan_noalloc",
     ]

@@ -1221,3 +1221,70 @@ int few_release_workspace(void)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------- allocation-free mode sum
+
+// Workspace of few_get_waveform_plan_noalloc: one copy per GPU, part of the
+// code object, so it exists once the library's kernels are loaded on a device.
+// Host code never uses the name `scratch` as an address (on the host it is a
+// stand-in symbol): the device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[FEW_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[64];  // device address of `scratch` per device ordinal
+
+// Device address of this device's `scratch`, looked up on the first call on
+// the device (with it current) and kept.
+static int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    FEW_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        set_error("device index %d out of range", dev);
+        return FEW_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("get_waveform_plan_noalloc: hipGetSymbolAddress of the scratch array "
+                      "failed: %s", hipGetErrorString(e));
+            return FEW_ERR_HIP;
+        }
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return FEW_OK;
+}
+
+extern "C" {
+
+int few_get_waveform_plan_noalloc(void *d_waveform, const double *interp_array,
+                                  const double *d_phase_interp_t,
+                                  const double *phase_interp_coeffs, const void *plan,
+                                  const int *d_m, const int *d_k, const int *d_n, int init_len,
+                                  int out_len, int num_teuk_modes, const void *d_Ylms,
+                                  double delta_t, const double *d_h_t, int precision, int variant,
+                                  int stage, void *stream)
+{
+    int rc = check_mode_args(d_waveform, interp_array, phase_interp_coeffs, d_m, d_k, d_n, init_len,
+                             out_len, num_teuk_modes, d_Ylms, delta_t);
+    if (rc) return rc;
+    const size_t need = few_plan_workspace_bytes(plan, init_len, out_len, precision);
+    if (need > FEW_NOALLOC_SCRATCH_BYTES) {
+        set_error("get_waveform_plan_noalloc: needs %zu bytes of workspace, more than the %u of "
+                  "the library's scratch array (use few_get_waveform_plan_async with a caller "
+                  "workspace)", need, (unsigned)FEW_NOALLOC_SCRATCH_BYTES);
+        return FEW_ERR_LIMIT;
+    }
+    void *ws = nullptr;
+    rc = noalloc_scratch(&ws);
+    if (rc) return rc;
+    return few_get_waveform_plan_async(d_waveform, interp_array, d_phase_interp_t,
+                                       phase_interp_coeffs, plan, d_m, d_k, d_n, init_len, out_len,
+                                       num_teuk_modes, d_Ylms, delta_t, d_h_t, precision, variant,
+                                       stage, ws, FEW_NOALLOC_SCRATCH_BYTES, stream);
+}
+
+}  // extern "C"

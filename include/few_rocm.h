@@ -397,6 +397,30 @@ int few_aak_get_waveform(void *waveform, double *interp_array, double M_phys, do
 /* Frees the library-owned workspaces of the synchronous drop-in entries. */
 int few_release_workspace(void);
 
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): few_get_waveform_plan_async with the same arguments
+ * except the workspace, which comes from a library-owned static device
+ * array of FEW_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends).
+ * A call whose few_plan_workspace_bytes() exceeds that size is refused with
+ * FEW_ERR_LIMIT (FP64 needs ~36 B per knot; MIXED16 adds the coefficient
+ * tiles: 10.8 MB for 120 knots and the 3843-mode set, 31.5 MB for 200 knots
+ * and n in [-55, 55]).  The call allocates nothing: no hipMalloc / hipFree
+ * and no allocating library routine.  Every call on one GPU shares that
+ * GPU's array, which keeps the last call's contents: a stage-2 call sums
+ * with what the last stage-1 call on the same GPU prepared, and calls that
+ * overlap on one GPU (other streams or host threads) race.  The first call
+ * on a device looks the array's address up once (hipGetSymbolAddress).
+ */
+#define FEW_NOALLOC_SCRATCH_BYTES (32u << 20)
+int few_get_waveform_plan_noalloc(void *d_waveform, const double *interp_array,
+                                  const double *d_phase_interp_t,
+                                  const double *phase_interp_coeffs, const void *plan,
+                                  const int *d_m, const int *d_k, const int *d_n, int init_len,
+                                  int out_len, int num_teuk_modes, const void *d_Ylms,
+                                  double delta_t, const double *d_h_t, int precision, int variant,
+                                  int stage, void *stream);
 
 #ifdef __cplusplus
 }
