@@ -1,25 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the interpolated mode sum on one MI355X.
 
-Workload (default): full1 -- T = 1 yr, dt = 10 s (3,155,815 samples), 120
-knots, all 3843 modes (l = 2..10, m = 0..l, n = -30..30), seeded synthetic
-inputs resident in HBM.  One step = spline build (2K+3 systems) + interval
-planning / coefficient tiles + mode sum: the whole path from knot values to
-the complex128 waveform.  The sum writes every output sample (h where an
-interval covers it, 0 after the last one), so the step has no separate
-zeroing pass.  The timed steps replay the step captured once as a HIP graph
-(as a serving loop would); the per-kernel breakdown comes from eager steps
-with HIP events between the launches.
+Headline workload (default): the batched uneven workload -- 256 waveforms with
+durations U(0.1, 2) yr at dt = 10 s, knots U{50..200} and U{50..3000} modes of
+the 3843-mode set (l = 2..10, m = 0..l, n = -30..30), seeded synthetic inputs
+resident in HBM.  One step = every waveform's spline build (2K+3 systems each),
+interval plans and coefficient tiles, and the persistent mode sum over the
+whole (waveform, interval, tile) sequence, which writes every output sample:
+the whole path from knot values to 256 complex128 waveforms.  The timed steps
+replay the step captured once as a HIP graph (as a serving loop would); the
+per-stage breakdown comes from eager steps with HIP events between the
+launches, on the stream the kernels run on.
 
 The headline path is the mixed-precision factorised sum (FP64 phases and
 spline build; the n-contraction on FP16 MFMA with hi/lo split operands and
-FP32 accumulation, or FP32 MFMA when the mode set's tile does not fit LDS);
-the plain FP64 GPU sum (the reference's arithmetic) is timed beside it as the
-denominator of the speed-up.  Further legs: config 2 (2 yr at 5 s, 2000 modes
-with n to 55), the 256-waveform uneven batch, the differentiation (forward +
-backward), the frequency-domain sum, AAK, interp2D.  The oracle
-(oracle/, test infrastructure) runs only in the CPU leg: as the CPU baseline and
-as the checker of values the GPU legs kept.  Prints ONE JSON line.
+FP32 accumulation); the plain FP64 GPU sum (the reference's arithmetic) is
+timed beside it as the denominator of the speed-up.  Further legs: the single
+full-year waveform (config 1, all 3843 modes), config 2 (2 yr at 5 s, 2000
+modes with n to 55), the differentiation (forward + backward), the
+frequency-domain sum, AAK, interp2D.  The oracle (oracle/, test
+infrastructure) runs only in the CPU leg: as the CPU baseline and as the
+checker of values the GPU legs kept.  Prints ONE JSON line.
 """
 
 from __future__ import annotations
@@ -40,6 +41,10 @@ PEAK = {"fp64": 78.6, "fp32_mfma": 157.3, "fp16_mfma": 2500.0}  # MI355X TFLOP/s
 # (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a 1 GiB copy in the same
 # process): profiles/fact16_full1_hbm_traffic.md
 FACT16_FULL1_TRAFFIC = 2.00 * 16244.8 * 1024 + 1.00 * 49313.2 * 1024
+# HBM bytes per launch of batch_fact16_kernel on the 256-waveform batch, from two
+# rocprofv3 PMC runs (None until measured)
+BATCH_FACT16_TRAFFIC = None
+BATCH_TRAFFIC_SOURCE = None
 DIRECT_FLOP = 37  # algorithmic FLOP per mode-sample of the direct sum (sincos counted 0)
 
 
@@ -73,14 +78,13 @@ def cpu_baseline(inp, seconds_target: float = 12.0):
     t0 = time.perf_counter()
     oracle.waveform_samples(idx, *args, nthreads=1)
     el = time.perf_counter() - t0
-    # the same restatement with OpenMP over samples, all host threads available (<= 16)
-    nth = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
-                     else (os.cpu_count() or 1)))
+    # the same restatement with OpenMP over samples, one thread per physical core
+    nth, model = physical_cores()
     t0 = time.perf_counter()
     oracle.waveform_samples(idx, *args, nthreads=nth)
     el_omp = time.perf_counter() - t0
     return {"value": nsamp * inp.K / el, "unit": "mode-samples/s", "cores": 1, "kind": "port",
-            "omp": {"value": nsamp * inp.K / el_omp, "threads": nth},
+            "omp": {"value": nsamp * inp.K / el_omp, "threads": nth, "cpu_model": model},
             "sample": f"{nsamp} evenly spread output samples x {inp.K} modes of the same workload "
                       f"({nsamp * inp.K:.3g} mode-samples, {el:.1f} s); oracle/few_oracle.c "
                       "restating interpolate.cu's CPU branch (-O2 -ffp-contract=off), 1 thread"}
@@ -93,7 +97,7 @@ def cpu_checks(inp, head, batch_chk, fd_chk, aak_chk=None, amp_chk=None, mixed2_
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle
     from helpers import oracle_waveform
-    nth = min(16, os.cpu_count() or 1)
+    nth = physical_cores()[0]
     res = {}
     if head is not None:
         idx, h_idx = head
@@ -184,14 +188,17 @@ def time_path(prob, precision, steps, warmup):
     return el / steps * 1e3, ms
 
 
-def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256):
-    """Uneven batch (durations U(0.1, 2) yr, knots U{50..200}, modes U{50..3000} of the
-    3843 set, dt = 10 s): the whole batch in one launch per stage, replayed as one
-    captured HIP graph, variant from the autotuner; FP64 control run beside it."""
+def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256, sum_reps: int = 5):
+    """The headline: the uneven batch (durations U(0.1, 2) yr, knots U{50..200}, modes
+    U{50..3000} of the 3843 set, dt = 10 s), one launch per stage for the whole batch,
+    replayed as one captured HIP graph, variant from the autotuner; per-stage times
+    from eager launches with HIP events on the launching stream; the FP64 control
+    run beside it."""
     import torch
     from fastemriwaveforms_amd import autotune, synthetic
-    from fastemriwaveforms_amd.pipeline import (BATCH_STAGE_PLAN, BATCH_STAGE_SPLINES,
-                                                BATCH_STAGE_SUM, BatchProblem)
+    from fastemriwaveforms_amd.pipeline import (BATCH_STAGE_OVERWRITE, BATCH_STAGE_PLAN,
+                                                BATCH_STAGE_SPLINES, BATCH_STAGE_SUM,
+                                                BatchProblem)
     t0 = time.perf_counter()
     inputs = synthetic.make_batch(count)
     bp = BatchProblem(inputs, dev)
@@ -205,25 +212,30 @@ def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256):
         g.replay()
     torch.cuda.synchronize()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
     a.record()
     for _ in range(steps):
         g.replay()
     b.record()
     torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - w0) / steps * 1e3
     ms_graph = a.elapsed_time(b) / steps
-    # stage breakdown (eager, same launches)
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    bp.wave.zero_()
-    ev[0].record()
-    bp.stages("mixed16", BATCH_STAGE_SPLINES)
-    ev[1].record()
-    bp.stages("mixed16", BATCH_STAGE_PLAN)
-    ev[2].record()
-    bp.stages("mixed16", BATCH_STAGE_SUM)
-    ev[3].record()
-    torch.cuda.synchronize()
-    stage_ms = {"spline": ev[0].elapsed_time(ev[1]), "plan_tiles": ev[1].elapsed_time(ev[2]),
-                "sum": ev[2].elapsed_time(ev[3])}
+    # stage breakdown (eager, same launches, events on the launching stream)
+    stage_ms = {"spline": [], "plan_tiles": [], "sum": []}
+    for _ in range(sum_reps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        ev[0].record()
+        bp.stages("mixed16", BATCH_STAGE_SPLINES)
+        ev[1].record()
+        bp.stages("mixed16", BATCH_STAGE_PLAN)
+        ev[2].record()
+        bp.stages("mixed16", BATCH_STAGE_SUM | BATCH_STAGE_OVERWRITE)
+        ev[3].record()
+        torch.cuda.synchronize()
+        stage_ms["spline"].append(ev[0].elapsed_time(ev[1]))
+        stage_ms["plan_tiles"].append(ev[1].elapsed_time(ev[2]))
+        stage_ms["sum"].append(ev[2].elapsed_time(ev[3]))
+    stage_ms = {k: float(np.mean(v)) for k, v in stage_ms.items()}
     h16 = bp.wave.clone()
     # kept for the CPU leg's oracle check: 8 waveforms spread over the batch, spread samples
     chk = []
@@ -232,12 +244,14 @@ def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256):
         idx = spread_indices(inp.num_pts, count=300, seed=int(bi))
         chk.append((inp, idx, bp.waveform(int(bi)).cpu().numpy()[idx]))
     out = {"waveforms": count, "samples": bp.samples, "mode_samples": bp.mode_samples,
-           "intervals": int(bp.totals[8]), "variant": best["variant"], "graph_ms_per_step": ms_graph,
-           "waveforms_per_s": count / (ms_graph * 1e-3),
+           "intervals": int(bp.totals[8]), "variant": best["variant"], "ms_per_step": ms_graph,
+           "wall_ms_per_step": wall_ms, "waveforms_per_s": count / (ms_graph * 1e-3),
            "mode_samples_per_s": bp.mode_samples / (ms_graph * 1e-3), "stage_ms": stage_ms,
-           "rel_l2_vs_oracle_max_of_8": None}
-    log(f"[bench] batch mixed16 graph: {ms_graph:.2f} ms/step, {out['waveforms_per_s']:.1f} "
-        f"waveforms/s, {out['mode_samples_per_s']:.4g} mode-samples/s, stages {stage_ms}")
+           "rel_l2_vs_oracle_max_of_8": None, "plan_rows": int(bp.plan_rows),
+           "plan_cols": int(bp.plan_cols), "plan_row_blocks": int(bp.plan_row_blocks)}
+    log(f"[bench] batch mixed16 graph: {ms_graph:.2f} ms/step (wall {wall_ms:.2f}), "
+        f"{out['waveforms_per_s']:.1f} waveforms/s, {out['mode_samples_per_s']:.4g} "
+        f"mode-samples/s, stages {stage_ms}")
     if fp64:
         reps = 2
         bp.run("fp64")
@@ -257,7 +271,81 @@ def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256):
             f"mixed16 rel-L2 vs fp64 (all samples) {float(diff):.3e}")
     del bp, h16, g
     torch.cuda.empty_cache()
-    return out, chk
+    return out, chk, inputs
+
+
+def cpu_baseline_batch(inputs, seconds_target: float = 15.0, nwave: int = 8):
+    """The oracle, single thread (the reference CPU backend is single-threaded), on a
+    bounded sample of the batch: evenly spread output samples of 8 waveforms spread
+    over the batch, all of each waveform's modes; mode-samples/s over that sample.
+    The same sample with OpenMP over samples on every physical core is reported
+    beside it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    from helpers import oracle_coefficients
+    pick = [inputs[i] for i in np.linspace(0, len(inputs) - 1, nwave).astype(int)]
+    prepared = []
+    for inp in pick:
+        interp_tl, pcoef = oracle_coefficients(inp)
+        prepared.append((inp, (interp_tl, inp.t, pcoef, inp.m, inp.k, inp.n, inp.L, inp.num_pts,
+                               inp.K, inp.ylms, inp.dt, inp.t)))
+    # per-sample cost probe, then a per-waveform sample count for ~seconds_target in total
+    t0 = time.perf_counter()
+    ms_probe = 0
+    for inp, args in prepared:
+        probe = np.linspace(0, inp.num_pts - 1, 50).astype(np.int64)
+        oracle.waveform_samples(probe, *args, nthreads=1)
+        ms_probe += 50 * inp.K
+    per_ms = (time.perf_counter() - t0) / ms_probe
+    budget_ms = seconds_target / max(per_ms, 1e-12)
+    ksum = sum(inp.K for inp, _ in prepared)
+    nsamp = max(200, int(budget_ms / ksum))
+    idxs = [np.linspace(0, inp.num_pts - 1, min(nsamp, inp.num_pts)).astype(np.int64)
+            for inp, _ in prepared]
+    msamp = sum(len(ix) * inp.K for ix, (inp, _) in zip(idxs, prepared))
+    t0 = time.perf_counter()
+    for ix, (inp, args) in zip(idxs, prepared):
+        oracle.waveform_samples(ix, *args, nthreads=1)
+    el = time.perf_counter() - t0
+    cores, model = physical_cores()
+    t0 = time.perf_counter()
+    for ix, (inp, args) in zip(idxs, prepared):
+        oracle.waveform_samples(ix, *args, nthreads=cores)
+    el_omp = time.perf_counter() - t0
+    return {"value": msamp / el, "unit": "mode-samples/s", "cores": 1, "kind": "port",
+            "omp": {"value": msamp / el_omp, "threads": cores, "cpu_model": model},
+            "sample": f"{len(idxs[0])} evenly spread output samples of each of {nwave} waveforms "
+                      f"spread over the batch, all their modes ({msamp:.3g} mode-samples, "
+                      f"{el:.1f} s); oracle/few_oracle.c restating interpolate.cu's CPU branch "
+                      "(-O2 -ffp-contract=off), 1 thread"}
+
+
+def physical_cores():
+    """(physical cores available to this process, CPU model) from /proc/cpuinfo and
+    the affinity mask: OpenMP legs use one thread per physical core."""
+    avail = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else \
+        list(range(os.cpu_count() or 1))
+    model, core_of, cur = "unknown", {}, {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    if "processor" in cur:
+                        core_of[int(cur["processor"])] = (cur.get("physical id", "0"),
+                                                          cur.get("core id", cur["processor"]))
+                    cur = {}
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                cur[k] = v
+                if k == "model name":
+                    model = v
+        if "processor" in cur:
+            core_of[int(cur["processor"])] = (cur.get("physical id", "0"),
+                                              cur.get("core id", cur["processor"]))
+    except OSError:
+        pass
+    phys = {core_of.get(c, (0, c)) for c in avail}
+    return max(1, len(phys)), model
 
 
 def mixed2_leg(dev, steps: int = 4):
@@ -635,24 +723,94 @@ def ModeSumFunction_apply(eng, t_ri, p_k):
     return ModeSumFunction.apply(eng, t_ri, p_k)
 
 
+def fact16_roofline(kernel, rows, cols, row_blocks, samples, sum_ms, traffic=None,
+                    traffic_source=None):
+    """Roofline entry of a split-FP16 factorised sum launch: algorithmic FLOP = 8 FLOP x
+    4 powers x rows x cols per output sample (the complex n-contraction of the (m, n)
+    cell grid) x samples; against the published dense FP16 MFMA peak.  The issue rate
+    of the split products (3 per FP32-accurate product, padded to the MFMA tiles) is
+    reported beside it."""
+    flop_per_sample = 8 * 4 * rows * cols
+    achieved = flop_per_sample * samples / (sum_ms * 1e-3) / 1e12
+    kb = (2 * cols + 31) // 32
+    issued_per_sample = 3 * 2 * (16 * row_blocks) * (32 * kb)
+    issued = issued_per_sample * samples / (sum_ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp16_mfma"], "unit": "TFLOP/s",
+            "frac": achieved / PEAK["fp16_mfma"], "traffic": traffic, "kernel": kernel,
+            "algorithmic": f"8 FLOP x 4 powers x {rows} rows x {cols} cols = {flop_per_sample} "
+                           f"FLOP/sample x {samples} samples per launch",
+            "kernel_ms": sum_ms, "traffic_source": traffic_source,
+            "split_product_issue": {"tflops": issued, "frac_of_peak": issued / PEAK["fp16_mfma"],
+                                    "flop_per_sample": issued_per_sample,
+                                    "basis": "3 FP16 products (lo*hi, hi*lo, hi*hi) per "
+                                             "FP32-accurate product, rows and columns padded "
+                                             f"to {16 * row_blocks} x {32 * kb} real"}}
+
+
+def single_leg(dev, args):
+    """Config 1, one waveform: T = 1 yr, dt = 10 s (3,155,815 samples), 120 knots, all
+    3843 modes; the step captured as a HIP graph and replayed; the plain FP64 GPU
+    path beside it."""
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.pipeline import ModeSumProblem
+    inp = synthetic.make_workload("full1")
+    prob = ModeSumProblem(inp, dev)
+    K, N = inp.K, inp.num_pts
+    prec = "mixed16" if prob.plan.supported16 else "mixed"
+    ms_eager, ms = time_path(prob, prec, 50, args.single_warmup)
+    ms_step = time_graph(prob, prec, args.single_steps, args.single_warmup)
+    h = prob.wave.cpu().numpy()
+    value = K * N / (ms_step * 1e-3)
+    log(f"[bench] full1 {prec}: step {ms_step:.3f} ms (graph replay; eager {ms_eager:.3f}) | "
+        f"spline {ms['spline']:.3f} prepare {ms['prepare']:.3f} sum {ms['sum']:.3f} ms | "
+        f"{value:.4g} mode-samples/s")
+    idx = spread_indices(N, count=1500, seed=5)
+    out = {"workload": f"config 1: T={inp.T_years} yr, dt={inp.dt} s, N={N} samples, "
+                       f"L={inp.L} knots, K={K} modes",
+           "precision": prec, "ms_per_step": ms_step, "eager_ms_per_step": ms_eager,
+           "kernel_ms": ms, "mode_samples_per_s": value, "waveforms_per_s": 1e3 / ms_step,
+           "rel_l2_vs_oracle": None,
+           "roofline": fact16_roofline("fact16_kernel", prob.plan.rows, prob.plan.cols,
+                                       prob.plan.row_blocks, N, ms["sum"],
+                                       FACT16_FULL1_TRAFFIC,
+                                       "profiles/fact16_full1_hbm_traffic.md (rocprofv3 PMC)")}
+    chk = (idx, h[idx].copy())
+    if not args.no_fp64:
+        ms64_step, ms64 = time_path(prob, "fp64", 3, 1)
+        h64 = prob.wave.cpu().numpy()
+        fa, fb = np.fft.fft(h), np.fft.fft(h64)
+        overlap = np.vdot(fa, fb) / np.sqrt(np.vdot(fa, fa) * np.vdot(fb, fb))
+        out["fp64_plain_gpu"] = {
+            "ms_per_step": ms64_step, "mode_samples_per_s": K * N / (ms64_step * 1e-3),
+            "kernel_ms": ms64, "rel_l2_vs_oracle": None,
+            "mixed16_rel_l2_vs_fp64_all_samples": float(np.linalg.norm(h - h64) / np.linalg.norm(h64)),
+            "mixed16_mismatch_vs_fp64": float(1.0 - overlap.real),
+            "direct_fp64_roofline": {"pipe": "FP64 VALU", "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                                     "achieved": DIRECT_FLOP * K * N / (ms64["sum"] * 1e-3) / 1e12,
+                                     "algorithmic": f"{DIRECT_FLOP} FLOP/mode-sample, sincos "
+                                                    "counted 0"}}
+        out["speedup_vs_fp64_plain_gpu"] = ms64_step / ms_step
+        chk = (idx, h[idx].copy(), h64[idx].copy())
+        log(f"[bench] full1 fp64: step {ms64_step:.3f} ms, speed-up {ms64_step / ms_step:.2f}x, "
+            f"mixed16 rel-L2 vs fp64 {out['fp64_plain_gpu']['mixed16_rel_l2_vs_fp64_all_samples']:.3e}")
+    del prob
+    return out, chk, inp
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # the headline step is ~0.5 ms: 50 untimed steps bring the clocks to their
-    # loaded state (10 steps from idle read ~15% slow), 100 timed steps ~50 ms
-    ap.add_argument("--steps", type=int, default=100,
-                    help="timed headline steps (the other legs use fixed counts)")
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="full1")
-    ap.add_argument("--precision", default="auto",
-                    help="headline path: auto (mixed16 when the mode set fits, else mixed) | "
-                         "mixed16 | mixed | fp64")
-    ap.add_argument("--variant", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 leg")
-    ap.add_argument("--no-batch", action="store_true",
-                    help="skip the 256-waveform uneven batch leg")
+    # the headline step (the 256-waveform batch) is ~0.1 s: 5 untimed replays bring the
+    # clocks to their loaded state, 20 timed replays ~2 s
+    ap.add_argument("--steps", type=int, default=20, help="timed headline (batch) steps")
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--single-steps", type=int, default=100,
+                    help="timed steps of the single-waveform (config 1) leg")
+    ap.add_argument("--single-warmup", type=int, default=50)
+    ap.add_argument("--no-single", action="store_true", help="skip the config-1 leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 controls")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiation leg (forward + backward, FD check)")
     ap.add_argument("--no-fd", action="store_true",
@@ -676,18 +834,9 @@ def main():
         sys.exit(2)
 
     import torch
-    from fastemriwaveforms_amd import synthetic
-    from fastemriwaveforms_amd.pipeline import ModeSumProblem
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    log(f"[bench] workload {args.workload}")
-    inp = synthetic.make_workload(args.workload)
-    prob = ModeSumProblem(inp, dev)
-    prob.variant = args.variant
-    K, N = inp.K, inp.num_pts
-    log(f"[bench] K={K} N={N} L={inp.L} mode-samples={K * N:.4g} plan rows={prob.plan.rows} "
-        f"cols={prob.plan.cols} supported={prob.plan.supported}")
 
     if args.copy_mb > 0:
         nel = args.copy_mb * (1 << 20) // 8
@@ -699,69 +848,19 @@ def main():
         log(f"[bench] calibration copy: {args.copy_mb} MiB read + {args.copy_mb} MiB written")
         del src, dst
 
-    if args.precision == "auto":
-        args.precision = "mixed16" if prob.plan.supported16 else "mixed"
-    # per-kernel breakdown from eager steps (events between the launches), then the
-    # timed steps: the same step replayed as one captured HIP graph
-    ms_eager, ms = time_path(prob, args.precision, max(10, min(args.steps, 50)), args.warmup)
-    ms_step = time_graph(prob, args.precision, args.steps, max(args.warmup, 1))
-    h = prob.wave.cpu().numpy()
-    value = K * N / (ms_step * 1e-3)
-    log(f"[bench] {args.precision}: step {ms_step:.3f} ms (graph replay; eager "
-        f"{ms_eager:.3f}) | spline {ms['spline']:.3f} prepare {ms['prepare']:.3f} "
-        f"sum {ms['sum']:.3f} ms | {value:.4g} mode-samples/s")
+    batch, batch_chk, batch_inputs = batch_leg(dev, args.steps, args.warmup, not args.no_fp64,
+                                               args.batch_count)
+    roofline = fact16_roofline("batch_fact16_kernel", batch["plan_rows"], batch["plan_cols"],
+                               batch["plan_row_blocks"], batch["samples"],
+                               batch["stage_ms"]["sum"], BATCH_FACT16_TRAFFIC,
+                               BATCH_TRAFFIC_SOURCE)
+    log(f"[bench] batch sum roofline: {roofline['achieved']:.1f} TFLOP/s = "
+        f"{roofline['frac']:.3f} of the dense FP16 peak (split-product issue "
+        f"{roofline['split_product_issue']['frac_of_peak']:.3f})")
 
-    idx = spread_indices(N, count=1500, seed=5)
-    head_chk = (idx, h[idx].copy())
-
-    fp64 = None
-    fp64_chk = None
-    if not args.no_fp64 and args.precision != "fp64":
-        ms64_step, ms64 = time_path(prob, "fp64", 3, 1)
-        h64 = prob.wave.cpu().numpy()
-        fa, fb = np.fft.fft(h), np.fft.fft(h64)
-        overlap = np.vdot(fa, fb) / np.sqrt(np.vdot(fa, fa) * np.vdot(fb, fb))
-        fp64 = {"ms_per_step": ms64_step, "mode_samples_per_s": K * N / (ms64_step * 1e-3),
-                "kernel_ms": ms64, "rel_l2_vs_oracle": None,
-                "headline_rel_l2_vs_fp64_all_samples":
-                    float(np.linalg.norm(h - h64) / np.linalg.norm(h64)),
-                "headline_mismatch_vs_fp64": float(1.0 - overlap.real)}
-        fp64_chk = h64[idx].copy()
-        log(f"[bench] fp64: step {ms64_step:.3f} ms, speed-up {ms64_step / ms_step:.2f}x, "
-            f"headline rel-L2 vs fp64 {fp64['headline_rel_l2_vs_fp64_all_samples']:.3e}")
-
-    if args.precision in ("mixed", "mixed16") and prob.plan.supported:
-        flop_per_sample = 8 * 4 * prob.plan.rows * prob.plan.cols
-        flops = flop_per_sample * N
-        achieved = flops / (ms["sum"] * 1e-3) / 1e12
-        alg = (f"8 FLOP x 4 powers x {prob.plan.rows} rows x {prob.plan.cols} cols = "
-               f"{flop_per_sample} FLOP/sample x {N} samples per launch")
-        if args.precision == "mixed":
-            roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp32_mfma"],
-                        "unit": "TFLOP/s", "frac": achieved / PEAK["fp32_mfma"], "traffic": None,
-                        "kernel": "fact_f32_kernel", "algorithmic": alg}
-        else:
-            peak = PEAK["fp16_mfma"] / 3.0  # three FP16 products per FP32-accurate product
-            traffic = FACT16_FULL1_TRAFFIC if args.workload == "full1" else None
-            roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                        "frac": achieved / peak, "traffic": traffic, "kernel": "fact16_kernel",
-                        "traffic_source": "profiles/fact16_full1_hbm_traffic.md (rocprofv3 PMC)"
-                        if traffic else None,
-                        "algorithmic": alg,
-                        "peak_basis": "FP16 dense MFMA 2500 TFLOP/s / 3 split products",
-                        "frac_of_fp32_mfma_peak": achieved / PEAK["fp32_mfma"]}
-    else:
-        flops = DIRECT_FLOP * K * N
-        achieved = flops / (ms["sum"] * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp64"], "unit": "TFLOP/s",
-                    "frac": achieved / PEAK["fp64"], "traffic": None, "kernel": "direct_fp64_kernel",
-                    "pipe": "FP64 VALU", "algorithmic": f"{DIRECT_FLOP} FLOP/mode-sample x {K * N} "
-                                                        "mode-samples, sincos counted 0"}
-
-    batch = batch_chk = None
-    if not args.no_batch:
-        batch, batch_chk = batch_leg(dev, 5, 1, not args.no_fp64,
-                                     args.batch_count)
+    single = single_chk = inp1 = None
+    if not args.no_single:
+        single, single_chk, inp1 = single_leg(dev, args)
 
     mixed2 = mixed2_chk = None
     if not args.no_mixed2:
@@ -781,21 +880,23 @@ def main():
         adjoint = adjoint_leg(dev, batch=args.grad_batch, fd_coords=args.fd_coords)
 
     cpu = None
-    err = None
     if not args.no_cpu:
         log("[bench] CPU leg: oracle checks and baseline (1 thread)")
-        chk = cpu_checks(inp, head_chk, batch_chk, fd_chk, aak_chk, amp_chk, mixed2_chk)
-        err = chk["headline"]
-        log(f"[bench] rel-L2 vs oracle ({len(idx)} samples) {err:.3e}")
-        if fp64 is not None:
-            ref = chk["headline_ref"]
-            fp64["rel_l2_vs_oracle"] = float(np.linalg.norm(fp64_chk - ref) / np.linalg.norm(ref))
+        head = (single_chk[0], single_chk[1]) if single_chk is not None else None
+        chk = cpu_checks(inp1, head, batch_chk, fd_chk, aak_chk, amp_chk, mixed2_chk)
+        batch["rel_l2_vs_oracle_max_of_8"] = chk["batch_max_of_8"]
+        log(f"[bench] batch rel-L2 (8 waveforms vs oracle) max {chk['batch_max_of_8']:.3e}")
+        if single is not None:
+            single["rel_l2_vs_oracle"] = chk["headline"]
+            log(f"[bench] full1 rel-L2 vs oracle ({len(single_chk[0])} samples) "
+                f"{chk['headline']:.3e}")
+            if len(single_chk) > 2 and "fp64_plain_gpu" in single:
+                ref = chk["headline_ref"]
+                single["fp64_plain_gpu"]["rel_l2_vs_oracle"] = float(
+                    np.linalg.norm(single_chk[2] - ref) / np.linalg.norm(ref))
         if mixed2 is not None:
             mixed2["rel_l2_vs_oracle_600_samples"] = chk["mixed2"]
             log(f"[bench] mixed2 rel-L2 vs oracle (600 samples) {chk['mixed2']:.3e}")
-        if batch is not None:
-            batch["rel_l2_vs_oracle_max_of_8"] = chk["batch_max_of_8"]
-            log(f"[bench] batch rel-L2 (8 waveforms vs oracle) max {chk['batch_max_of_8']:.3e}")
         if fd is not None:
             fd["rel_l2_vs_oracle"] = chk["fd"]
             fd["cpu_baseline"] = chk["fd_cpu"]
@@ -812,35 +913,43 @@ def main():
             log(f"[bench] aak rel-L2 vs oracle {chk['aak']:.3e}, CPU {chk['aak_cpu']['value']:.4g} "
                 f"samples/s; interp2D bitwise {chk['interp2d']}, CPU "
                 f"{chk['interp2d_cpu']['value']:.4g} evaluations/s")
-        cpu = cpu_baseline(inp)
-        log(f"[bench] CPU {cpu['value']:.4g} mode-samples/s")
+        cpu = cpu_baseline_batch(batch_inputs)
+        log(f"[bench] CPU {cpu['value']:.4g} mode-samples/s (1 thread), "
+            f"{cpu['omp']['value']:.4g} with {cpu['omp']['threads']} threads ({cpu['omp']['cpu_model']})")
+        if single is not None:
+            single["cpu_baseline"] = cpu_baseline(inp1)
 
     out = {
-        "metric": "mode-samples/s of the interpolated mode sum at rel-L2 < 1e-6 vs the FP64 oracle",
-        "value": value,
+        "metric": "mode-samples/s & waveforms/s of the batched interpolated mode sum at rel-L2 "
+                  "< 1e-6 vs the FP64 oracle (and vs the plain FP64 GPU sum)",
+        "value": batch["mode_samples_per_s"],
         "unit": "mode-samples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
-        "timing": "steps replayed as one captured HIP graph (spline build, plan + tiles, sum "
-                  "writing every sample); kernel_ms from eager steps with HIP events",
-        "eager_ms_per_step": ms_eager,
+        "ms_per_step": batch["ms_per_step"],
+        "timing": "steps replayed as one captured HIP graph (every waveform's splines, plans + "
+                  "tiles, the persistent sum writing every sample); stage_ms from eager "
+                  "launches with HIP events on the launching stream",
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"mixed": "f64 phases + f32 MFMA", "mixed16": "f64 phases + split-f16 MFMA (f32 acc)",
-                  "fp64": "f64"}[args.precision],
+        "dtype": "f64 phases + split-f16 MFMA (f32 acc)",
         "data": "synthetic, seeded (fastemriwaveforms_amd/synthetic.py)",
-        "config": {"workload": f"{args.workload}: T={inp.T_years} yr, dt={inp.dt} s, N={N} samples, "
-                               f"L={inp.L} knots, K={K} modes",
-                   "precision": args.precision, "mode_samples": K * N},
-        "waveforms_per_s": 1e3 / ms_step,
-        "rel_l2_vs_oracle": err,
-        "kernel_ms": ms,
-        "fp64_plain_gpu": fp64,
-        "speedup_vs_fp64_plain_gpu": (fp64["ms_per_step"] / ms_step) if fp64 else None,
+        "config": {"workload": f"config 3: {batch['waveforms']} uneven waveforms, durations "
+                               "U(0.1, 2) yr at dt = 10 s, knots U{50..200}, modes U{50..3000} "
+                               f"of the 3843-mode set; {batch['samples']} samples, "
+                               f"{batch['mode_samples']:.4g} mode-samples, "
+                               f"{batch['intervals']} knot intervals",
+                   "precision": "mixed16", "mode_samples": batch["mode_samples"],
+                   "waveforms": batch["waveforms"]},
+        "waveforms_per_s": batch["waveforms_per_s"],
+        "rel_l2_vs_fp64_all_samples": (batch.get("fp64_control") or {}).get(
+            "mixed16_rel_l2_vs_fp64_all_samples"),
+        "rel_l2_vs_oracle": batch["rel_l2_vs_oracle_max_of_8"],
+        "speedup_vs_fp64_plain_gpu": batch.get("speedup_vs_fp64_control"),
         "batch": batch,
+        "single_waveform": single,
         "mixed2": mixed2,
         "adjoint": adjoint,
         "fd": fd,

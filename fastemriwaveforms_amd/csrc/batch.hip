@@ -267,8 +267,9 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
         set_error("batch_run: null argument");
         return FEW_ERR_ARG;
     }
-    if (stage < 1 || stage > 7) {
-        set_error("batch_run: stage is a mask of 1 (splines), 2 (plan + tiles), 4 (sum)");
+    if (stage < 1 || stage > 15 || ((stage & 8) && !(stage & 4))) {
+        set_error("batch_run: stage is a mask of 1 (splines), 2 (plan + tiles), 4 (sum), "
+                  "8 (with 4: overwrite)");
         return FEW_ERR_ARG;
     }
     const BatchPlan &bp = *static_cast<const BatchPlan *>(batch);
@@ -288,7 +289,7 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     char *w = static_cast<char *>(workspace);
     BatchRunDev r{reinterpret_cast<Segment *>(w + L.segs), reinterpret_cast<int *>(w + L.tpref),
-                  reinterpret_cast<int *>(w + L.wt)};
+                  reinterpret_cast<int *>(w + L.wt), 0};
     BatchPtrs p{t, teuk, phase_knots, ylm, interp, pcoef, reinterpret_cast<double *>(w + L.region),
                 static_cast<double2 *>(wave)};
     const int tile = precision == FEW_PRECISION_FP64 ? direct_tile(FEW_PRECISION_FP64)
@@ -321,6 +322,10 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
                 set_error("batch_run: too many tiles");
                 return FEW_ERR_LIMIT;
             }
+            if (stage & 8) {
+                rc = launch_zero_complex(static_cast<double2 *>(wave), bp.tot[6] / 2, s);
+                if (rc) return rc;
+            }
             return launch_batch_direct_fp64(bp.dev, r, p, (int)bp.direct_blocks, s);
         }
         return FEW_OK;
@@ -331,7 +336,7 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
     }
     unsigned char *tiles = reinterpret_cast<unsigned char *>(w + L.region);
     const long tb = (long)fact16_tile_bytes(bp.uni);
-    const int sub = ((stage & 2) ? 1 : 0) | ((stage & 4) ? 2 : 0);
+    const int sub = ((stage & 2) ? 1 : 0) | ((stage & 4) ? 2 : 0) | ((stage & 8) ? 4 : 0);
     if (!sub) return FEW_OK;
     return launch_batch_fact16(bp.dev, r, p, tiles, tb, fact16_kb(bp.uni),
                                variant, sub, s);

@@ -51,6 +51,7 @@ struct BatchRunDev {
     Segment *segs;       // (total_intervals,)
     int *tpref;          // (sum L,) per-waveform exclusive tile prefix + total
     int *wt_prefix;      // (B+1,) tiles before each waveform
+    int overwrite;       // sum writes every output sample (0 after each waveform's last interval)
 };
 
 struct BatchPtrs {

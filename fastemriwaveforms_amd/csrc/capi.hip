@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void zero_complex_kernel(double2 *w, long n)
         w[i] = make_double2(0.0, 0.0);
 }
 
-static int launch_zero_complex(double2 *w, long n, hipStream_t stream)
+int launch_zero_complex(double2 *w, long n, hipStream_t stream)
 {
     if (n <= 0) return FEW_OK;
     const long blocks = std::min<long>((n + 255) / 256, 4096);

@@ -41,6 +41,8 @@ struct PrepKey {
     int precision, tile, L, out_len;
 };
 int prep_stage(bool prepares, bool sums, const PrepKey &k, const char *what);
+// Zero fill of n complex128 values by a kernel (capturable in a graph).
+int launch_zero_complex(double2 *w, long n, hipStream_t stream);
 
 #define FEW_LDS(kernel, bytes)                                                  \
     do {                                                                        \

@@ -536,6 +536,16 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
     const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
     const int t0 = (int)blockIdx.x * per;
     const int t1 = min(total, t0 + per);
+    if (r.overwrite) {
+        // each waveform's samples after its last interval: zero (its intervals cover [0, end))
+        for (int wb = 0; wb < B.B; wb++) {
+            const WaveDesc dw = B.desc[wb];
+            const long end = dw.L > 1 ? r.segs[dw.seg_off + dw.L - 2].end : 0;
+            double2 *w = p.wave + dw.wave_off / 2;
+            for (long i = end + (long)blockIdx.x * NT + tid; i < dw.N; i += (long)gridDim.x * NT)
+                w[i] = make_double2(0.0, 0.0);
+        }
+    }
     if (t0 >= t1) return;
     int b = upper_index(r.wt_prefix, B.B, t0);
     WaveDesc d = B.desc[b];
@@ -575,7 +585,7 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
         v4f acc[RB][F16_NB];
         double2 wold[F16_NB];
         F16V V;
-        f16_prefetch(sgj, base_i, wave, wold);
+        f16_prefetch(sgj, base_i, wave, wold, r.overwrite != 0);
         f16_setup<ARITH>(P, sPh, sgj, base_i, d.dt, V);
         f16_contract_prio<RB>(sTile, KB, V, acc, sched.prio);
         f16_epilogue<RB, ARITH>(P, sTile, KB, sPh, sgj, base_i, d.dt, acc, V, wold, wave);
@@ -696,14 +706,16 @@ int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs
         set_error("batch mixed16 sum: coefficient tile of %zu bytes exceeds LDS", lds);
         return FEW_ERR_UNSUPPORTED;
     }
+    BatchRunDev rs = r;
+    rs.overwrite = (stage & 4) ? 1 : 0;
     const int blocks = device_cus16() * (lds <= 80 * 1024 ? 2 : 1);
     const bool arith = B.uni.rows_arith != 0;
     const F16Sched sched = fact16_sched(variant);
     if (variant == 4)
-        return arith ? launch_batch_fact16_w<4, true>(B, r, p, tiles, tile_bytes, KB, blocks, lds, sched, stream)
-                     : launch_batch_fact16_w<4, false>(B, r, p, tiles, tile_bytes, KB, blocks, lds, sched, stream);
-    return arith ? launch_batch_fact16_w<8, true>(B, r, p, tiles, tile_bytes, KB, blocks, lds, sched, stream)
-                 : launch_batch_fact16_w<8, false>(B, r, p, tiles, tile_bytes, KB, blocks, lds, sched, stream);
+        return arith ? launch_batch_fact16_w<4, true>(B, rs, p, tiles, tile_bytes, KB, blocks, lds, sched, stream)
+                     : launch_batch_fact16_w<4, false>(B, rs, p, tiles, tile_bytes, KB, blocks, lds, sched, stream);
+    return arith ? launch_batch_fact16_w<8, true>(B, rs, p, tiles, tile_bytes, KB, blocks, lds, sched, stream)
+                 : launch_batch_fact16_w<8, false>(B, rs, p, tiles, tile_bytes, KB, blocks, lds, sched, stream);
 }
 
 }  // namespace few

@@ -143,6 +143,7 @@ class ModeSumProblem:
 BATCH_STAGE_SPLINES = 1
 BATCH_STAGE_PLAN = 2
 BATCH_STAGE_SUM = 4
+BATCH_STAGE_OVERWRITE = 8  # with BATCH_STAGE_SUM: write every output sample, no zeroing pass
 
 
 class BatchProblem:
@@ -179,6 +180,11 @@ class BatchProblem:
               "batch_layout")
         self.totals, self.offsets = totals, offsets
         self.supported16 = bool(totals[9] & 2)
+        # the union (row, column) grid every waveform's sum contracts
+        union = ModePlan(m, k, n)
+        self.plan_rows, self.plan_cols = union.rows, union.cols
+        self.plan_row_blocks, self.plan_ksteps = union.row_blocks, union.ksteps
+        del union
         self.mode_samples = int(np.sum(K.astype(np.int64) * N))
         self.samples = int(np.sum(N))
         # phases as dense output (every waveform) are an input in pcoef; else splined per step
@@ -222,9 +228,8 @@ class BatchProblem:
         check(rc, "batch_run")
 
     def run(self, precision: str = "mixed16") -> torch.Tensor:
-        """Zero the outputs and run every stage for every waveform."""
-        self.wave.zero_()
-        self.stages(precision, 7)
+        """Every stage for every waveform; the sum writes every output sample."""
+        self.stages(precision, 15)
         return self.wave
 
     def capture(self, precision: str = "mixed16") -> "torch.cuda.CUDAGraph":

@@ -176,7 +176,9 @@ int few_get_waveform_plan_async(void *d_waveform, const double *interp_array,
  * few_batch_run_async: precision FEW_PRECISION_FP64 (control run, the
  * reference arithmetic) or FEW_PRECISION_MIXED16; stage is a mask: 1 =
  * splines of every waveform, 2 = interval plan (+ coefficient tiles for
- * MIXED16), 4 = the mode sum.  Stage 1 overwrites the coefficient tiles, so
+ * MIXED16), 4 = the mode sum, 8 (with 4) = the sum overwrites: every output
+ * sample is written (0 after a waveform's last interval), so the outputs
+ * need no zeroing first.  Stage 1 overwrites the coefficient tiles, so
  * run 2 after it.  phase_knots may be NULL: pcoef then holds the caller's
  * dense-output phase coefficients (the trajectory's DOPR853 output,
  * src/few/trajectory/inspiral.py:143-147), which stage 1 leaves as they are.

@@ -121,3 +121,46 @@ def test_batch_sum_must_match_plan(gpu):
     bp.stages("mixed16", BATCH_STAGE_SPLINES | BATCH_STAGE_PLAN)
     bp.stages("mixed16", BATCH_STAGE_SUM)
     assert torch.equal(bp.wave, ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed16"])
+def test_batch_overwrite_equals_zero_then_accumulate(gpu, prec):
+    """Batch stage flag 8 (overwrite with the sum): every sample of every
+    waveform written, whatever the buffer held -- bit for bit zeroing and
+    accumulating; the batch holds a member whose output ends before its
+    trajectory and one with samples after its last knot."""
+    import torch
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.pipeline import (BATCH_STAGE_OVERWRITE, BATCH_STAGE_PLAN,
+                                                BATCH_STAGE_SPLINES, BATCH_STAGE_SUM,
+                                                BatchProblem)
+    inputs = _small_batch(seed=31, count=4)
+    longer = synthetic.make_input(0.002, 10.0, 12, "m_nonneg_30", 25, 79, name="after")
+    longer.num_pts += 2500
+    inputs.append(longer)
+    bp = BatchProblem(inputs)
+    bp.wave.zero_()
+    bp.stages(prec, BATCH_STAGE_SPLINES | BATCH_STAGE_PLAN | BATCH_STAGE_SUM)
+    ref = bp.wave.clone()
+    assert torch.count_nonzero(bp.waveform(bp.B - 1)[-2000:]) == 0
+    bp.wave.fill_(complex(float("nan"), 3.0))
+    bp.stages(prec, BATCH_STAGE_SUM | BATCH_STAGE_OVERWRITE)
+    assert torch.equal(torch.view_as_real(bp.wave), torch.view_as_real(ref))
+    bp.wave.fill_(complex(-7.0, float("nan")))
+    bp.run(prec)  # every stage, overwriting
+    assert torch.equal(torch.view_as_real(bp.wave), torch.view_as_real(ref))
+
+
+def test_batch_refuses_sum_after_splines_without_plan(gpu):
+    """Stage 1 reuses the mixed16 tile region as spline scratch: a sum in the
+    same call without stage 2, or a later sum-only call, is refused."""
+    from fastemriwaveforms_amd._lib import FewRocmError
+    from fastemriwaveforms_amd.pipeline import (BATCH_STAGE_SPLINES, BATCH_STAGE_SUM,
+                                                BatchProblem)
+    bp = BatchProblem(_small_batch(seed=3, count=2))
+    bp.run("mixed16")
+    with pytest.raises(FewRocmError, match="stage 2"):
+        bp.stages("mixed16", BATCH_STAGE_SPLINES | BATCH_STAGE_SUM)
+    bp.stages("mixed16", BATCH_STAGE_SPLINES)
+    with pytest.raises(FewRocmError, match="prepared"):
+        bp.stages("mixed16", BATCH_STAGE_SUM)

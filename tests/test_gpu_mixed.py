@@ -105,22 +105,32 @@ def test_mixed_accumulates_and_edges(gpu, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed16"])
-def test_overwrite_equals_zero_then_accumulate(gpu, prec):
+@pytest.mark.parametrize("tail", ["after_trajectory", "output_cut"])
+def test_overwrite_equals_zero_then_accumulate(gpu, prec, tail):
     """stage flag 4 (overwrite): every output sample written -- h inside the
     intervals, 0 in the tail after the last knot -- bit for bit what zeroing
-    the buffer and accumulating gives, whatever the buffer held before."""
+    the buffer and accumulating gives, whatever the buffer held before; also
+    when the output ends several knot spacings before the trajectory (the
+    find_start_inds stop: the intervals after it own no samples)."""
     import torch
     from fastemriwaveforms_amd import synthetic
     from fastemriwaveforms_amd.pipeline import ModeSumProblem
     inp = synthetic.small_input(seed=29, L=14, K=30, T_years=0.002)
-    inp.num_pts += 3000  # samples after the last knot
+    if tail == "after_trajectory":
+        inp.num_pts += 3000  # samples after the last knot
+    else:
+        # the output ends between knots L-4 and L-3: the last two intervals own no samples
+        inp.num_pts = int(0.5 * (inp.t[-4] + inp.t[-3]) / inp.dt)
+        assert inp.t[-4] < (inp.num_pts - 1) * inp.dt < inp.t[-3]
     p = ModeSumProblem(inp)
     p.build_splines()
     p.wave.zero_()
     p.mode_sum(prec)
     ref = p.wave.clone()
-    tail = ref[-2000:]
-    assert torch.count_nonzero(tail) == 0
+    if tail == "after_trajectory":
+        assert torch.count_nonzero(ref[-2000:]) == 0
+    else:
+        assert torch.count_nonzero(ref[-100:]) == 100
     gen = torch.Generator(device=p.wave.device).manual_seed(5)
     junk = torch.randn(inp.num_pts, 2, dtype=torch.float64, device=p.wave.device, generator=gen)
     p.wave.copy_(torch.view_as_complex(junk))
