@@ -122,6 +122,18 @@ class RocmBackend(Backend):
             raise MissingHardware("'rocm' backend needs a visible AMD GPU.")
         super().__init__(name="rocm", methods=methods, features=Backend.Feature.NUMPY)
 
+    @property
+    def precision(self) -> str:
+        """Arithmetic of get_waveform_wrap ("fp64", "mixed16", "mixed"); the
+        FEW_ROCM_PRECISION environment variable sets it at import."""
+        from . import pyinterp
+        return pyinterp.get_precision()
+
+    @precision.setter
+    def precision(self, value: str) -> None:
+        from . import pyinterp
+        pyinterp.set_precision(value)
+
 
 KNOWN_BACKENDS = {"rocm": RocmBackend}
 

@@ -121,12 +121,17 @@ static PlanLayout plan_layout(int L)
 
 // Workspace of the factorised path: plan layout, then the coefficient tiles.
 static size_t fact_coef_offset(int L) { return plan_layout(L).total; }
-static size_t fact_ws_bytes(const FactPlanHost &p, int L)
+static size_t fact_tiles_bytes(const FactPlanHost &p, int L)
 {
     const size_t n = (size_t)(L > 1 ? L - 1 : 1);
     const size_t f32 = align256(sizeof(float) * fact_coef_floats_per_interval(p) * n);
     const size_t f16 = align256(fact16_tile_bytes(p) * n);
-    return fact_coef_offset(L) + (f32 > f16 ? f32 : f16);
+    return f32 > f16 ? f32 : f16;
+}
+// plan layout | coefficient tiles | row maxima of the mixed16 tile prep
+static size_t fact_ws_bytes(const FactPlanHost &p, int L)
+{
+    return fact_coef_offset(L) + fact_tiles_bytes(p, L) + fact16_max_bytes(p, L - 1);
 }
 static bool fact16_supported(const FactPlanHost &p)
 {
@@ -278,8 +283,10 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
             PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
             int rc = launch_plan(p, stream);
             if (rc) return rc;
+            double *maxes = reinterpret_cast<double *>(ws + fact_coef_offset(L) +
+                                                       fact_tiles_bytes(ph, L));
             Fact16PrepArgs pa{interp, ylm, h_t, L, tiles, tb, fact16_kb(ph), ph.RB,
-                              plan->dev};
+                              plan->dev, maxes};
             rc = launch_fact16_prep(pa, stream);
             if (rc) return rc;
         }

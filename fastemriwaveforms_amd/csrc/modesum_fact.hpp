@@ -97,14 +97,14 @@ int launch_fact_prep(const FactPrepArgs &a, hipStream_t stream);
 
 // ---- FP16 split-operand variant (modesum_fact16.hip)
 // Tile of one interval: two FP16 planes (hi, lo), each KB x RB blocks of
-// 16 real rows x 32 halfs (block (kb, rb) at (kb * RB + rb) KiB), then RB*8
-// floats of inverse row scales, padded to 256 bytes.  The four 16-byte
+// 16 real rows x 32 halfs (block (kb, rb) at (kb * RB + rb) KiB), then 4
+// floats of inverse scales (one per power p), padded to 256 bytes.  The four 16-byte
 // chunks of a row are XOR-swizzled by (row >> 2) & 3, which puts the 16 rows
 // of a fragment read on distinct LDS banks without padding.
 inline int fact16_kb(const FactPlanHost &p) { return (2 * p.Nc + 31) / 32; }
 inline size_t fact16_tile_bytes(const FactPlanHost &p)
 {
-    const size_t b = 2 * (size_t)fact16_kb(p) * p.RB * 1024 + (size_t)p.RB * 8 * 4;
+    const size_t b = 2 * (size_t)fact16_kb(p) * p.RB * 1024 + 4 * sizeof(float);
     return (b + 255) & ~(size_t)255;
 }
 
@@ -117,7 +117,15 @@ struct Fact16PrepArgs {
     long tile_bytes;
     int KB, RB;
     FactPlanDev plan;
+    double *maxes;                // (L-1) x 2RB x 4 row maxima (workspace)
 };
+
+// Workspace of the row maxima the coefficient-tile prep passes between its two launches.
+inline size_t fact16_max_bytes(const FactPlanHost &p, long intervals)
+{
+    const size_t b = sizeof(double) * 8 * (size_t)p.RB * (size_t)(intervals > 0 ? intervals : 1);
+    return (b + 255) & ~(size_t)255;
+}
 
 // Wave priority of the mixed16 sum kernels (modesum_fact16.hip): with prio
 // a wave raises its priority over its MFMA contraction, so the two waves

@@ -5,9 +5,10 @@
 // v_mfma_f32_16x16x32_f16 with each operand split into two FP16 parts,
 //     A = A_hi + A_lo,  B = B_hi + B_lo   (A_hi = fp16(A), A_lo = fp16(A - A_hi)),
 // and three products A_lo B_hi + A_hi B_lo + A_hi B_hi accumulated in FP32
-// (the dropped A_lo B_lo is below 2^-22 relative).  Each complex row of A is
-// scaled by a power of two so its largest entry sits near 2^14 (no FP16
-// overflow, small entries keep their absolute precision); B = 256 v_n(t).
+// (the dropped A_lo B_lo is below 2^-22 relative).  The rows of each power p
+// of A are scaled by one power of two so their largest entry sits near 2^14
+// (no FP16 overflow; FP16 keeps its relative precision below that); B =
+// 256 v_n(t).
 // The scales are undone exactly in the epilogue.  Phases stay FP64 and the
 // output accumulates in FP64, as in the FP32 kernel.
 //
@@ -20,7 +21,7 @@
 //       row are XOR-swizzled by (row >> 2) & 3 (distinct LDS banks for the 16
 //       rows a fragment read touches).  The A fragment of a lane is 8
 //       consecutive halfs (one ds_read_b128);
-//   inv_sigma: RB*8 floats, 1/scale of complex row R.
+//   inv_sigma: 4 floats, 1/scale of the rows of power p = 0..3.
 
 #include <math.h>
 
@@ -51,108 +52,224 @@ __device__ __forceinline__ int f16_lane_off(int li, int g) { return li * 32 + ((
 #define INV_2PI16 0.15915494309189533577
 
 // ------------------------------------------------------------- prep kernel
-// One block per (interval j, row group r): gathers C_p[r, a] for the 4
-// powers p (FP64, fixed order per cell), scales each complex row R = 4r + p
-// by a power of two and writes the split FP16 planes.
-__device__ void fact16_prep_block(const FactPlanDev &P, const double *interp, const double2 *ylm,
-                                  const double *h_t, int L, int j, int r, unsigned char *tile,
-                                  int KB, int RB)
+// Gathers C_p[r, a] for every cell and power p (FP64, fixed order per cell),
+// scales the rows of each power of an interval by one power of two (exact;
+// the largest entry of the power near 2^14, so no FP16 overflow) and writes
+// the split FP16 planes.  Two launches over (interval, group of four rows):
+// the row maxima, then the interval's scales and the split values (the cells
+// are gathered again rather than held).
+#define F16_PREP_ROWS 4  // complex rows r per prep workgroup, one wave each
+
+// C_p[r, col] * h^p of one cell (zero for padding rows and columns)
+__device__ __forceinline__ void f16_cell(const FactPlanDev &P, const double *interp,
+                                         const double2 *ylm, int L, int j, double hw, int r,
+                                         int col, double (&cre)[4], double (&cim)[4])
 {
-    __shared__ double s_max[4][256];
-    const int tid = threadIdx.x;
-    const int KP = 32 * KB;
-    const int ncolc = KP / 2;  // complex columns incl. padding
-    double cre[2][4], cim[2][4];
-    double mx[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        cre[p] = 0.0;
+        cim[p] = 0.0;
+    }
+    if (r >= P.Nr || col >= P.Nc) return;
     const long K2 = 2L * P.K;
+    const int cell = r * P.Nc + col;
+    for (int e = P.cell_start[cell]; e < P.cell_start[cell + 1]; e++) {
+        const int ent = P.entries[e];
+        const int qm = ent >> 1;
+        const bool cj = ent & 1;
+        const double2 Y = ylm[cj ? P.K + qm : qm];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const long base = ((long)p * L + j) * K2;
+            const double cr = interp[base + qm];
+            const double ci = cj ? -interp[base + P.K + qm] : interp[base + P.K + qm];
+            cre[p] += cr * Y.x - ci * Y.y;
+            cim[p] += cr * Y.y + ci * Y.x;
+        }
+    }
+    double hp = 1.0;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        cre[p] *= hp;
+        cim[p] *= hp;
+        hp *= hw;
+    }
+}
+
+// Pass 1: workgroup (j, row group) -- one wave per complex row r, lanes over
+// the columns -- writes the row's largest |Re|, |Im| of each power to
+// maxes[r][p] of interval j.
+__device__ void fact16_rowmax_block(const FactPlanDev &P, const double *interp, const double2 *ylm,
+                                    const double *h_t, int L, int j, int rg, int KB, int RB,
+                                    double *maxes)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int r = rg * F16_PREP_ROWS + wv;
+    if (r >= 2 * RB) return;  // uniform over the wave
+    const int ncolc = 16 * KB;
     const double hw = h_t[j + 1] - h_t[j];
+    double mx[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int col = lane; col < ncolc; col += 64) {
+        double cre[4], cim[4];
+        f16_cell(P, interp, ylm, L, j, hw, r, col, cre, cim);
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-        const int col = tid + 256 * u;
-#pragma unroll
-        for (int p = 0; p < 4; p++) { cre[u][p] = 0.0; cim[u][p] = 0.0; }
-        if (r < P.Nr && col < P.Nc) {
-            const int cell = r * P.Nc + col;
-            for (int e = P.cell_start[cell]; e < P.cell_start[cell + 1]; e++) {
-                const int ent = P.entries[e];
-                const int qm = ent >> 1;
-                const bool cj = ent & 1;
-                const double2 Y = ylm[cj ? P.K + qm : qm];
-#pragma unroll
-                for (int p = 0; p < 4; p++) {
-                    const long base = ((long)p * L + j) * K2;
-                    const double cr = interp[base + qm];
-                    const double ci = cj ? -interp[base + P.K + qm] : interp[base + P.K + qm];
-                    cre[u][p] += cr * Y.x - ci * Y.y;
-                    cim[u][p] += cr * Y.y + ci * Y.x;
-                }
-            }
-            double hp = 1.0;
-#pragma unroll
-            for (int p = 0; p < 4; p++) {
-                cre[u][p] *= hp;
-                cim[u][p] *= hp;
-                hp *= hw;
-                mx[p] = fmax(mx[p], fmax(fabs(cre[u][p]), fabs(cim[u][p])));
-            }
-        }
+        for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], fmax(fabs(cre[p]), fabs(cim[p])));
     }
 #pragma unroll
-    for (int p = 0; p < 4; p++) s_max[p][tid] = mx[p];
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (tid < off) {
+    for (int p = 0; p < 4; p++) {
 #pragma unroll
-            for (int p = 0; p < 4; p++) s_max[p][tid] = fmax(s_max[p][tid], s_max[p][tid + off]);
-        }
-        __syncthreads();
+        for (int off = 32; off > 0; off >>= 1) mx[p] = fmax(mx[p], __shfl_xor(mx[p], off, 64));
     }
+    if (lane < 4) maxes[r * 4 + lane] = lane == 0 ? mx[0] : lane == 1 ? mx[1] : lane == 2 ? mx[2] : mx[3];
+}
+
+// Pass 2: the interval's scale per power from the row maxima, then the split
+// planes of the workgroup's rows.
+__device__ void fact16_split_block(const FactPlanDev &P, const double *interp, const double2 *ylm,
+                                   const double *h_t, int L, int j, int rg, int KB, int RB,
+                                   const double *maxes, unsigned char *tile)
+{
+    __shared__ double s_sigma[4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const long plane = (long)KB * RB * F16_BLK;
     _Float16 *hi = reinterpret_cast<_Float16 *>(tile);
     _Float16 *lo = hi + plane;
-    float *inv_sigma = reinterpret_cast<float *>(lo + plane);
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const double m = s_max[p][0];
+    if (tid < 4) {
+        double m = 0.0;
+        for (int rr = 0; rr < 2 * RB; rr++) m = fmax(m, maxes[rr * 4 + tid]);
         // sigma = 2^e with m * sigma in [2^13, 2^14): exact scaling
         int ex = 0;
         if (m > 0.0) {
             frexp(m, &ex);  // m = f * 2^ex, f in [0.5, 1)
             ex = 14 - ex;
         }
-        const double sigma = ldexp(1.0, ex);
-        const int R = 4 * r + p;
-        if (tid == 0) inv_sigma[R] = (float)ldexp(1.0, -ex);
+        s_sigma[tid] = ldexp(1.0, ex);
+        if (rg == 0) reinterpret_cast<float *>(lo + plane)[tid] = (float)ldexp(1.0, -ex);
+    }
+    __syncthreads();
+    const int r = rg * F16_PREP_ROWS + wv;
+    if (r >= 2 * RB) return;
+    const int ncolc = 16 * KB;
+    const double hw = h_t[j + 1] - h_t[j];
+    double sigma[4];
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
-            const int col = tid + 256 * u;
-            if (col < ncolc) {
-                const float xr = f32_opaque((float)(cre[u][p] * sigma));
-                const float xi = f32_opaque((float)(cim[u][p] * sigma));
-                const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
-                const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
-                // row 2R:   (Re, -Im);  row 2R+1: (Im, Re)
-                const long o0 = f16_tile_off(2 * R, 2 * col, RB);
-                const long o1 = f16_tile_off(2 * R + 1, 2 * col, RB);
-                hi[o0] = rh; hi[o0 + 1] = -ih; lo[o0] = rl; lo[o0 + 1] = -il;
-                hi[o1] = ih; hi[o1 + 1] = rh;  lo[o1] = il; lo[o1 + 1] = rl;
-            }
+    for (int p = 0; p < 4; p++) sigma[p] = s_sigma[p];
+    for (int col = lane; col < ncolc; col += 64) {
+        double cre[4], cim[4];
+        f16_cell(P, interp, ylm, L, j, hw, r, col, cre, cim);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const int R = 4 * r + p;
+            const float xr = f32_opaque((float)(cre[p] * sigma[p]));
+            const float xi = f32_opaque((float)(cim[p] * sigma[p]));
+            const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+            const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
+            // row 2R:   (Re, -Im);  row 2R+1: (Im, Re)
+            const long o0 = f16_tile_off(2 * R, 2 * col, RB);
+            const long o1 = f16_tile_off(2 * R + 1, 2 * col, RB);
+            hi[o0] = rh; hi[o0 + 1] = -ih; lo[o0] = rl; lo[o0 + 1] = -il;
+            hi[o1] = ih; hi[o1 + 1] = rh;  lo[o1] = il; lo[o1 + 1] = rl;
         }
     }
 }
 
-__global__ __launch_bounds__(256) void fact16_prep_kernel(Fact16PrepArgs a)
+// Both passes in one workgroup of F16_PREP_THREADS per interval (the block
+// reduces the maxima itself): for launches with many intervals (batches),
+// where one workgroup per interval already fills the device.
+#define F16_PREP_THREADS 512
+#define F16_PREP_WHOLE_MIN 1024  // intervals from which a launch takes this form
+
+__device__ void fact16_prep_interval(const FactPlanDev &P, const double *interp, const double2 *ylm,
+                                     const double *h_t, int L, int j, unsigned char *tile, int KB,
+                                     int RB)
+{
+    __shared__ double s_red[F16_PREP_THREADS / 64][4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int ncolc = 16 * KB;         // complex columns incl. padding
+    const int ncell = 2 * RB * ncolc;  // complex rows r incl. padding x columns
+    const double hw = h_t[j + 1] - h_t[j];
+    double mx[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int ci = tid; ci < ncell; ci += F16_PREP_THREADS) {
+        const int r = ci / ncolc, col = ci - r * ncolc;
+        double cre[4], cim[4];
+        f16_cell(P, interp, ylm, L, j, hw, r, col, cre, cim);
+#pragma unroll
+        for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], fmax(fabs(cre[p]), fabs(cim[p])));
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx[p] = fmax(mx[p], __shfl_xor(mx[p], off, 64));
+        if (lane == 0) s_red[wv][p] = mx[p];
+    }
+    __syncthreads();
+    const long plane = (long)KB * RB * F16_BLK;
+    _Float16 *hi = reinterpret_cast<_Float16 *>(tile);
+    _Float16 *lo = hi + plane;
+    double sigma[4];
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        double m = 0.0;
+        for (int w = 0; w < F16_PREP_THREADS / 64; w++) m = fmax(m, s_red[w][p]);
+        int ex = 0;
+        if (m > 0.0) {
+            frexp(m, &ex);
+            ex = 14 - ex;
+        }
+        sigma[p] = ldexp(1.0, ex);
+        if (tid == 0) reinterpret_cast<float *>(lo + plane)[p] = (float)ldexp(1.0, -ex);
+    }
+    for (int ci = tid; ci < ncell; ci += F16_PREP_THREADS) {
+        const int r = ci / ncolc, col = ci - r * ncolc;
+        double cre[4], cim[4];
+        f16_cell(P, interp, ylm, L, j, hw, r, col, cre, cim);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const int R = 4 * r + p;
+            const float xr = f32_opaque((float)(cre[p] * sigma[p]));
+            const float xi = f32_opaque((float)(cim[p] * sigma[p]));
+            const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+            const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
+            const long o0 = f16_tile_off(2 * R, 2 * col, RB);
+            const long o1 = f16_tile_off(2 * R + 1, 2 * col, RB);
+            hi[o0] = rh; hi[o0 + 1] = -ih; lo[o0] = rl; lo[o0 + 1] = -il;
+            hi[o1] = ih; hi[o1 + 1] = rh;  lo[o1] = il; lo[o1 + 1] = rl;
+        }
+    }
+}
+
+__global__ __launch_bounds__(F16_PREP_THREADS) void fact16_prep_whole_kernel(Fact16PrepArgs a)
 {
     const int j = blockIdx.x;
-    fact16_prep_block(a.plan, a.interp, a.ylm, a.h_t, a.L, j, blockIdx.y,
-                      a.tiles + (long)j * a.tile_bytes, a.KB, a.RB);
+    fact16_prep_interval(a.plan, a.interp, a.ylm, a.h_t, a.L, j, a.tiles + (long)j * a.tile_bytes,
+                         a.KB, a.RB);
+}
+
+__global__ __launch_bounds__(64 * F16_PREP_ROWS) void fact16_rowmax_kernel(Fact16PrepArgs a)
+{
+    const int j = blockIdx.x;
+    fact16_rowmax_block(a.plan, a.interp, a.ylm, a.h_t, a.L, j, blockIdx.y, a.KB, a.RB,
+                        a.maxes + (long)j * 8 * a.RB);
+}
+
+__global__ __launch_bounds__(64 * F16_PREP_ROWS) void fact16_prep_kernel(Fact16PrepArgs a)
+{
+    const int j = blockIdx.x;
+    fact16_split_block(a.plan, a.interp, a.ylm, a.h_t, a.L, j, blockIdx.y, a.KB, a.RB,
+                       a.maxes + (long)j * 8 * a.RB, a.tiles + (long)j * a.tile_bytes);
 }
 
 int launch_fact16_prep(const Fact16PrepArgs &a, hipStream_t stream)
 {
     if (a.L < 2) return FEW_OK;
-    dim3 grid(a.L - 1, 2 * a.RB);
-    hipLaunchKernelGGL(fact16_prep_kernel, grid, dim3(256), 0, stream, a);
+    if (a.L - 1 >= F16_PREP_WHOLE_MIN) {
+        hipLaunchKernelGGL(fact16_prep_whole_kernel, dim3(a.L - 1), dim3(F16_PREP_THREADS), 0,
+                           stream, a);
+    } else {
+        const dim3 grid(a.L - 1, (2 * a.RB + F16_PREP_ROWS - 1) / F16_PREP_ROWS);
+        hipLaunchKernelGGL(fact16_rowmax_kernel, grid, dim3(64 * F16_PREP_ROWS), 0, stream, a);
+        hipLaunchKernelGGL(fact16_prep_kernel, grid, dim3(64 * F16_PREP_ROWS), 0, stream, a);
+    }
     FEW_HIP(hipGetLastError());
     return FEW_OK;
 }
@@ -279,10 +396,9 @@ __device__ __forceinline__ void f16_prefetch(const Segment &sg, int base_i, cons
 // n = n_lo + 16 kb + 4 g + t, t = 0..3, scaled by 256.
 template <bool ARITH>
 __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sPh, const Segment &sg,
-                                          int base_i, double dt, F16V &V)
+                                          double inv_pw, int base_i, double dt, F16V &V)
 {
     const int lane = threadIdx.x & 63, g = lane >> 4, li = lane & 15;
-    const double inv_pw = 1.0 / sg.pw;
 #pragma unroll
     for (int c = 0; c < F16_NB; c++) {
         const double t = dt * (double)(base_i + c * 16);
@@ -370,16 +486,14 @@ __device__ __forceinline__ void f16_contract(const unsigned char *sTile, int KB,
 template <int RB, bool ARITH>
 __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigned char *sTile,
                                              int KB, const double *sPh, const Segment &sg,
-                                             int base_i, double dt,
+                                             double inv_pw, double inv_hw, int base_i, double dt,
                                              const v4f (&acc)[RB][F16_NB], const F16V &V,
                                              double2 (&wold)[F16_NB], double2 *wave)
 {
     const int g = (threadIdx.x & 63) >> 4;
     const int li = threadIdx.x & 15;
     const long plane = (long)KB * RB * F16_BLK;
-    const float *sInv = reinterpret_cast<const float *>(sTile + 4 * plane);
-    const double inv_pw = 1.0 / sg.pw;
-    const double inv_hw = 1.0 / sg.hw;
+    const float *sInv = reinterpret_cast<const float *>(sTile + 4 * plane);  // 1/scale per power
     const int f = g & 1;
     const int p0 = 2 * f;
 #pragma unroll
@@ -391,6 +505,9 @@ __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigne
         const float s2f = s1f * s1f;
         const float spa = (p0 == 0 ? 1.f : s2f) * (1.0f / 256.0f);
         const float spb = (p0 == 0 ? s1f : s2f * s1f) * (1.0f / 256.0f);
+        // powers p0, p0 + 1 of this lane's rows, with their tile scales undone
+        const float ka = spa * sInv[p0];
+        const float kb2 = spb * sInv[p0 + 1];
         double hre = 0.0, him = 0.0;
         if constexpr (ARITH) {
             // Rows of one lane are r = 2 rb + (g >> 1) with m_r = m_first + 2 rb:
@@ -400,10 +517,7 @@ __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigne
             const double e_re = __shfl(V.pc[c], li + 16), e_im = -__shfl(V.ps[c], li + 16);
 #pragma unroll
             for (int rb = RB - 1; rb >= 0; rb--) {
-                // rows r >= Nr hold zeros (W = 0, scale 1)
-                const int R0 = 8 * rb + 2 * g;
-                const float ka = spa * sInv[R0];
-                const float kb2 = spb * sInv[R0 + 1];
+                // rows r >= Nr hold zeros (W = 0)
                 const v4f W = acc[rb][c];
                 const double zr = (double)(ka * W[0] + kb2 * W[2]);
                 const double zi = (double)(ka * W[1] + kb2 * W[3]);
@@ -429,10 +543,7 @@ __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigne
                 double sn, cs;
                 sincos_turns_short(reduce_turns((double)mk.x * uph + (double)mk.y * uth), sn, cs);
                 const float ur = (float)cs, ui = (float)(-sn);
-                // rows r >= Nr hold zeros (W = 0, scale 1)
-                const int R0 = 8 * rb + 2 * g;
-                const float ka = spa * sInv[R0];
-                const float kb2 = spb * sInv[R0 + 1];
+                // rows r >= Nr hold zeros (W = 0)
                 const v4f W = acc[rb][c];
                 const float zr = ka * W[0] + kb2 * W[2];
                 const float zi = ka * W[1] + kb2 * W[3];
@@ -494,6 +605,7 @@ void fact16_kernel(Fact16Args a)
     const long np = (long)a.nseg * 3;
     bool stale = true;
     Segment sgj;
+    double inv_pw = 1.0, inv_hw = 1.0;  // per interval, computed when the interval changes
     for (int tile = t0; tile < t1; tile++) {
         while (tile >= jnext_start) {
             j++;
@@ -502,6 +614,8 @@ void fact16_kernel(Fact16Args a)
         }
         if (stale) {
             sgj = a.segs[j];
+            inv_pw = 1.0 / sgj.pw;
+            inv_hw = 1.0 / sgj.hw;
             __syncthreads();
             const uint4 *src = reinterpret_cast<const uint4 *>(a.tiles + (long)j * a.tile_bytes);
             uint4 *dst = reinterpret_cast<uint4 *>(sTile);
@@ -516,9 +630,10 @@ void fact16_kernel(Fact16Args a)
         double2 wold[F16_NB];
         F16V V;
         f16_prefetch(sgj, base_i, a.wave, wold, a.overwrite != 0);
-        f16_setup<ARITH>(a.plan, sPh, sgj, base_i, a.dt, V);
+        f16_setup<ARITH>(a.plan, sPh, sgj, inv_pw, base_i, a.dt, V);
         f16_contract_prio<RB>(sTile, a.KB, V, acc, a.sched.prio);
-        f16_epilogue<RB, ARITH>(a.plan, sTile, a.KB, sPh, sgj, base_i, a.dt, acc, V, wold, a.wave);
+        f16_epilogue<RB, ARITH>(a.plan, sTile, a.KB, sPh, sgj, inv_pw, inv_hw, base_i, a.dt, acc, V,
+                                wold, a.wave);
     }
 }
 
@@ -554,6 +669,7 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
     FactPlanDev P = batch_wave_plan(B, d);
     bool stale = true;
     Segment sgj;
+    double inv_pw = 1.0, inv_hw = 1.0;  // per interval, computed when the interval changes
     for (int tile = t0; tile < t1; tile++) {
         while (tile >= r.wt_prefix[b + 1]) {  // next waveform with tiles
             b++;
@@ -570,6 +686,8 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
         }
         if (stale) {
             sgj = r.segs[d.seg_off + j];
+            inv_pw = 1.0 / sgj.pw;
+            inv_hw = 1.0 / sgj.hw;
             __syncthreads();
             const uint4 *src = reinterpret_cast<const uint4 *>(tiles + (long)(d.seg_off + j) * tile_bytes);
             uint4 *dst = reinterpret_cast<uint4 *>(sTile);
@@ -586,23 +704,48 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
         double2 wold[F16_NB];
         F16V V;
         f16_prefetch(sgj, base_i, wave, wold, r.overwrite != 0);
-        f16_setup<ARITH>(P, sPh, sgj, base_i, d.dt, V);
+        f16_setup<ARITH>(P, sPh, sgj, inv_pw, base_i, d.dt, V);
         f16_contract_prio<RB>(sTile, KB, V, acc, sched.prio);
-        f16_epilogue<RB, ARITH>(P, sTile, KB, sPh, sgj, base_i, d.dt, acc, V, wold, wave);
+        f16_epilogue<RB, ARITH>(P, sTile, KB, sPh, sgj, inv_pw, inv_hw, base_i, d.dt, acc, V, wold,
+                                wave);
     }
 }
 
-__global__ __launch_bounds__(256) void batch_fact16_prep_kernel(BatchDev B, BatchPtrs p,
-                                                                 unsigned char *tiles,
-                                                                 long tile_bytes, int KB, int RB)
+__global__ __launch_bounds__(F16_PREP_THREADS) void batch_fact16_prep_whole_kernel(
+    BatchDev B, BatchPtrs p, unsigned char *tiles, long tile_bytes, int KB, int RB)
 {
     const int gi = blockIdx.x;
     const int b = upper_index(B.seg_prefix, B.B, gi);
     const WaveDesc d = B.desc[b];
     const int j = gi - B.seg_prefix[b];
-    fact16_prep_block(batch_wave_plan(B, d), p.interp + d.interp_off,
-                      reinterpret_cast<const double2 *>(p.ylm + d.ylm_off), p.t + d.t_off, d.L, j,
-                      blockIdx.y, tiles + (long)gi * tile_bytes, KB, RB);
+    fact16_prep_interval(batch_wave_plan(B, d), p.interp + d.interp_off,
+                         reinterpret_cast<const double2 *>(p.ylm + d.ylm_off), p.t + d.t_off, d.L,
+                         j, tiles + (long)gi * tile_bytes, KB, RB);
+}
+
+__global__ __launch_bounds__(64 * F16_PREP_ROWS) void batch_fact16_rowmax_kernel(
+    BatchDev B, BatchPtrs p, double *maxes, int KB, int RB)
+{
+    const int gi = blockIdx.x;
+    const int b = upper_index(B.seg_prefix, B.B, gi);
+    const WaveDesc d = B.desc[b];
+    const int j = gi - B.seg_prefix[b];
+    fact16_rowmax_block(batch_wave_plan(B, d), p.interp + d.interp_off,
+                        reinterpret_cast<const double2 *>(p.ylm + d.ylm_off), p.t + d.t_off, d.L, j,
+                        blockIdx.y, KB, RB, maxes + (long)gi * 8 * RB);
+}
+
+__global__ __launch_bounds__(64 * F16_PREP_ROWS) void batch_fact16_prep_kernel(
+    BatchDev B, BatchPtrs p, const double *maxes, unsigned char *tiles, long tile_bytes, int KB,
+    int RB)
+{
+    const int gi = blockIdx.x;
+    const int b = upper_index(B.seg_prefix, B.B, gi);
+    const WaveDesc d = B.desc[b];
+    const int j = gi - B.seg_prefix[b];
+    fact16_split_block(batch_wave_plan(B, d), p.interp + d.interp_off,
+                       reinterpret_cast<const double2 *>(p.ylm + d.ylm_off), p.t + d.t_off, d.L, j,
+                       blockIdx.y, KB, RB, maxes + (long)gi * 8 * RB, tiles + (long)gi * tile_bytes);
 }
 
 int fact16_tile(int variant) { return 16 * F16_NB * (variant == 4 ? 4 : 8); }
@@ -689,14 +832,23 @@ static int launch_batch_fact16_w(const BatchDev &B, const BatchRunDev &r, const 
 }
 
 int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
-                        unsigned char *tiles, long tile_bytes, int KB, int variant,
+                        unsigned char *tiles, long tile_bytes, double *maxes, int KB, int variant,
                         int stage, hipStream_t stream)
 {
     if (stage & 1) {
         if (B.total_intervals > 0) {
-            dim3 grid(B.total_intervals, 2 * B.uni.RB);
-            hipLaunchKernelGGL(batch_fact16_prep_kernel, grid, dim3(256), 0, stream, B, p, tiles,
-                               tile_bytes, KB, B.uni.RB);
+            if (B.total_intervals >= F16_PREP_WHOLE_MIN) {
+                hipLaunchKernelGGL(batch_fact16_prep_whole_kernel, dim3(B.total_intervals),
+                                   dim3(F16_PREP_THREADS), 0, stream, B, p, tiles, tile_bytes, KB,
+                                   B.uni.RB);
+            } else {
+                const dim3 grid(B.total_intervals,
+                                (2 * B.uni.RB + F16_PREP_ROWS - 1) / F16_PREP_ROWS);
+                hipLaunchKernelGGL(batch_fact16_rowmax_kernel, grid, dim3(64 * F16_PREP_ROWS), 0,
+                                   stream, B, p, maxes, KB, B.uni.RB);
+                hipLaunchKernelGGL(batch_fact16_prep_kernel, grid, dim3(64 * F16_PREP_ROWS), 0,
+                                   stream, B, p, maxes, tiles, tile_bytes, KB, B.uni.RB);
+            }
             FEW_HIP(hipGetLastError());
         }
     }

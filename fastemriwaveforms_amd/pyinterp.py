@@ -12,9 +12,19 @@ Array arguments may be
   CPU backend leaves in them.
 
 Scalars pass through, like ``obj_to_ptr`` (src/few/cutils/wrappers.py:13-25).
+
+Precision of ``get_waveform_wrap`` (the reference has one, FP64): the
+environment variable ``FEW_ROCM_PRECISION`` or :func:`set_precision` selects
+``"fp64"`` (default: the reference's arithmetic), ``"mixed16"`` or
+``"mixed"`` (the factorised MFMA sums, rel-L2 ~1.5e-7 against FP64).  The
+factorised sums need a mode plan, built once per mode set (m, k, n) and
+cached; a mode set whose tiles do not fit runs in FP64.
 """
 
 from __future__ import annotations
+
+import logging
+import os
 
 import numpy as np
 import torch
@@ -23,7 +33,56 @@ from . import _lib
 from ._lib import check, lib
 
 __all__ = ["interpolate_arrays_wrap", "get_waveform_wrap", "get_waveform_generic_fd_wrap",
-           "interp2D", "pyWaveform"]
+           "interp2D", "pyWaveform", "set_precision", "get_precision"]
+
+_PRECISION_CODES = {"fp64": _lib.FEW_PRECISION_FP64, "mixed": _lib.FEW_PRECISION_MIXED,
+                    "mixed16": _lib.FEW_PRECISION_MIXED16}
+
+
+def _check_precision(p: str) -> str:
+    if p not in _PRECISION_CODES:
+        raise ValueError(f"precision must be one of {sorted(_PRECISION_CODES)}, got {p!r}")
+    return p
+
+
+_precision = _check_precision(os.environ.get("FEW_ROCM_PRECISION", "fp64"))
+
+
+def set_precision(precision: str) -> None:
+    """Arithmetic of get_waveform_wrap: "fp64" (the reference's), "mixed16" or "mixed"."""
+    global _precision
+    _precision = _check_precision(precision)
+
+
+def get_precision() -> str:
+    return _precision
+
+
+_plans: dict = {}        # mode-set key -> ModePlan (most recent few)
+_workspaces: dict = {}   # device index -> uint8 tensor
+
+
+def _mode_plan(m, k, n):
+    from .pipeline import ModePlan
+    host = [np.ascontiguousarray(x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else x,
+                                 np.int32) for x in (m, k, n)]
+    key = b"".join(h.tobytes() for h in host)
+    plan = _plans.get(key)
+    if plan is None:
+        if len(_plans) >= 4:
+            _plans.pop(next(iter(_plans)))
+        plan = ModePlan(*host)
+        _plans[key] = plan
+    return plan
+
+
+def _workspace(nbytes: int) -> torch.Tensor:
+    dev = _device()
+    ws = _workspaces.get(dev.index)
+    if ws is None or ws.numel() < nbytes:
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        _workspaces[dev.index] = ws
+    return ws
 
 
 def _device() -> torch.device:
@@ -129,9 +188,31 @@ def get_waveform_wrap(*args, **kwargs):
     # The reference selects `dev` with cudaSetDevice (interpolate.cu:753); the
     # buffers here already live on one device, which is the one used.
     dev_index = wave.dev.device.index
-    rc = lib().few_get_waveform(wave.ptr, interp.ptr, pt.ctypes.data, pcoef.ptr, m.ptr, k.ptr, n.ptr,
-                                L, N, K, ylm.ptr, float(delta_t), ht.ctypes.data,
-                                int(dev_index), _stream_ptr())
+    prec = _precision
+    plan = None
+    if prec != "fp64" and K > 0:
+        plan = _mode_plan(d_m, d_k, d_n)
+        if not (plan.supported16 if prec == "mixed16" else plan.supported):
+            logging.getLogger("fastemriwaveforms_amd").warning(
+                "get_waveform_wrap: the mode set's tiles do not fit the %s sum; running FP64", prec)
+            prec = "fp64"
+    if prec == "fp64":
+        rc = lib().few_get_waveform(wave.ptr, interp.ptr, pt.ctypes.data, pcoef.ptr, m.ptr, k.ptr,
+                                    n.ptr, L, N, K, ylm.ptr, float(delta_t), ht.ctypes.data,
+                                    int(dev_index), _stream_ptr())
+    else:
+        # the factorised sum: device copies of the two host time arrays, the
+        # cached plan and workspace; accumulates into d_waveform like the reference
+        d_pt = torch.from_numpy(pt[:L]).to(wave.dev.device)
+        d_ht = torch.from_numpy(ht[:L]).to(wave.dev.device)
+        code = _PRECISION_CODES[prec]
+        ws = _workspace(plan.workspace_bytes(L, N, prec))
+        rc = lib().few_get_waveform_plan_async(
+            wave.ptr, interp.ptr, d_pt.data_ptr(), pcoef.ptr, plan.handle, m.ptr, k.ptr, n.ptr, L,
+            N, K, ylm.ptr, float(delta_t), d_ht.data_ptr(), code, 0, 3, ws.data_ptr(), ws.numel(),
+            _stream_ptr())
+        if rc == 0:
+            torch.cuda.current_stream().synchronize()  # synchronous, as the reference
     check(rc, "get_waveform")
     wave.finish()
 

@@ -42,3 +42,18 @@ def test_error_paths_without_gpu():
     rc = lib.few_spline_build_async(1, 1, 1, 1, 0, 1, 1, 1, 1, 1, 5, 2, 0, 1, None)
     assert rc != 0 and b"length >= 3" in lib.few_last_error()
     assert lib.few_abi_version() == 1
+
+
+def test_dropin_precision_setting():
+    """The drop-in get_waveform_wrap's precision: FP64 (the reference's) unless set;
+    unknown names are refused (no GPU calls)."""
+    from fastemriwaveforms_amd import pyinterp
+    before = pyinterp.get_precision()
+    try:
+        pyinterp.set_precision("mixed16")
+        assert pyinterp.get_precision() == "mixed16"
+        with pytest.raises(ValueError):
+            pyinterp.set_precision("fp16")
+        assert pyinterp.get_precision() == "mixed16"
+    finally:
+        pyinterp.set_precision(before)

@@ -164,3 +164,54 @@ def test_batch_refuses_sum_after_splines_without_plan(gpu):
     bp.stages("mixed16", BATCH_STAGE_SPLINES)
     with pytest.raises(FewRocmError, match="prepared"):
         bp.stages("mixed16", BATCH_STAGE_SUM)
+
+
+def test_batch_many_intervals(gpu):
+    """A batch with more than 1024 knot intervals (the coefficient tiles are then
+    prepared one workgroup per interval): mixed16 against the FP64 control on
+    every sample and against the oracle on two waveforms."""
+    import torch
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.pipeline import BatchProblem
+    rng = np.random.default_rng(41)
+    inputs = [synthetic.make_input(0.004, 10.0, 100, "m_nonneg_30", int(rng.integers(100, 400)),
+                                   900 + b) for b in range(12)]
+    bp = BatchProblem(inputs)
+    assert int(bp.totals[8]) >= 1024
+    h64 = bp.run("fp64").clone()
+    h16 = bp.run("mixed16")
+    err = float(torch.linalg.vector_norm(h16 - h64) / torch.linalg.vector_norm(h64))
+    print(f"many intervals ({int(bp.totals[8])}): mixed16 vs fp64 {err:.3e}")
+    assert err < 1e-6
+    for b in (0, 11):
+        ref = oracle_waveform(inputs[b])
+        assert oracle.rel_l2(bp.waveform(b).cpu().numpy(), ref) < 1e-6
+
+
+def test_batch_config3_full_size(gpu):
+    """The 256-waveform uneven batch at its real size (durations U(0.1, 2) yr at
+    dt = 10 s, knots U{50..200}, U{50..3000} of the 3843 modes; 8.3e8 samples):
+    mixed16 against the FP64 control on every sample, and 8 waveforms spread over
+    the batch against the oracle on spread samples."""
+    import torch
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.pipeline import BatchProblem
+    from helpers import spread_indices
+    inputs = synthetic.make_batch(256)
+    bp = BatchProblem(inputs)
+    assert bp.supported16 and bp.samples > 8e8
+    h16 = bp.run("mixed16").clone()
+    h64 = bp.run("fp64")
+    err = float(torch.linalg.vector_norm(h16 - h64) / torch.linalg.vector_norm(h64))
+    print(f"config 3: {bp.samples} samples, mixed16 vs fp64 (all samples) {err:.3e}")
+    assert err < 1e-6
+    for b in np.linspace(0, 255, 8).astype(int):
+        x = inputs[b]
+        idx = spread_indices(x.num_pts, count=200, seed=int(b))
+        o = int(bp.offsets[b, 6]) // 2
+        got = h16[o:o + x.num_pts][torch.from_numpy(idx).to(h16.device)].cpu().numpy()
+        e = oracle.rel_l2(got, oracle_waveform(x, idx))
+        print(f"  waveform {b}: K={x.K} L={x.L} N={x.num_pts} rel-L2 vs oracle {e:.3e}")
+        assert e < 1e-6
+    del h16
+    torch.cuda.empty_cache()

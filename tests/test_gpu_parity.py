@@ -113,6 +113,43 @@ def test_dropin_get_waveform_numpy(gpu):
     assert oracle.rel_l2(wave, ref) <= FP64_TOL
 
 
+@pytest.mark.parametrize("precision", ["mixed16", "mixed"])
+def test_dropin_get_waveform_numpy_mixed(gpu, precision):
+    """The reference-shaped wrappers driven exactly as InterpolatedModeSum.sum drives
+    them (interpolatedmodesum.py:326-375), numpy arrays, workload cpu0, with the
+    backend's precision set to a factorised sum: rel-L2 < 1e-6 against the oracle;
+    the mode plan is built once and reused by a second call."""
+    from fastemriwaveforms_amd import pyinterp, synthetic
+    from fastemriwaveforms_amd.backend import RocmBackend
+    inp = synthetic.make_workload("cpu0")
+    L, K = inp.L, inp.K
+    y_all = np.concatenate([inp.teuk.T.real, inp.teuk.T.imag])
+    interp = np.zeros((4, 2 * K, L))
+    interp[0] = y_all
+    interp = interp.flatten()
+    tt = np.tile(inp.t, (2 * K, 1)).flatten()
+    B, ud, dg, ld = (np.zeros(2 * K * L) for _ in range(4))
+    pyinterp.interpolate_arrays_wrap(tt, interp, 2 * K, L, B, ud, dg, ld)
+    interp_tl = np.transpose(interp.reshape(4, 2 * K, L), [0, 2, 1]).flatten()
+    _, pcoef = oracle_coefficients(inp)
+    ref = oracle_waveform(inp)
+    backend = RocmBackend()
+    before = backend.precision
+    backend.precision = precision
+    try:
+        pyinterp._plans.clear()
+        for _ in range(2):
+            wave = np.zeros(inp.num_pts, dtype=np.complex128)
+            pyinterp.get_waveform_wrap(wave, interp_tl, inp.t, pcoef.flatten(), inp.m, inp.k,
+                                       inp.n, L, inp.num_pts, K, inp.ylms, inp.dt, inp.t, 0)
+            err = oracle.rel_l2(wave, ref)
+            print(f"drop-in {precision}: rel-L2 vs oracle {err:.3e}")
+            assert err < 1e-6
+        assert len(pyinterp._plans) == 1
+    finally:
+        backend.precision = before
+
+
 def test_native_interpolated_mode_sum(gpu):
     """Native InterpolatedModeSum.__call__ (summation/base.py:66 + interpolatedmodesum.py:283)."""
     from fastemriwaveforms_amd import synthetic
@@ -297,3 +334,62 @@ def test_spline_shared_knots_bitwise(gpu, length):
         assert info == (3 if length == 3 else 0)
         got = out.cpu().numpy()
         assert np.array_equal(got, ref), (nsys, length)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed16"])
+def test_native_pad_output_odd_len(gpu, precision):
+    """SummationBase.__call__'s output sizing (summation/base.py:83-112): with
+    pad_output the waveform runs to T (zeros after the trajectory), odd_len makes
+    the length odd; the summed samples equal the oracle's."""
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.summation import InterpolatedModeSum
+    inp = synthetic.small_input(seed=15, L=16, K=80, T_years=0.003)
+    _, pcoef = oracle_coefficients(inp)
+    dopr = np.transpose(pcoef, (1, 2, 0))
+    ylms = inp.ylms.copy()
+    ylms[inp.K:] *= (-1.0) ** inp.l
+    ref = oracle_waveform(inp)
+    T = 1.7 * inp.T_years  # observation longer than the trajectory
+    for pad, odd in [(True, False), (True, True), (False, True)]:
+        summ = InterpolatedModeSum(precision=precision, pad_output=pad, odd_len=odd)
+        h = summ(inp.t, inp.teuk, ylms, inp.t, dopr, inp.l, inp.m, inp.k, inp.n, T=T,
+                 dt=inp.dt).cpu().numpy()
+        npts, npad = oracle.num_points(inp.t[0], inp.t[-1], T, inp.dt, pad_output=pad,
+                                       odd_len=odd)
+        assert npts == inp.num_pts and summ.num_pts == inp.num_pts
+        assert len(h) == npts + npad
+        if odd:
+            assert len(h) % 2 == 1
+        if pad:
+            assert len(h) > inp.num_pts + 1000
+        err = oracle.rel_l2(h[:inp.num_pts], ref)
+        assert err <= (FP64_TOL if precision == "fp64" else 1e-6), (pad, odd, err)
+        assert np.count_nonzero(h[inp.num_pts:]) == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed16"])
+def test_native_integrate_backwards(gpu, precision):
+    """integrate_backwards (interpolatedmodesum.py:341-345): amplitude and phase knots
+    shifted by t[-1] - int(t[-1] / dt) dt before the sum, against the oracle on the
+    same shifted knots."""
+    from fastemriwaveforms_amd import synthetic
+    from fastemriwaveforms_amd.summation import InterpolatedModeSum
+    inp = synthetic.small_input(seed=17, L=14, K=60, T_years=0.003)
+    interp_tl, pcoef = oracle_coefficients(inp)
+    dopr = np.transpose(pcoef, (1, 2, 0))
+    ylms = inp.ylms.copy()
+    ylms[inp.K:] *= (-1.0) ** inp.l
+    offset = inp.t[-1] - int(inp.t[-1] / inp.dt) * inp.dt
+    assert offset > 0
+    ts = inp.t - offset
+    ref = oracle.get_waveform(interp_tl, ts, pcoef, inp.m, inp.k, inp.n, inp.L, inp.num_pts, inp.K,
+                              inp.ylms, inp.dt, ts)
+    summ = InterpolatedModeSum(precision=precision)
+    h = summ(inp.t, inp.teuk, ylms, inp.t, dopr, inp.l, inp.m, inp.k, inp.n, T=inp.T_years,
+             dt=inp.dt, integrate_backwards=True).cpu().numpy()
+    err = oracle.rel_l2(h, ref)
+    print(f"integrate_backwards {precision}: rel-L2 {err:.3e}")
+    assert err <= (FP64_TOL if precision == "fp64" else 1e-6)
+    # the unshifted sum differs: the shift matters
+    plain = oracle_waveform(inp)
+    assert oracle.rel_l2(plain, ref) > 1e-6
