@@ -494,10 +494,14 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
     teuk = torch.from_numpy(np.ascontiguousarray(inp.teuk)).to(dev)
     pk = torch.from_numpy(np.ascontiguousarray(inp.phase_knots)).to(dev)
     rng = np.random.default_rng(2024)
-    # fp64: FP64 forward + FP64 adjoint; mixed16: split-FP16 MFMA forward + MFMA-moment adjoint
-    engines = {p: DifferentiableModeSum(inp.t, inp.m, inp.k, inp.n, inp.ylms, inp.dt, inp.num_pts,
-                                        precision=p, device=dev, backward=p)
-               for p in ("fp64", "mixed16")}
+    # (forward, backward) pairs: "fp64" = the per-mode FP64 adjoint (the reference's
+    # arithmetic, differentiated); "fact64" = the factorised FP64 moments on the FP64
+    # matrix cores; "mixed16" = the split-FP16 moments (approximate phase gradients)
+    pairs = {"fp64": ("fp64", "fp64"), "fp64+fact64": ("fp64", "fact64"),
+             "mixed16+fact64": ("mixed16", "fact64"), "mixed16": ("mixed16", "mixed16")}
+    engines = {name: DifferentiableModeSum(inp.t, inp.m, inp.k, inp.n, inp.ylms, inp.dt,
+                                           inp.num_pts, precision=f, device=dev, backward=b)
+               for name, (f, b) in pairs.items()}
     with torch.no_grad():
         h0 = engines["fp64"](teuk, pk)
         sig = 0.1 * float(torch.sqrt(torch.mean(torch.abs(h0) ** 2)))
@@ -523,8 +527,8 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
         bwd = float(np.mean([b.elapsed_time(c) for _, b, c in ev[1:]]))
         grads[prec] = (t_ri.grad.detach().clone(), p_k.grad.detach().clone())
         out[prec] = {"fwd_ms": fwd, "bwd_ms": bwd, "bwd_over_fwd": bwd / fwd,
-                     "backward_precision": prec}
-        log(f"[bench] adjoint {prec} forward: fwd {fwd:.2f} ms, bwd {bwd:.2f} ms, "
+                     "forward_precision": pairs[prec][0], "backward_precision": pairs[prec][1]}
+        log(f"[bench] adjoint {prec}: fwd {fwd:.2f} ms, bwd {bwd:.2f} ms, "
             f"bwd/fwd {bwd / fwd:.2f}")
     # FD check: FP64 central differences of the FP64 GPU forward (equal to the oracle to ~4e-16)
     eng = engines["fp64"]
@@ -537,15 +541,20 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
         return float(torch.sum(d.real * wave.real + d.imag * wave.imag))
 
     g_t, g_p = grads["fp64"]
-    ga = float(torch.linalg.vector_norm(grads["mixed16"][0] - g_t) / torch.linalg.vector_norm(g_t))
-    gph = float(torch.linalg.vector_norm(grads["mixed16"][1] - g_p) / torch.linalg.vector_norm(g_p))
-    out["mixed16"]["grad_rel_l2_vs_fp64"] = {"amplitude_knots": ga, "phase_knots": gph}
-    log(f"[bench] adjoint mixed16 gradient rel-L2 vs fp64: amplitudes {ga:.2e}, phases {gph:.2e}")
+    for name in pairs:
+        if name == "fp64":
+            continue
+        ga = float(torch.linalg.vector_norm(grads[name][0] - g_t) / torch.linalg.vector_norm(g_t))
+        gph = float(torch.linalg.vector_norm(grads[name][1] - g_p) / torch.linalg.vector_norm(g_p))
+        out[name]["grad_rel_l2_vs_fp64"] = {"amplitude_knots": ga, "phase_knots": gph}
+        log(f"[bench] adjoint {name} gradient rel-L2 vs fp64: amplitudes {ga:.2e}, "
+            f"phases {gph:.2e}")
     if fd_coords <= 0:
         return out
     base_t = torch.view_as_real(teuk).contiguous()
     amp_scale = float(torch.abs(teuk).max())
-    ad, fd = [], []
+    ad, fd, ad_f = [], [], []
+    g_tf, g_pf = grads["mixed16+fact64"]
     for c in range(fd_coords):
         if c < fd_coords * 3 // 4:   # amplitude knot (linear: step relative to max |A|)
             l, mo, part = int(rng.integers(inp.L)), int(rng.integers(inp.K)), int(rng.integers(2))
@@ -557,6 +566,7 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
             tm[l, mo, part] -= step
             fd.append((loss_at(tp, pk) - loss_at(tm, pk)) / (2 * step))
             ad.append(float(g_t[l, mo, part]))
+            ad_f.append(float(g_tf[l, mo, part]))
         else:                        # phase knot (Phi_phi or Phi_r): Richardson of 5-point stencils
             q, l = int(rng.choice([0, 2])), int(rng.integers(inp.L))
 
@@ -570,7 +580,8 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
 
             fd.append((16 * d5(5e-4) - d5(1e-3)) / 15)
             ad.append(float(g_p[q, l]))
-    ad, fd = np.array(ad), np.array(fd)
+            ad_f.append(float(g_pf[q, l]))
+    ad, fd, ad_f = np.array(ad), np.array(fd), np.array(ad_f)
     per = np.abs(ad - fd) / np.maximum(np.abs(fd), 1e-300)
     n_amp = fd_coords * 3 // 4
     # per coordinate, relative to the scale of its gradient array (a single knot can
@@ -589,14 +600,16 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
                                 "linear in them); phases: Richardson extrapolation of 5-point "
                                 "stencils at 1e-3 and 5e-4 rad (FP64 rounding of 1e5-1e7 rad "
                                 "phases floors a 1e-6 rad central difference at ~1e-3)",
-                       "function": "FP64 GPU forward (rel-L2 ~4e-16 vs the oracle)"}
+                       "function": "FP64 GPU forward (rel-L2 ~4e-16 vs the oracle)",
+                       "mixed16_fact64_rel_l2": float(np.linalg.norm(ad_f - fd) / np.linalg.norm(fd))}
     log(f"[bench] adjoint FD check ({fd_coords} coords): rel-L2 {out['fd_check']['rel_l2']:.2e}, "
-        f"max per-coordinate {per.max():.2e}, median {np.median(per):.2e}")
+        f"max per-coordinate {per.max():.2e}, median {np.median(per):.2e}; mixed16+fact64 "
+        f"rel-L2 {out['fd_check']['mixed16_fact64_rel_l2']:.2e}")
     # gradient throughput: 16 full-size waveforms (own knots and amplitudes), fwd + bwd each
     if batch <= 0:
         return out
     batch_inputs = [synthetic.make_workload("full1", seed=5000 + b) for b in range(batch)]
-    for prec in ("fp64", "mixed16"):
+    for prec in pairs:
         eng = engines[prec]
         dev_in = [(torch.from_numpy(x.t).to(dev), torch.view_as_real(torch.from_numpy(
             np.ascontiguousarray(x.teuk)).to(dev)).clone().requires_grad_(True),

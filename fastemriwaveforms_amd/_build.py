@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 ARCH = os.environ.get("FEW_ROCM_ARCH", "gfx950")
 
 SOURCES = ["capi.hip", "spline.hip", "modesum_direct.hip", "modesum_fact.hip", "modesum_fact16.hip", "batch.hip",
-           "adjoint.hip", "adjoint16.hip",
+           "adjoint.hip", "adjoint16.hip", "adjoint64.hip",
            "eval.hip", "fd.hip", "amp2d.hip",
            "aak.hip"]
 

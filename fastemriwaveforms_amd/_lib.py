@@ -15,6 +15,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libfew_rocm.so")
 
 FEW_PRECISION_FP64 = 0
+FEW_PRECISION_FACT64 = 1
 FEW_PRECISION_MIXED = 2
 FEW_PRECISION_MIXED16 = 3
 

@@ -15,9 +15,14 @@ the forward's evaluation is stored), then
    tridiagonal solve) back to dL/d(knot values), amplitudes and phases.
 
 Step 1 runs in FP64 (``backward="fp64"``, the exact derivative of the FP64
-sum) or in mixed precision (``backward="mixed16"``: the per-interval moments
-sum_i sigma^s conj(g_i) u_r v_n on split-FP16 MFMA, finished per mode in
-FP64); step 2 is always FP64.  Inputs are real views: amplitudes as (L, K, 2) float64
+sum, per mode on the vector units), as the factorised FP64 moments
+(``backward="fact64"``: the per-interval moments sum_i sigma^s conj(g_i) u_r
+v_n with FP64 operands and accumulation on the matrix cores, finished per
+mode in FP64 -- FP64-accurate gradients at a fraction of the per-mode cost)
+or in mixed precision (``backward="mixed16"``: the same moments on
+split-FP16 MFMA with FP32 accumulation -- amplitude gradients to ~4e-7, but
+the phase gradients, remainders of sums that cancel by up to ~1e4, only to
+~1e-4); step 2 is always FP64.  Inputs are real views: amplitudes as (L, K, 2) float64
 (``torch.view_as_real`` of the complex (L, K) knot values), phases as (3, L)
 float64 (Phi_phi, Phi_theta, Phi_r).  The waveform comes back as (N, 2)
 float64; :meth:`DifferentiableModeSum.__call__` wraps it as complex.
@@ -31,6 +36,10 @@ import torch
 from . import _lib
 from ._lib import check, lib
 from .pipeline import PRECISION_CODES, ModePlan, _stream
+
+
+_BACKWARD_CODES = {"fp64": _lib.FEW_PRECISION_FP64, "fact64": _lib.FEW_PRECISION_FACT64,
+                   "mixed16": _lib.FEW_PRECISION_MIXED16}
 
 
 class DifferentiableModeSum:
@@ -58,8 +67,8 @@ class DifferentiableModeSum:
         if precision not in PRECISION_CODES:
             raise ValueError(f"precision must be one of {sorted(PRECISION_CODES)}")
         self.precision = precision
-        if backward not in ("fp64", "mixed16"):
-            raise ValueError("backward must be 'fp64' or 'mixed16'")
+        if backward not in _BACKWARD_CODES:
+            raise ValueError(f"backward must be one of {sorted(_BACKWARD_CODES)}")
         self.backward = backward
         self.variant = 0
         L, K = self.L, self.K
@@ -79,9 +88,9 @@ class DifferentiableModeSum:
         fwd = self.plan.workspace_bytes(L, self.N, precision)
         adj = int(lib().few_adjoint_workspace_bytes(L, self.N, K, _lib.FEW_PRECISION_FP64))
         adj16 = 0
-        if backward == "mixed16":
+        if backward != "fp64":
             adj16 = int(lib().few_plan_adjoint_workspace_bytes(self.plan.handle, L, self.N,
-                                                               _lib.FEW_PRECISION_MIXED16))
+                                                               _BACKWARD_CODES[backward]))
         self.ws = torch.empty(max(fwd, adj, adj16), dtype=torch.uint8, device=dev)
 
     # -- device stages -------------------------------------------------------
@@ -105,7 +114,7 @@ class DifferentiableModeSum:
                 grad_phase: torch.Tensor | None) -> None:
         """grad_wave (N,) complex128 -> grad_teuk (L, K, 2), grad_phase (3, L) (or None).
         The splines in self.interp / self.pcoef must be those of the knot values."""
-        prec = PRECISION_CODES[self.backward]
+        prec = _BACKWARD_CODES[self.backward]
         rc = lib().few_waveform_adjoint_plan_async(
             grad_wave.data_ptr(), self.interp.data_ptr(), self.t.data_ptr(),
             self.pcoef.data_ptr(), self.plan.handle, self.m.data_ptr(), self.k.data_ptr(),

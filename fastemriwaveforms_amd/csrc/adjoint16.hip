@@ -446,12 +446,15 @@ int launch_adjoint16(const Adj16Args &a, int max_chunks, hipStream_t stream)
     return FEW_OK;
 }
 
-int launch_adjoint16_finish(const Adj16Args &a, const Adj16FinishArgs &f, hipStream_t stream)
+int launch_adjoint16_finish(const Adj16Args &a, const Adj16FinishArgs &f, hipStream_t stream,
+                            bool reduce_chunks)
 {
     const long tile = f.tile;
-    hipLaunchKernelGGL(adjoint16_reduce_kernel, dim3((unsigned)((tile + 255) / 256), a.nseg), dim3(256),
-                       0, stream, a.part, a.prefix, a.nseg, tile, f.M);
-    FEW_HIP(hipGetLastError());
+    if (reduce_chunks && a.nseg > 0) {
+        hipLaunchKernelGGL(adjoint16_reduce_kernel, dim3((unsigned)((tile + 255) / 256), a.nseg),
+                           dim3(256), 0, stream, a.part, a.prefix, a.nseg, tile, f.M);
+        FEW_HIP(hipGetLastError());
+    }
     const int groups = (f.plan.K + 255) / 256;
     hipLaunchKernelGGL(adjoint16_finish_kernel, dim3(groups, f.L), dim3(256), 0, stream, f);
     FEW_HIP(hipGetLastError());

@@ -428,6 +428,27 @@ static Adj16Layout adj16_layout(const FactPlanHost &h, int L, int out_len)
     return a;
 }
 
+// FP64 factorised adjoint: the split-FP16 adjoint's shapes, FP64 chunk moments.
+static Adj16Layout adj64_layout(const FactPlanHost &h, int L, int out_len)
+{
+    Adj16Layout a;
+    a.RBM = (2 * A16_MOM * h.Nr + 15) / 16;
+    a.NBC = (h.Nc + 15) / 16;
+    a.groups = (h.K + 255) / 256;
+    a.tile = (long)a.RBM * 16 * a.NBC * 16;
+    a.max_chunks = ((long)out_len + A64_CHUNK - 1) / A64_CHUNK + (L > 1 ? L - 1 : 1);
+    size_t off = 0;
+    a.segs = off; off = align256(off + sizeof(Segment) * (size_t)(L > 1 ? L - 1 : 1));
+    a.prefix = off; off = align256(off + sizeof(int) * (size_t)L);
+    a.part = off; off = align256(off + sizeof(double) * (size_t)a.max_chunks * a.tile);
+    a.M = off; off = align256(off + sizeof(double) * (size_t)(L > 1 ? L - 1 : 1) * a.tile);
+    a.phase = off; off = align256(off + sizeof(double) * (size_t)L * a.groups * 12);
+    a.total = off;
+    a.ok = h.ok && a.RBM >= 1 && a.RBM <= 24 && a.NBC >= 1 && a.NBC <= 8 &&
+           adjoint64_lds_bytes(a.RBM, a.NBC) <= 160 * 1024;
+    return a;
+}
+
 
 }  // namespace few
 
@@ -778,10 +799,12 @@ int few_waveform_adjoint_async(const void *grad_wave, const double *interp_array
 size_t few_plan_adjoint_workspace_bytes(const void *plan, int init_len, int out_len, int precision)
 {
     if (init_len < 2 || out_len < 0) return 0;
-    if (precision != FEW_PRECISION_MIXED16 || !plan)
+    if ((precision != FEW_PRECISION_MIXED16 && precision != FEW_PRECISION_FACT64) || !plan)
         return adj_layout(init_len, out_len, plan ? static_cast<const FactPlan *>(plan)->host.K : 0)
             .total;
-    return adj16_layout(static_cast<const FactPlan *>(plan)->host, init_len, out_len).total;
+    const FactPlanHost &h = static_cast<const FactPlan *>(plan)->host;
+    return (precision == FEW_PRECISION_FACT64 ? adj64_layout(h, init_len, out_len)
+                                              : adj16_layout(h, init_len, out_len)).total;
 }
 
 int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_array,
@@ -800,10 +823,11 @@ int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_
     int rc = check_mode_args(grad_wave, interp_array, phase_interp_coeffs, d_m, d_k, d_n, init_len,
                              out_len, num_teuk_modes, d_Ylms, delta_t);
     if (rc) return rc;
-    if (precision != FEW_PRECISION_MIXED16) {
+    if (precision != FEW_PRECISION_MIXED16 && precision != FEW_PRECISION_FACT64) {
         set_error("waveform_adjoint: precision %d is not available", precision);
         return FEW_ERR_UNSUPPORTED;
     }
+    const bool f64 = precision == FEW_PRECISION_FACT64;
     if (!plan || !phase_interp_t || !h_t || !g_interp || !g_phase || !workspace) {
         set_error("waveform_adjoint: null pointer argument");
         return FEW_ERR_ARG;
@@ -820,10 +844,10 @@ int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_
         return FEW_ERR_ARG;
     }
     const int L = init_len;
-    const Adj16Layout al = adj16_layout(ph, L, out_len);
+    const Adj16Layout al = f64 ? adj64_layout(ph, L, out_len) : adj16_layout(ph, L, out_len);
     if (!al.ok) {
-        set_error("waveform_adjoint: mode set too large for the mixed adjoint (%d rows, %d columns)",
-                  ph.Nr, ph.Nc);
+        set_error("waveform_adjoint: mode set too large for the factorised adjoint (%d rows, %d "
+                  "columns)", ph.Nr, ph.Nc);
         return FEW_ERR_UNSUPPORTED;
     }
     if (workspace_bytes < al.total) {
@@ -838,7 +862,8 @@ int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_
     char *ws = static_cast<char *>(workspace);
     Segment *segs = reinterpret_cast<Segment *>(ws + al.segs);
     int *prefix = reinterpret_cast<int *>(ws + al.prefix);
-    PlanArgs p{h_t, phase_interp_t, L, delta_t, out_len, A16_CHUNK, segs, prefix};
+    const int chunk = f64 ? A64_CHUNK : A16_CHUNK;
+    PlanArgs p{h_t, phase_interp_t, L, delta_t, out_len, chunk, segs, prefix};
     rc = launch_plan(p, st);
     if (rc) return rc;
     Adj16Args a;
@@ -848,13 +873,15 @@ int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_
     a.prefix = prefix;
     a.nseg = L - 1;
     a.dt = delta_t;
-    a.chunk = A16_CHUNK;
+    a.chunk = chunk;
     a.plan = pl->dev;
     a.RBM = al.RBM;
     a.NBC = al.NBC;
-    a.part = reinterpret_cast<float *>(ws + al.part);
+    a.part = f64 ? nullptr : reinterpret_cast<float *>(ws + al.part);
+    a.part64 = f64 ? reinterpret_cast<double *>(ws + al.part) : nullptr;
     if (out_len > 0) {
-        rc = launch_adjoint16(a, (int)al.max_chunks, st);
+        rc = f64 ? launch_adjoint64(a, (int)al.max_chunks, st)
+                 : launch_adjoint16(a, (int)al.max_chunks, st);
         if (rc) return rc;
     }
     Adj16FinishArgs f;
@@ -872,7 +899,12 @@ int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_
     f.g_interp = g_interp;
     f.g_phase = g_phase;
     f.part_phase = reinterpret_cast<double *>(ws + al.phase);
-    return launch_adjoint16_finish(a, f, st);
+    if (f64) {
+        rc = launch_adjoint64_reduce(a, al.tile, f.M, st);
+        if (rc) return rc;
+        return launch_adjoint16_finish(a, f, st, false);
+    }
+    return launch_adjoint16_finish(a, f, st, true);
 }
 
 int few_spline_adjoint_async(const double *t, const double *g_coef, long g_sys, long g_p,

@@ -27,6 +27,7 @@ extern "C" {
 #define FEW_ROCM_ABI_VERSION 1
 
 #define FEW_PRECISION_FP64 0   /* plain FP64 direct sum (reference arithmetic) */
+#define FEW_PRECISION_FACT64 1 /* factorised sum in FP64 on the matrix cores (adjoint) */
 #define FEW_PRECISION_MIXED 2  /* factorised MFMA sum: FP64 phases, FP32 MFMA, FP64 output */
 #define FEW_PRECISION_MIXED16 3 /* factorised sum on FP16 MFMA with split (hi+lo) operands */
 
@@ -226,8 +227,13 @@ int few_waveform_adjoint_async(const void *grad_wave, const double *interp_array
  * FEW_PRECISION_MIXED16 computes the per-interval moments
  * sum_i sigma^s conj(g_i) u_r v_n (s = 0..6, u_r v_n the factorised phasors)
  * on split-FP16 MFMA with FP32 accumulation per 1024-sample chunk, adds the
- * chunks in FP64 and finishes per mode in FP64.  The mixed path needs
- * phase_interp_t == h_t (one knot array, as InterpolatedModeSum passes it).
+ * chunks in FP64 and finishes per mode in FP64.  FEW_PRECISION_FACT64
+ * computes the same moments with FP64 operands and FP64 accumulation on the
+ * matrix cores (v_mfma_f64_16x16x4f64): FP64-accurate gradients, phase
+ * gradients included (their moments cancel by up to ~1e4, which the FP32
+ * accumulation of MIXED16 cannot resolve: ~1e-4 relative there).  Both
+ * factorised paths need phase_interp_t == h_t (one knot array, as
+ * InterpolatedModeSum passes it).
  * Workspace: few_plan_adjoint_workspace_bytes().
  */
 size_t few_plan_adjoint_workspace_bytes(const void *plan, int init_len, int out_len, int precision);
