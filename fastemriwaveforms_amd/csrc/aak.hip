@@ -33,172 +33,201 @@ namespace few {
 #define AAK_GPCINSEC 1.02927125054339e+17    // global.h:26
 #define AAK_AUSEC 499.00478383615643         // global.h:27
 
-// gpuAAK.cu:8-32
-__host__ __device__ inline double aak_j0(double x)
+// Bessel functions of the first kind as Numerical Recipes computes them
+// (Press et al., Numerical Recipes in C, 2nd ed., sec. 6.5), the routines of
+// gpuAAK.cu:8-118.  The fitted constants and the order of every operation are
+// those routines' (bit-for-bit agreement with the CPU reference needs them);
+// the code is organised around the three methods: a rational fit below
+// |x| = 8, Hankel's asymptotic form above, and recurrences in the order.
+
+// c[0] + y (c[1] + y (... + y c[N-1])): the nested order of the fits.
+template <int N>
+__host__ __device__ inline double nr_nested(const double (&c)[N], double y)
 {
-    double ax, z, xx, y, ans, ans1, ans2;
-    if ((ax = fabs(x)) < 8.0) {
-        y = x * x;
-        ans1 = 57568490574.0 + y * (-13362590354.0 + y * (651619640.7 + y * (-11214424.18 + y * (77392.33017 + y * (-184.9052456)))));
-        ans2 = 57568490411.0 + y * (1029532985.0 + y * (9494680.718 + y * (59272.64853 + y * (267.8532712 + y * 1.0))));
-        ans = ans1 / ans2;
-    } else {
-        z = 8.0 / ax;
-        y = z * z;
-        xx = ax - 0.785398164;
-        ans1 = 1.0 + y * (-0.1098628627e-2 + y * (0.2734510407e-4 + y * (-0.2073370639e-5 + y * 0.2093887211e-6)));
-        ans2 = -0.1562499995e-1 + y * (0.1430488765e-3 + y * (-0.6911147651e-5 + y * (0.7621095161e-6 - y * 0.934935152e-7)));
-        double sx, cx;
-        sincos(xx, &sx, &cx);
-        ans = sqrt(0.636619772 / ax) * (cx * ans1 - z * sx * ans2);
-    }
-    return ans;
+    double acc = c[N - 1];
+#pragma unroll
+    for (int i = N - 2; i >= 0; i--) acc = c[i] + y * acc;
+    return acc;
 }
 
-// gpuAAK.cu:34-60
-__host__ __device__ inline double aak_j1(double x)
+// Hankel's form for |x| >= 8: sqrt(2 / (pi x)) (cos(x - w) P - (8/x) sin(x - w) Q),
+// P and Q nested in y = (8/x)^2, w = pi/4 for order 0 and 3 pi/4 for order 1.
+template <int NP, int NQ>
+__host__ __device__ inline double nr_hankel(double ax, double w, const double (&pc)[NP],
+                                            const double (&qc)[NQ])
 {
-    double ax, z, xx, y, ans, ans1, ans2;
-    if ((ax = fabs(x)) < 8.0) {
-        y = x * x;
-        ans1 = x * (72362614232.0 + y * (-7895059235.0 + y * (242396853.1 + y * (-2972611.439 + y * (15704.48260 + y * (-30.16036606))))));
-        ans2 = 144725228442.0 + y * (2300535178.0 + y * (18583304.74 + y * (99447.43394 + y * (376.9991397 + y * 1.0))));
-        ans = ans1 / ans2;
-    } else {
-        z = 8.0 / ax;
-        y = z * z;
-        xx = ax - 2.356194491;
-        ans1 = 1.0 + y * (0.183105e-2 + y * (-0.3516396496e-4 + y * (0.2457520174e-5 + y * (-0.240337019e-6))));
-        ans2 = 0.04687499995 + y * (-0.2002690873e-3 + y * (0.8449199096e-5 + y * (-0.88228987e-6 + y * 0.105787412e-6)));
-        double sx, cx;
-        sincos(xx, &sx, &cx);
-        ans = sqrt(0.636619772 / ax) * (cx * ans1 - z * sx * ans2);
-        if (x < 0.0) ans = -ans;
-    }
-    return ans;
+    const double z = 8.0 / ax;
+    const double y = z * z;
+    const double P = nr_nested(pc, y);
+    const double Q = nr_nested(qc, y);
+    double sn, cs;
+    sincos(ax - w, &sn, &cs);
+    return sqrt(0.636619772 / ax) * (cs * P - z * sn * Q);
 }
 
-// gpuAAK.cu:62-118: forward recurrence above the order, Miller's backward
-// recurrence normalised by the sum rule below it
-__host__ __device__ inline double aak_jn(int n, double x)
+// J0 (gpuAAK.cu:8-32)
+__host__ __device__ inline double nr_j0(double x)
 {
-    int j, jsum, m;
-    double ax, bj, bjm, bjp, sum, tox, ans;
-    ax = fabs(x);
-    if (n == 0) return aak_j0(ax);
-    if (n == 1) return aak_j1(ax);
+    const double ax = fabs(x);
+    if (ax < 8.0) {
+        const double num[6] = {57568490574.0, -13362590354.0, 651619640.7, -11214424.18, 77392.33017,
+                               -184.9052456};
+        const double den[6] = {57568490411.0, 1029532985.0, 9494680.718, 59272.64853, 267.8532712, 1.0};
+        const double y = x * x;
+        return nr_nested(num, y) / nr_nested(den, y);
+    }
+    const double pc[5] = {1.0, -0.1098628627e-2, 0.2734510407e-4, -0.2073370639e-5, 0.2093887211e-6};
+    const double qc[5] = {-0.1562499995e-1, 0.1430488765e-3, -0.6911147651e-5, 0.7621095161e-6,
+                          -0.934935152e-7};
+    return nr_hankel(ax, 0.785398164, pc, qc);
+}
+
+// J1 (gpuAAK.cu:34-60), odd in x
+__host__ __device__ inline double nr_j1(double x)
+{
+    const double ax = fabs(x);
+    if (ax < 8.0) {
+        const double num[6] = {72362614232.0, -7895059235.0, 242396853.1, -2972611.439, 15704.48260,
+                               -30.16036606};
+        const double den[6] = {144725228442.0, 2300535178.0, 18583304.74, 99447.43394, 376.9991397, 1.0};
+        const double y = x * x;
+        return x * nr_nested(num, y) / nr_nested(den, y);
+    }
+    const double pc[5] = {1.0, 0.183105e-2, -0.3516396496e-4, 0.2457520174e-5, -0.240337019e-6};
+    const double qc[5] = {0.04687499995, -0.2002690873e-3, 0.8449199096e-5, -0.88228987e-6, 0.105787412e-6};
+    const double v = nr_hankel(ax, 2.356194491, pc, qc);
+    return x < 0.0 ? -v : v;
+}
+
+// J_n(ax), ax > n: the upward recurrence J_{k+1} = (2k/x) J_k - J_{k-1} from
+// J0 and J1, stable while the order stays below the argument.
+__host__ __device__ inline double nr_jn_upward(int n, double ax)
+{
+    const double two_over_x = 2.0 / ax;
+    double lower = nr_j0(ax), upper = nr_j1(ax);
+    for (int k = 1; k < n; k++) {
+        const double next = k * two_over_x * upper - lower;
+        lower = upper;
+        upper = next;
+    }
+    return upper;
+}
+
+// Miller's algorithm: J_n(ax), 0 < ax <= n.  The downward recurrence runs
+// from the even start 2 floor((n + sqrt(40 n)) / 2) with arbitrary seeds,
+// the terms are rescaled by 1e-10 whenever one exceeds 1e10, and the value
+// at order n is normalised by 1 = J0 + 2 (J2 + J4 + ...).
+__host__ __device__ inline int nr_miller_start(int n)
+{
+    return 2 * ((n + (int)sqrt(40.0 * n)) / 2);
+}
+
+__host__ __device__ inline double nr_jn_miller(int n, double ax)
+{
+    const double two_over_x = 2.0 / ax;
+    double cur = 1.0, prev = 0.0, even_sum = 0.0, at_n = 0.0;
+    bool odd_step = false;  // the start is even, so every second term is an even order
+    for (int k = nr_miller_start(n); k > 0; k--) {
+        const double below = k * two_over_x * cur - prev;
+        prev = cur;
+        cur = below;
+        if (fabs(cur) > 1.0e10) {
+            cur *= 1.0e-10;
+            prev *= 1.0e-10;
+            at_n *= 1.0e-10;
+            even_sum *= 1.0e-10;
+        }
+        if (odd_step) even_sum += cur;
+        odd_step = !odd_step;
+        if (k == n) at_n = prev;
+    }
+    return at_n / (2.0 * even_sum - cur);
+}
+
+// J_n(x) (gpuAAK.cu:62-118).  As there, orders 0 and 1 return J(|x|), and
+// the sign for negative x is applied only to odd n >= 2 (n % 2 == 1).
+__host__ __device__ inline double nr_jn(int n, double x)
+{
+    const double ax = fabs(x);
+    if (n == 0) return nr_j0(ax);
+    if (n == 1) return nr_j1(ax);
     if (ax == 0.0) return 0.0;
-    if (ax > (double)n) {
-        tox = 2.0 / ax;
-        bjm = aak_j0(ax);
-        bj = aak_j1(ax);
-        for (j = 1; j < n; j++) {
-            bjp = j * tox * bj - bjm;
-            bjm = bj;
-            bj = bjp;
-        }
-        ans = bj;
-    } else {
-        tox = 2.0 / ax;
-        m = 2 * ((n + (int)sqrt(40.0 * n)) / 2);
-        jsum = 0;
-        bjp = ans = sum = 0.0;
-        bj = 1.0;
-        for (j = m; j > 0; j--) {
-            bjm = j * tox * bj - bjp;
-            bjp = bj;
-            bj = bjm;
-            if (fabs(bj) > 1.0e10) {
-                bj *= 1.0e-10;
-                bjp *= 1.0e-10;
-                ans *= 1.0e-10;
-                sum *= 1.0e-10;
-            }
-            if (jsum) sum += bj;
-            jsum = !jsum;
-            if (j == n) ans = bjp;
-        }
-        sum = 2.0 * sum - bj;
-        ans /= sum;
-    }
-    return x < 0.0 && n % 2 == 1 ? -ans : ans;
+    const double v = ax > (double)n ? nr_jn_upward(n, ax) : nr_jn_miller(n, ax);
+    return x < 0.0 && n % 2 == 1 ? -v : v;
 }
 
-// One step j of Miller's backward recurrence (aak_jn's x <= n branch) for
-// the chains LO..4 of aak_jn5: each chain's operations and their order are
-// aak_jn's (update, renormalisation, the sum over even terms -- j odd, as m
-// is even -- and the capture at j = order).
+// One step k of nr_jn_miller for the chains LO..4 of nr_jn5 (each chain's
+// operations in nr_jn_miller's order: the step, the rescaling, the even-order
+// sum -- k odd, as every start is even -- and the capture at k = order).
 template <int LO>
-__host__ __device__ inline void aak_miller_step(int j, double tox, int nu0, double (&bj)[5],
-                                                double (&bjp)[5], double (&sum)[5],
-                                                double (&ans)[5])
+__host__ __device__ inline void nr_miller_step(int k, double two_over_x, int nu0, double (&cur)[5],
+                                               double (&prev)[5], double (&even_sum)[5],
+                                               double (&at_n)[5])
 {
-    const double jt = j * tox;
+    const double kt = k * two_over_x;
     bool big = false;
 #pragma unroll
     for (int i = LO; i < 5; i++) {
-        const double bjm = jt * bj[i] - bjp[i];
-        bjp[i] = bj[i];
-        bj[i] = bjm;
-        big = big || fabs(bjm) > 1.0e10;
+        const double below = kt * cur[i] - prev[i];
+        prev[i] = cur[i];
+        cur[i] = below;
+        big = big || fabs(below) > 1.0e10;
     }
     if (big) {
 #pragma unroll
         for (int i = LO; i < 5; i++) {
-            if (fabs(bj[i]) > 1.0e10) {
-                bj[i] *= 1.0e-10;
-                bjp[i] *= 1.0e-10;
-                ans[i] *= 1.0e-10;
-                sum[i] *= 1.0e-10;
+            if (fabs(cur[i]) > 1.0e10) {
+                cur[i] *= 1.0e-10;
+                prev[i] *= 1.0e-10;
+                at_n[i] *= 1.0e-10;
+                even_sum[i] *= 1.0e-10;
             }
         }
     }
-    if (j & 1) {
+    if (k & 1) {
 #pragma unroll
-        for (int i = LO; i < 5; i++) sum[i] += bj[i];
+        for (int i = LO; i < 5; i++) even_sum[i] += cur[i];
     }
 #pragma unroll
     for (int i = LO; i < 5; i++)
-        if (j == nu0 + i) ans[i] = bjp[i];
+        if (k == nu0 + i) at_n[i] = prev[i];
 }
 
-// aak_jn(nu0 + i, x), i = 0..4, for x >= 0 (the five Bessel values of one
-// harmonic), bit for bit.  Miller's recurrences of the five orders run in
-// lockstep (independent chains, instruction-level parallelism; the start
-// m(order) grows with the order, so chain i joins at j = m_i); orders 0 and
-// 1, x = 0 and x > order take aak_jn's other branches (the lockstep chain of
-// such an order runs and is discarded).
-__host__ __device__ inline void aak_jn5(int nu0, double x, double (&out)[5])
+// nr_jn(nu0 + i, x), i = 0..4, for x >= 0 (the five Bessel values of one
+// harmonic), bit for bit.  The five Miller recurrences run in lockstep
+// (independent chains, instruction-level parallelism; the start grows with
+// the order, so chain i joins at k = its start); orders 0 and 1, x = 0 and
+// x > order take nr_jn's other branches (the lockstep chain of such an order
+// runs and is discarded).
+__host__ __device__ inline void nr_jn5(int nu0, double x, double (&out)[5])
 {
     const double ax = fabs(x);
-    const double tox = 2.0 / ax;
-    int m[5];
+    const double two_over_x = 2.0 / ax;
+    int start[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
         const int nu = nu0 + i;
-        m[i] = nu >= 2 ? 2 * ((nu + (int)sqrt(40.0 * nu)) / 2) : 0;
+        start[i] = nu >= 2 ? nr_miller_start(nu) : 0;
     }
-    double bj[5], bjp[5], sum[5], ans[5];
+    double cur[5], prev[5], even_sum[5], at_n[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-        bj[i] = 1.0;
-        bjp[i] = ans[i] = sum[i] = 0.0;
+        cur[i] = 1.0;
+        prev[i] = at_n[i] = even_sum[i] = 0.0;
     }
-    int j = m[4];
-    for (; j > m[3]; j--) aak_miller_step<4>(j, tox, nu0, bj, bjp, sum, ans);
-    for (; j > m[2]; j--) aak_miller_step<3>(j, tox, nu0, bj, bjp, sum, ans);
-    for (; j > m[1]; j--) aak_miller_step<2>(j, tox, nu0, bj, bjp, sum, ans);
-    for (; j > m[0]; j--) aak_miller_step<1>(j, tox, nu0, bj, bjp, sum, ans);
-    for (; j > 0; j--) aak_miller_step<0>(j, tox, nu0, bj, bjp, sum, ans);
+    int k = start[4];
+    for (; k > start[3]; k--) nr_miller_step<4>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
+    for (; k > start[2]; k--) nr_miller_step<3>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
+    for (; k > start[1]; k--) nr_miller_step<2>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
+    for (; k > start[0]; k--) nr_miller_step<1>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
+    for (; k > 0; k--) nr_miller_step<0>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
 #pragma unroll
     for (int i = 0; i < 5; i++) {
         const int nu = nu0 + i;
-        if (nu >= 2 && ax != 0.0 && !(ax > (double)nu)) {
-            const double sm = 2.0 * sum[i] - bj[i];
-            out[i] = ans[i] / sm;
-        } else {
-            out[i] = aak_jn(nu, x);
-        }
+        if (nu >= 2 && ax != 0.0 && !(ax > (double)nu))
+            out[i] = at_n[i] / (2.0 * even_sum[i] - cur[i]);
+        else
+            out[i] = nr_jn(nu, x);
     }
 }
 
@@ -359,8 +388,8 @@ __host__ __device__ inline double2 aak_sample(const AakArgs &a, const double (*s
         }
         const double ne = n * e;
         double Jv[5];
-        aak_jn5(n - 2, ne, Jv);
-        const double J0 = (n == 1) ? -1.0 * aak_j1(ne) : Jv[0];
+        nr_jn5(n - 2, ne, Jv);
+        const double J0 = (n == 1) ? -1.0 * nr_j1(ne) : Jv[0];
         const double J1 = Jv[1], J2 = Jv[2], J3 = Jv[3], J4 = Jv[4];
         double sn, cn;
         sincos(nPhi, &sn, &cn);

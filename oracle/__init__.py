@@ -108,6 +108,8 @@ def lib() -> ctypes.CDLL:
         _lib.orc_bessel_jn.restype = ctypes.c_double
         _lib.orc_bessel_j1_export.argtypes = [ctypes.c_double]
         _lib.orc_bessel_j1_export.restype = ctypes.c_double
+        _lib.orc_aak_harmonic.argtypes = [ctypes.c_int, ctypes.c_double, d, d, d]
+        _lib.orc_aak_harmonic.restype = None
         _lib.orc_aak_rot_coeff.argtypes = [d, d, d, d, d, d] + [ctypes.c_double] * 6
         _lib.orc_aak_rot_coeff.restype = None
         _lib.orc_aak_start_inds.argtypes = [i, d, ctypes.c_double, ctypes.c_int, ctypes.c_int]
@@ -276,6 +278,14 @@ def bessel_jn(n, x):
     if n == -1:
         return -lib().orc_bessel_j1_export(float(x))
     return lib().orc_bessel_jn(int(n), float(x))
+
+
+def aak_harmonic(n, e):
+    """gpuAAK.cu:421-446: the Bessel brackets (X, Y, J_n) of harmonic n at
+    eccentricity e, as the waveform uses them."""
+    out = [np.zeros(1) for _ in range(3)]
+    lib().orc_aak_harmonic(int(n), float(e), *(_dp(o) for o in out))
+    return tuple(float(o[0]) for o in out)
 
 
 def aak_rot_coeff(iota, theta_S, phi_S, theta_K, phi_K, alpha):

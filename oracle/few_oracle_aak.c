@@ -191,6 +191,27 @@ static double orc_dense_dot(const double *r, double s, double s2, double s3, dou
            r[7] * (4. * s3 - 15. * s4 + 18. * s5 - 7. * s6);
 }
 
+/* gpuAAK.cu:421-446 -- the Bessel brackets of harmonic n (Peters & Mathews
+ * 1963; Barack & Cutler 2004, eq. 9), with J = J_k(n e):
+ *   X = J_{n-2} - 2e J_{n-1} + (2/n) J_n + 2e J_{n+1} - J_{n+2},
+ *   Y = J_{n-2} - 2 J_n + J_{n+2},   Jn = J_n;
+ * the harmonic's a, b, c are -n Amp X cos(n Phi), -n Amp sqrt(1-e^2) Y sin(n Phi)
+ * and 2 Amp Jn cos(n Phi).  J_{-1} = -J_1 at n = 1. */
+void orc_aak_harmonic(int n, double e, double *X, double *Y, double *Jn)
+{
+    const double ne = n * e;
+    double J0, J1, J2, J3, J4;
+    if (n == 1) J0 = -1.0 * orc_bessel_j1(ne);
+    else J0 = orc_bessel_jn(n - 2, ne);
+    J1 = orc_bessel_jn(n - 1, ne);
+    J2 = orc_bessel_jn(n, ne);
+    J3 = orc_bessel_jn(n + 1, ne);
+    J4 = orc_bessel_jn(n + 2, ne);
+    *X = J0 - 2. * e * J1 + 2. / n * J2 + 2. * e * J3 - J4;
+    *Y = J0 - 2. * J2 + J4;
+    *Jn = J2;
+}
+
 /* gpuAAK.cu:181-452 (CPU branch) for samples [start_ind, end_ind) of
  * segment old_ind: overwrites waveform[i] = hI - i hII.  interp_array is
  * (8, init_length - 1, 6) (coefficient, segment, parameter p e Y Phi_phi
@@ -301,19 +322,13 @@ static void orc_aak_segment(double *waveform, const double *interp_array, double
             } else {
                 nPhi = n * Phi;
             }
-            const double ne = n * e;
-            double J0, J1, J2, J3, J4;
-            if (n == 1) J0 = -1.0 * orc_bessel_j1(ne);
-            else J0 = orc_bessel_jn(n - 2, ne);
-            J1 = orc_bessel_jn(n - 1, ne);
-            J2 = orc_bessel_jn(n, ne);
-            J3 = orc_bessel_jn(n + 1, ne);
-            J4 = orc_bessel_jn(n + 2, ne);
+            double X, Y, Jn;
+            orc_aak_harmonic(n, e, &X, &Y, &Jn);
             double snp, cnp;
             sincos(nPhi, &snp, &cnp);
-            const double a = -n * Amp * (J0 - 2. * e * J1 + 2. / n * J2 + 2. * e * J3 - J4) * cnp;
-            const double b = -n * Amp * sqrt(1 - e * e) * (J0 - 2. * J2 + J4) * snp;
-            const double c = 2. * Amp * J2 * cnp;
+            const double a = -n * Amp * X * cnp;
+            const double b = -n * Amp * sqrt(1 - e * e) * Y * snp;
+            const double c = 2. * Amp * Jn * cnp;
             double Aplus = -(1. + Ldotn2) * (a * cos2gam - b * sin2gam) + c * (1 - Ldotn2);
             double Acros = 2. * Ldotn * (b * cos2gam + a * sin2gam);
             const double Aplusold = Aplus, Acrosold = Acros;

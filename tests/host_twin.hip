@@ -291,6 +291,18 @@ int twin_aak_waveform(double *wave, const double *interp, double M, double S, do
     return 0;
 }
 
+// The kernel's Bessel values: nr_jn(order, x) into out1 and the lockstep
+// nr_jn5(order, x) (orders order..order+4) into out5, per (order, x) pair.
+void twin_bessel(const int *order, const double *x, long n, double *out1, double *out5)
+{
+    for (long i = 0; i < n; i++) {
+        out1[i] = nr_jn(order[i], x[i]);
+        double v[5];
+        nr_jn5(order[i], x[i], v);
+        for (int j = 0; j < 5; j++) out5[5 * i + j] = v[j];
+    }
+}
+
 void twin_fd_cube_roots(const double *abcd, long n, double *out)
 {
     const fcx xi = fd_xi();

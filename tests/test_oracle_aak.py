@@ -8,6 +8,8 @@ gpuAAK.cu's get_waveform) and of the kernel's per-sample code against it
 * the rotation coefficients against vector identities computed with numpy
   (cos = (L.S - (n.L)(n.S)) / (|n x L||n x S|), sin = 2 n.(S x L) / ...);
 * the segment plan against known answers (gpuAAK.cu:458-483);
+* the eccentric harmonics' Bessel brackets against the Peters-Mathews power
+  per harmonic (scipy's jv) and summed over harmonics (closed form f(e));
 * the harmonic sum in the circular limit (e = 0): only the n = 2 harmonic is
   non-zero, so 2 harmonics and 12 give the same waveform; the amplitude
   scales as mu / dist and as M^(2/3).
@@ -46,6 +48,39 @@ def test_bessel_special_values():
     assert oracle.bessel_jn(5, 0.0) == 0.0
     assert abs(oracle.bessel_jn(-1, 0.3) + sp.jv(1, 0.3)) < 1e-8   # gpuAAK's J0 at n = 1
     assert abs(oracle.bessel_jn(3, -0.7) + oracle.bessel_jn(3, 0.7)) == 0.0   # odd order, odd
+
+
+@pytest.mark.parametrize("e", [0.05, 0.3, 0.5, 0.7])
+def test_harmonics_peters_mathews_power(e):
+    """The eccentric harmonic brackets the waveform is built from carry the
+    Peters-Mathews power: with alpha = n X, beta = n sqrt(1-e^2) Y, gamma = 2 J_n,
+    g(n, e) = n^2/32 (alpha^2 + beta^2 + gamma^2/3) (Peters & Mathews 1963), each
+    g(n, e) equals the closed form in scipy's Bessel functions, and the sum over
+    harmonics is f(e) = (1 + 73/24 e^2 + 37/96 e^4) / (1 - e^2)^(7/2)."""
+    total = 0.0
+    for n in range(1, 121):
+        X, Y, Jn = oracle.aak_harmonic(n, e)
+        g = n * n / 32.0 * ((n * X) ** 2 + n * n * (1 - e * e) * Y ** 2 + (2 * Jn) ** 2 / 3.0)
+        J = lambda k: sp.jv(k, n * e)
+        g_ref = n ** 4 / 32.0 * ((J(n - 2) - 2 * e * J(n - 1) + 2.0 / n * J(n) + 2 * e * J(n + 1)
+                                  - J(n + 2)) ** 2
+                                 + (1 - e * e) * (J(n - 2) - 2 * J(n) + J(n + 2)) ** 2
+                                 + 4.0 / (3 * n * n) * J(n) ** 2)
+        # the routines' J_2, J_3 carry up to 3e-7 (Miller's start is not
+        # converged for the lowest orders), which n^2 X^2 turns into ~1e-6
+        assert abs(g - g_ref) <= 5e-6 * max(g_ref, 1e-12), (n, g, g_ref)
+        total += g
+    f = (1 + 73 / 24 * e ** 2 + 37 / 96 * e ** 4) / (1 - e * e) ** 3.5
+    assert total == pytest.approx(f, rel=2e-6)
+
+
+def test_harmonics_circular_limit():
+    """e = 0: only n = 2 radiates; its brackets are X = Y = J0(0) = 1 (to the
+    rational fit's 3e-9 at 0) and J_2(0) = 0, so g(2, 0) = 1 = f(0)."""
+    X, Y, Jn = oracle.aak_harmonic(2, 0.0)
+    assert X == Y and abs(X - 1.0) < 5e-9 and Jn == 0.0
+    for n in (1, 3, 4, 7):
+        assert oracle.aak_harmonic(n, 0.0) == (0.0, 0.0, 0.0)
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -111,6 +146,31 @@ def test_dense_rate_matches_finite_difference():
     hstep = 1e-5
     fd = (dense(s + hstep) - dense(s - hstep)) / (2 * hstep) / w
     assert rate == pytest.approx(fd, rel=1e-7)
+
+
+def test_twin_bessel_bitwise(twin):
+    """The kernel's Bessel routines (nr_jn and the lockstep nr_jn5), run on the
+    host, equal the oracle's bit for bit over the orders and arguments the sum
+    uses (both recurrence branches, orders 0 and 1, x = 0)."""
+    twin.twin_bessel.argtypes = [P, P, ctypes.c_long, P, P]
+    rng = np.random.default_rng(7)
+    orders, xs = [], []
+    for n in range(1, 60):
+        for x in np.concatenate([[0.0], rng.uniform(0, 0.99, 6) * n, [n * 0.999, n + 0.5]]):
+            orders.append(n - 2)
+            xs.append(x)
+    orders = np.array(orders, dtype=np.int32)
+    xs = np.array(xs)
+    out1 = np.zeros(len(xs))
+    out5 = np.zeros((len(xs), 5))
+    twin.twin_bessel(_p(orders), _p(xs), len(xs), _p(out1), _p(out5))
+    for i, (o, x) in enumerate(zip(orders, xs)):
+        ref = [oracle.bessel_jn(o + j, x) for j in range(5)]
+        if o >= 0:
+            assert out1[i] == ref[0], (o, x)
+        assert list(out5[i][1:]) == ref[1:], (o, x)
+        if o >= 0:
+            assert out5[i][0] == ref[0], (o, x)
 
 
 def test_twin_aak_bitwise(twin):

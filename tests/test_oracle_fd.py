@@ -7,7 +7,8 @@ the kernel's per-bin code against it (host twin).
   the reference's own series, measured per range of x;
 * cube_roots against numpy.roots;
 * the whole sum, as test_fd.py checks the reference (tests/test_fd.py:70-140):
-  the overlap of h+ with the FFT of the time-domain waveform's real part;
+  the overlap of h+ with the FFT of the time-domain waveform's real part, and
+  its absolute scale: |h+|^2 = dt^2 |FFT|^2 band by band;
 * the kernel's item/bin routines, run on the host in the oracle's loop order,
   equal the oracle bit for bit.
 """
@@ -85,6 +86,34 @@ def test_fd_overlap_with_fft_of_td():
     fd = hp[mask]
     ac = np.vdot(ts, fd) / np.sqrt(np.vdot(ts, ts) * np.vdot(fd, fd))
     assert 1 - ac.real < 1e-2, 1 - ac.real
+
+
+def test_fd_absolute_scale_vs_fft_of_td():
+    """The stationary-phase h+ is the continuous Fourier transform of the
+    time-domain real part: |h+(f)|^2 = dt^2 |FFT(Re h)(f)|^2 band by band
+    (20 bands; those carrying 5 % of the power or more, which hold most of it,
+    agree to 1 %; the total to 2 %).  The overlap check above cannot see a
+    constant factor in the amplitude; this one does (a factor 2, sqrt(2) or
+    2 pi moves the ratio far outside the bounds)."""
+    inp = _small(T=0.05, L=40, K=16, seed=11, chirp="rising")
+    h_td = oracle_waveform(inp)
+    npts = inp.num_pts + (1 - inp.num_pts % 2)
+    h_td = np.concatenate([h_td, np.zeros(npts - len(h_td))])
+    freq = oracle.fd_frequency_grid(len(h_td), 0, inp.dt)
+    wave, _ = oracle_fd(inp, freq)
+    hp = oracle.fd_post(wave, freq)[0]
+    mask = freq >= 0.0
+    p_td = np.abs(np.fft.fftshift(np.fft.fft(np.real(h_td)))[mask]) ** 2 * inp.dt ** 2
+    p_fd = np.abs(hp[mask]) ** 2
+    assert p_fd.sum() / p_td.sum() == pytest.approx(1.0, abs=0.02)
+    edges = np.linspace(0, len(p_td), 21).astype(int)
+    covered = 0.0
+    for a, b in zip(edges[:-1], edges[1:]):
+        share = p_td[a:b].sum() / p_td.sum()
+        if share >= 0.05:
+            covered += share
+            assert p_fd[a:b].sum() / p_td[a:b].sum() == pytest.approx(1.0, abs=0.01), (a, b)
+    assert covered > 0.8
 
 
 def test_fd_segment_inds_tile_the_band():
