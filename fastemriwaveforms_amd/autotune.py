@@ -14,7 +14,12 @@ import os
 
 import torch
 
-CANDIDATES = [("mixed16", 0), ("mixed16", 4), ("mixed", 0), ("mixed", 4)]
+# mixed16 variants: 0 / 4 = 8 / 4 waves per workgroup; 16 / 20 = the same with
+# the row-streamed contraction (taken where the rows are arithmetic and the
+# plan has at most 8 column steps, else the regular kernel runs)
+CANDIDATES = [("mixed16", 0), ("mixed16", 4), ("mixed16", 16), ("mixed16", 20), ("mixed", 0),
+              ("mixed", 4)]
+BATCH_VARIANTS = (0, 4, 16, 20)
 
 
 def cache_path() -> str:
@@ -100,7 +105,7 @@ def tune_batch(bp, reps: int = 2, use_cache: bool = True):
         bp.variant = cache[key]["variant"]
         return cache[key]
     timings = {}
-    for var in (0, 4):
+    for var in BATCH_VARIANTS:
         bp.variant = var
         bp.wave.zero_()
         bp.stages("mixed16", BATCH_STAGE_SPLINES | BATCH_STAGE_PLAN)

@@ -99,8 +99,8 @@ int launch_fact_prep(const FactPrepArgs &a, hipStream_t stream);
 // Tile of one interval: two FP16 planes (hi, lo), each KB x RB blocks of
 // 16 real rows x 32 halfs (block (kb, rb) at (kb * RB + rb) KiB), then 4
 // floats of inverse scales (one per power p), padded to 256 bytes.  The four 16-byte
-// chunks of a row are XOR-swizzled by (row >> 2) & 3, which puts the 16 rows
-// of a fragment read on distinct LDS banks without padding.
+// chunks of a row are XOR-swizzled by (row >> 1) & 3, which puts each group of
+// eight lanes of a fragment read on distinct LDS banks without padding.
 inline int fact16_kb(const FactPlanHost &p) { return (2 * p.Nc + 31) / 32; }
 inline size_t fact16_tile_bytes(const FactPlanHost &p)
 {
@@ -134,8 +134,13 @@ struct F16Sched {
     int prio;
 };
 // variant: 0 = 8 waves with priority, 4 = 4 waves with priority,
-// 8 = 8 waves without
+// 8 = 8 waves without; 16 / 20 = the row-streamed contraction (arithmetic
+// rows and at most F16S_MAX_KB column steps; otherwise variant 0 / 4) with
+// 8 / 4 waves, with priority
 inline F16Sched fact16_sched(int variant) { return F16Sched{variant == 8 ? 0 : 1}; }
+inline int fact16_waves(int variant) { return variant == 4 || variant == 20 ? 4 : 8; }
+inline bool fact16_streamed(int variant) { return variant == 16 || variant == 20; }
+constexpr int F16S_MAX_KB = 8;
 
 struct Fact16Args {
     double2 *wave;

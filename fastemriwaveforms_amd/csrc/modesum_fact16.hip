@@ -18,8 +18,13 @@
 //       C[R, a]; stored as KB x RB blocks of 16 rows x 32 halfs (kb-major,
 //       one KiB each) so that every fragment address inside the K loop is a
 //       lane base plus a compile-time offset; the four 16-byte chunks of a
-//       row are XOR-swizzled by (row >> 2) & 3 (distinct LDS banks for the 16
-//       rows a fragment read touches).  The A fragment of a lane is 8
+//       row are XOR-swizzled by (row >> 1) & 3.  A ds_read_b128 is served
+//       eight lanes at a time from 32 four-byte banks (128 bytes); rows are
+//       64 bytes, so the eight rows of such a lane group fall on two bank
+//       halves, and the swizzle gives the four row pairs four different
+//       chunks: eight distinct 16-byte bank slots, no conflict (the previous
+//       (row >> 2) & 3 swizzle left a 2-way conflict on every read, measured
+//       with SQ_LDS_BANK_CONFLICT).  The A fragment of a lane is 8
 //       consecutive halfs (one ds_read_b128);
 //   inv_sigma: 4 floats, 1/scale of the rows of power p = 0..3.
 
@@ -42,12 +47,12 @@ constexpr int F16_BLK = 512;
 __device__ __forceinline__ long f16_tile_off(int rho, int k, int RB)
 {
     const int rr = rho & 15;
-    const int chunk = ((k & 31) >> 3) ^ ((rr >> 2) & 3);
+    const int chunk = ((k & 31) >> 3) ^ ((rr >> 1) & 3);
     return ((long)((k >> 5) * RB + (rho >> 4)) * 16 + rr) * 32 + chunk * 8 + (k & 7);
 }
 
 // Lane part of a fragment read: row li of the block, 16-byte chunk g.
-__device__ __forceinline__ int f16_lane_off(int li, int g) { return li * 32 + ((g ^ ((li >> 2) & 3)) << 3); }
+__device__ __forceinline__ int f16_lane_off(int li, int g) { return li * 32 + ((g ^ ((li >> 1) & 3)) << 3); }
 
 #define INV_2PI16 0.15915494309189533577
 
@@ -308,6 +313,30 @@ constexpr int F16_NB = 2;        // 16-sample column blocks per wave
 // with the full register file, 8 waves two per SIMD.
 constexpr int f16_tile_samples(int waves) { return 16 * F16_NB * waves; }
 
+// (ar + i ai)(br + i bi) in FP32 with the fused operations spelled out, so
+// every kernel that inlines it rounds the same way.
+__device__ __forceinline__ void cmul_f32(float ar, float ai, float br, float bi, float &cr, float &ci)
+{
+    cr = fmaf(ar, br, -(ai * bi));
+    ci = fmaf(ar, bi, ai * br);
+}
+
+// v_n -> v_n z^16 for the next 32-column step (FP32 complex products).
+__device__ __forceinline__ void f16_advance(f16v_t (&vr)[F16_NB][4], f16v_t (&vi)[F16_NB][4],
+                                            const f16v_t (&zr16)[F16_NB],
+                                            const f16v_t (&zi16)[F16_NB])
+{
+#pragma unroll
+    for (int c = 0; c < F16_NB; c++)
+#pragma unroll
+        for (int t2 = 0; t2 < 4; t2++) {
+            f16v_t nr, ni;
+            cmul_f32(vr[c][t2], vi[c][t2], zr16[c], zi16[c], nr, ni);
+            vr[c][t2] = nr;
+            vi[c][t2] = ni;
+        }
+}
+
 // One 32-column step of the n-contraction: B fragments from v_n (then v_n
 // advanced by z^16 when another step follows), A fragments from the tile
 // block column at aH / aL, and the three split products per row block.
@@ -320,17 +349,7 @@ __device__ __forceinline__ void f16_kstep(const _Float16 *aH, const _Float16 *aL
     v8h bh[F16_NB], bl[F16_NB];
 #pragma unroll
     for (int c = 0; c < F16_NB; c++) split_pack(vr[c], vi[c], bh[c], bl[c]);
-    if (advance) {
-#pragma unroll
-        for (int c = 0; c < F16_NB; c++)
-#pragma unroll
-            for (int t2 = 0; t2 < 4; t2++) {
-                f16v_t nr, ni;
-                cmul(vr[c][t2], vi[c][t2], zr16[c], zi16[c], nr, ni);
-                vr[c][t2] = nr;
-                vi[c][t2] = ni;
-            }
-    }
+    if (advance) f16_advance(vr, vi, zr16, zi16);
     // A fragments of the next row block are loaded while the current one
     // feeds its MFMAs; the scheduling barrier keeps the loads from being
     // hoisted further (register pressure).
@@ -519,8 +538,8 @@ __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigne
             for (int rb = RB - 1; rb >= 0; rb--) {
                 // rows r >= Nr hold zeros (W = 0)
                 const v4f W = acc[rb][c];
-                const double zr = (double)(ka * W[0] + kb2 * W[2]);
-                const double zi = (double)(ka * W[1] + kb2 * W[3]);
+                const double zr = (double)fmaf(ka, W[0], kb2 * W[2]);
+                const double zi = (double)fmaf(ka, W[1], kb2 * W[3]);
                 const double nr = fma(hre, e_re, fma(-him, e_im, zr));
                 const double ni = fma(hre, e_im, fma(him, e_re, zi));
                 hre = nr;
@@ -577,6 +596,228 @@ __device__ __forceinline__ void f16_contract_prio(const unsigned char *sTile, in
     if (prio) __builtin_amdgcn_s_setprio(1);
     f16_contract<RB>(sTile, KB, V, acc);
     if (prio) __builtin_amdgcn_s_setprio(0);
+}
+
+// ------------------------------------------------ row-streamed variant
+// Arithmetic rows only.  The B fragments of all KB column steps are split
+// once per tile and held in registers (16 KB VGPRs); the row blocks are then
+// contracted one at a time, from the last to the first, and each finished
+// block is folded into the epilogue's Horner sum while the MFMAs of the next
+// block run (software pipelining by one row block).  The FP64 row sum thus
+// overlaps the contraction inside the wave, and only two row blocks of
+// accumulators are live.  Every product, accumulation and rounding is the
+// one of fact16_kernel (same MFMA chains per (row block, column block), same
+// fold order), so the two give identical results.
+template <int KB>
+struct F16Bfrag {
+    v8h bh[KB][F16_NB], bl[KB][F16_NB];
+};
+
+template <int KB>
+__device__ __forceinline__ void f16_split_all(F16V &V, F16Bfrag<KB> &Bf)
+{
+#pragma unroll
+    for (int kb = 0; kb < KB; kb++) {
+#pragma unroll
+        for (int c = 0; c < F16_NB; c++) split_pack(V.vr[c], V.vi[c], Bf.bh[kb][c], Bf.bl[kb][c]);
+        if (kb + 1 < KB) f16_advance(V.vr, V.vi, V.zr16, V.zi16);
+    }
+}
+
+// One row block: acc = W[rows of block, samples] over all KB column steps
+// (aH / aL: the block's A fragment address in column step 0; kstride: the
+// distance between column steps).
+template <int KB>
+__device__ __forceinline__ void f16_rows16(const _Float16 *aH, const _Float16 *aL, int kstride,
+                                           const F16Bfrag<KB> &Bf, v4f (&acc)[F16_NB])
+{
+#pragma unroll
+    for (int kb = 0; kb < KB; kb++) {
+        const v8h ah = *reinterpret_cast<const v8h *>(aH + kb * kstride);
+        const v8h al = *reinterpret_cast<const v8h *>(aL + kb * kstride);
+#pragma unroll
+        for (int c = 0; c < F16_NB; c++)
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                al, Bf.bh[kb][c], kb == 0 ? v4f{0.f, 0.f, 0.f, 0.f} : acc[c], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < F16_NB; c++)
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, Bf.bl[kb][c], acc[c], 0, 0, 0);
+#pragma unroll
+        for (int c = 0; c < F16_NB; c++)
+            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, Bf.bh[kb][c], acc[c], 0, 0, 0);
+    }
+}
+
+// One Horner step of the row sum with a finished row block (the
+// arithmetic-rows branch of f16_epilogue, per row block).
+struct F16Fold {
+    float ka[F16_NB], kb2[F16_NB];
+    double er[F16_NB], ei[F16_NB], hr[F16_NB], hi[F16_NB];
+};
+
+__device__ __forceinline__ void f16_fold(const v4f (&acc)[F16_NB], F16Fold &F)
+{
+#pragma unroll
+    for (int c = 0; c < F16_NB; c++) {
+        const v4f W = acc[c];
+        const double zr = (double)fmaf(F.ka[c], W[0], F.kb2[c] * W[2]);
+        const double zi = (double)fmaf(F.ka[c], W[1], F.kb2[c] * W[3]);
+        const double nr = fma(F.hr[c], F.er[c], fma(-F.hi[c], F.ei[c], zr));
+        const double ni = fma(F.hr[c], F.ei[c], fma(F.hi[c], F.er[c], zi));
+        F.hr[c] = nr;
+        F.hi[c] = ni;
+    }
+}
+
+// Scheduling of one (row block, fold of the previous block) pair: per column
+// step the two A-fragment reads of the next step, then the six MFMAs in pairs
+// with two VALU instructions (the fold's) after each pair, so the fold issues
+// in the MFMA shadow instead of after the block and the LDS reads one step
+// ahead.
+template <int KB>
+__device__ __forceinline__ void f16s_interleave()
+{
+    __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // DS reads of step 0
+#pragma unroll
+    for (int kb = 0; kb < KB; kb++) {
+        if (kb + 1 < KB) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);  // of step kb + 1
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, 2, 0);  // VALU
+        }
+    }
+}
+
+// One tile of the row-streamed variant: setup, B split, streamed rows with
+// the folded epilogue, output update.
+template <int KB>
+__device__ __forceinline__ void f16s_tile(const FactPlanDev &P, const unsigned char *sTile, int RB,
+                                          const double *sPh, const Segment &sg, double inv_pw,
+                                          double inv_hw, int base_i, double dt, double2 *wave,
+                                          bool overwrite, int prio)
+{
+    const int g = (threadIdx.x & 63) >> 4;
+    const int li = threadIdx.x & 15;
+    double2 wold[F16_NB];
+    F16V V;
+    f16_prefetch(sg, base_i, wave, wold, overwrite);
+    f16_setup<true>(P, sPh, sg, inv_pw, base_i, dt, V);
+    const int plane = KB * RB * F16_BLK;
+    const float *sInv = reinterpret_cast<const float *>(sTile + 4 * (long)plane);
+    const int p0 = 2 * (g & 1);
+    F16Fold F;
+#pragma unroll
+    for (int c = 0; c < F16_NB; c++) {
+        const double t = dt * (double)(base_i + c * 16);
+        const float s1f = (float)((t - sg.h_t) * inv_hw);
+        const float s2f = s1f * s1f;
+        const float spa = (p0 == 0 ? 1.f : s2f) * (1.0f / 256.0f);
+        const float spb = (p0 == 0 ? s1f : s2f * s1f) * (1.0f / 256.0f);
+        F.ka[c] = spa * sInv[p0];
+        F.kb2[c] = spb * sInv[p0 + 1];
+        F.er[c] = __shfl(V.pc[c], li + 16);
+        F.ei[c] = -__shfl(V.ps[c], li + 16);
+        F.hr[c] = 0.0;
+        F.hi[c] = 0.0;
+    }
+    F16Bfrag<KB> Bf;
+    f16_split_all<KB>(V, Bf);
+    const _Float16 *sHi = reinterpret_cast<const _Float16 *>(sTile);
+    const _Float16 *aH = sHi + f16_lane_off(li, g);
+    const _Float16 *aL = aH + plane;
+    const int kstride = RB * F16_BLK;
+    v4f accA[F16_NB], accB[F16_NB];
+    if (prio) __builtin_amdgcn_s_setprio(1);
+    int rb = RB - 1;
+    f16_rows16<KB>(aH + rb * F16_BLK, aL + rb * F16_BLK, kstride, Bf, accA);
+    while (rb >= 2) {
+        f16_rows16<KB>(aH + (rb - 1) * F16_BLK, aL + (rb - 1) * F16_BLK, kstride, Bf, accB);
+        f16_fold(accA, F);
+        f16s_interleave<KB>();
+        f16_rows16<KB>(aH + (rb - 2) * F16_BLK, aL + (rb - 2) * F16_BLK, kstride, Bf, accA);
+        f16_fold(accB, F);
+        f16s_interleave<KB>();
+        rb -= 2;
+    }
+    if (rb == 1) {
+        f16_rows16<KB>(aH, aL, kstride, Bf, accB);
+        f16_fold(accA, F);
+        f16s_interleave<KB>();
+        f16_fold(accB, F);
+    } else {
+        f16_fold(accA, F);
+    }
+    if (prio) __builtin_amdgcn_s_setprio(0);
+#pragma unroll
+    for (int c = 0; c < F16_NB; c++) {
+        const int ul = li + 32 + 16 * (g >> 1);
+        const double u_re = __shfl(V.pc[c], ul), u_im = -__shfl(V.ps[c], ul);
+        double hre, him;
+        cmul(u_re, u_im, F.hr[c], F.hi[c], hre, him);
+        hre += __shfl_xor(hre, 16);
+        him += __shfl_xor(him, 16);
+        hre += __shfl_xor(hre, 32);
+        him += __shfl_xor(him, 32);
+        wold[c].x += hre;
+        wold[c].y += him;
+    }
+#pragma unroll
+    for (int c = 0; c < F16_NB; c++) {
+        const int idx = base_i + c * 16;
+        if (g == 0 && idx < sg.end) wave[idx] = wold[c];
+    }
+}
+
+// Single waveform, row-streamed variant (fact16_kernel's work split and
+// tile loads; RB at run time, KB at compile time).
+template <int KB, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
+void fact16s_kernel(Fact16Args a, int RB)
+{
+    constexpr int NT = 64 * WAVES;
+    extern __shared__ __attribute__((aligned(16))) unsigned char sTile[];
+    const int tid = threadIdx.x;
+    double *sPh = reinterpret_cast<double *>(sTile + a.tile_bytes);
+    const int total = a.prefix[a.nseg];
+    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * per;
+    const int t1 = min(total, t0 + per);
+    if (a.overwrite) {
+        const long end = a.nseg > 0 ? a.segs[a.nseg - 1].end : 0;
+        for (long i = end + (long)blockIdx.x * NT + tid; i < a.out_len; i += (long)gridDim.x * NT)
+            a.wave[i] = make_double2(0.0, 0.0);
+    }
+    if (t0 >= t1) return;
+    int j = upper_index(a.prefix, a.nseg, t0);
+    int jnext_start = a.prefix[j + 1];
+    const long np = (long)a.nseg * 3;
+    bool stale = true;
+    Segment sgj;
+    double inv_pw = 1.0, inv_hw = 1.0;
+    for (int tile = t0; tile < t1; tile++) {
+        while (tile >= jnext_start) {
+            j++;
+            jnext_start = a.prefix[j + 1];
+            stale = true;
+        }
+        if (stale) {
+            sgj = a.segs[j];
+            inv_pw = 1.0 / sgj.pw;
+            inv_hw = 1.0 / sgj.hw;
+            __syncthreads();
+            const uint4 *src = reinterpret_cast<const uint4 *>(a.tiles + (long)j * a.tile_bytes);
+            uint4 *dst = reinterpret_cast<uint4 *>(sTile);
+            const int nv = (int)(a.tile_bytes >> 4);
+            for (int i = tid; i < nv; i += NT) dst[i] = src[i];
+            if (tid < 24) sPh[tid] = a.pcoef[(tid & 7) * np + (long)j * 3 + (tid >> 3)];
+            __syncthreads();
+            stale = false;
+        }
+        f16s_tile<KB>(a.plan, sTile, RB, sPh, sgj, inv_pw, inv_hw,
+                      f16_base<WAVES>(sgj, tile - a.prefix[j]), a.dt, a.wave, a.overwrite != 0,
+                      a.sched.prio);
+    }
 }
 
 // Single waveform, persistent: block b owns a contiguous range of the
@@ -711,6 +952,72 @@ void batch_fact16_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned 
     }
 }
 
+// Batch, row-streamed variant (batch_fact16_kernel's work split).
+template <int KB, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
+void batch_fact16s_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned char *tiles,
+                          long tile_bytes, F16Sched sched)
+{
+    constexpr int NT = 64 * WAVES;
+    extern __shared__ __attribute__((aligned(16))) unsigned char sTile[];
+    const int tid = threadIdx.x;
+    const int RB = B.uni.RB;
+    double *sPh = reinterpret_cast<double *>(sTile + tile_bytes);
+    const int total = r.wt_prefix[B.B];
+    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * per;
+    const int t1 = min(total, t0 + per);
+    if (r.overwrite) {
+        for (int wb = 0; wb < B.B; wb++) {
+            const WaveDesc dw = B.desc[wb];
+            const long end = dw.L > 1 ? r.segs[dw.seg_off + dw.L - 2].end : 0;
+            double2 *w = p.wave + dw.wave_off / 2;
+            for (long i = end + (long)blockIdx.x * NT + tid; i < dw.N; i += (long)gridDim.x * NT)
+                w[i] = make_double2(0.0, 0.0);
+        }
+    }
+    if (t0 >= t1) return;
+    int b = upper_index(r.wt_prefix, B.B, t0);
+    WaveDesc d = B.desc[b];
+    const int *pref = r.tpref + d.tpref_off;
+    int j = upper_index(pref, d.L - 1, t0 - r.wt_prefix[b]);
+    FactPlanDev P = batch_wave_plan(B, d);
+    bool stale = true;
+    Segment sgj;
+    double inv_pw = 1.0, inv_hw = 1.0;
+    for (int tile = t0; tile < t1; tile++) {
+        while (tile >= r.wt_prefix[b + 1]) {
+            b++;
+            d = B.desc[b];
+            pref = r.tpref + d.tpref_off;
+            P = batch_wave_plan(B, d);
+            j = 0;
+            stale = true;
+        }
+        const int lt = tile - r.wt_prefix[b];
+        while (lt >= pref[j + 1]) {
+            j++;
+            stale = true;
+        }
+        if (stale) {
+            sgj = r.segs[d.seg_off + j];
+            inv_pw = 1.0 / sgj.pw;
+            inv_hw = 1.0 / sgj.hw;
+            __syncthreads();
+            const uint4 *src = reinterpret_cast<const uint4 *>(tiles + (long)(d.seg_off + j) * tile_bytes);
+            uint4 *dst = reinterpret_cast<uint4 *>(sTile);
+            const int nv = (int)(tile_bytes >> 4);
+            for (int i = tid; i < nv; i += NT) dst[i] = src[i];
+            const long np = (long)(d.L - 1) * 3;
+            if (tid < 24) sPh[tid] = p.pcoef[d.pc_off + (tid & 7) * np + (long)j * 3 + (tid >> 3)];
+            __syncthreads();
+            stale = false;
+        }
+        f16s_tile<KB>(P, sTile, RB, sPh, sgj, inv_pw, inv_hw, f16_base<WAVES>(sgj, lt - pref[j]),
+                      d.dt, p.wave + d.wave_off / 2, r.overwrite != 0, sched.prio);
+    }
+}
+
 __global__ __launch_bounds__(F16_PREP_THREADS) void batch_fact16_prep_whole_kernel(
     BatchDev B, BatchPtrs p, unsigned char *tiles, long tile_bytes, int KB, int RB)
 {
@@ -748,7 +1055,7 @@ __global__ __launch_bounds__(64 * F16_PREP_ROWS) void batch_fact16_prep_kernel(
                        blockIdx.y, KB, RB, maxes + (long)gi * 8 * RB, tiles + (long)gi * tile_bytes);
 }
 
-int fact16_tile(int variant) { return 16 * F16_NB * (variant == 4 ? 4 : 8); }
+int fact16_tile(int variant) { return 16 * F16_NB * fact16_waves(variant); }
 
 static int device_cus16()
 {
@@ -786,6 +1093,27 @@ static int launch_fact16_w(const Fact16Args &a, int RB, int blocks, size_t lds, 
     return FEW_OK;
 }
 
+template <int WAVES>
+static int launch_fact16s_w(const Fact16Args &a, int RB, int blocks, size_t lds, hipStream_t stream)
+{
+    switch (a.KB) {
+#define FEW_CASE16S(X)                                                                           \
+    case X:                                                                                      \
+        FEW_LDS((fact16s_kernel<X, WAVES>), lds);                                               \
+        hipLaunchKernelGGL((fact16s_kernel<X, WAVES>), dim3(blocks), dim3(64 * WAVES), lds,     \
+                           stream, a, RB);                                                       \
+        break;
+        FEW_CASE16S(1) FEW_CASE16S(2) FEW_CASE16S(3) FEW_CASE16S(4) FEW_CASE16S(5)
+        FEW_CASE16S(6) FEW_CASE16S(7) FEW_CASE16S(8)
+#undef FEW_CASE16S
+    default:
+        set_error("mixed16 sum: %d column steps unsupported by the streamed variant", a.KB);
+        return FEW_ERR_UNSUPPORTED;
+    }
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
 int launch_fact16_sum(const Fact16Args &a_in, int RB, int max_blocks, int variant, hipStream_t stream)
 {
     if (max_blocks <= 0) return FEW_OK;
@@ -799,7 +1127,12 @@ int launch_fact16_sum(const Fact16Args &a_in, int RB, int max_blocks, int varian
     int blocks = device_cus16() * (lds <= 80 * 1024 ? 2 : 1);
     if (blocks > max_blocks) blocks = max_blocks;
     const bool arith = a.plan.rows_arith != 0;
-    if (variant == 4) {
+    const int waves = fact16_waves(variant);
+    if (fact16_streamed(variant) && arith && a.KB >= 1 && a.KB <= F16S_MAX_KB) {
+        return waves == 4 ? launch_fact16s_w<4>(a, RB, blocks, lds, stream)
+                          : launch_fact16s_w<8>(a, RB, blocks, lds, stream);
+    }
+    if (waves == 4) {
         return arith ? launch_fact16_w<4, true>(a, RB, blocks, lds, stream)
                      : launch_fact16_w<4, false>(a, RB, blocks, lds, stream);
     }
@@ -825,6 +1158,29 @@ static int launch_batch_fact16_w(const BatchDev &B, const BatchRunDev &r, const 
 #undef FEW_CASEB
     default:
         set_error("batch mixed16 sum: %d row blocks unsupported", B.uni.RB);
+        return FEW_ERR_UNSUPPORTED;
+    }
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
+template <int WAVES>
+static int launch_batch_fact16s_w(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
+                                  const unsigned char *tiles, long tile_bytes, int KB, int blocks,
+                                  size_t lds, F16Sched sched, hipStream_t stream)
+{
+    switch (KB) {
+#define FEW_CASEBS(X)                                                                            \
+    case X:                                                                                      \
+        FEW_LDS((batch_fact16s_kernel<X, WAVES>), lds);                                         \
+        hipLaunchKernelGGL((batch_fact16s_kernel<X, WAVES>), dim3(blocks), dim3(64 * WAVES),    \
+                           lds, stream, B, r, p, tiles, tile_bytes, sched);                      \
+        break;
+        FEW_CASEBS(1) FEW_CASEBS(2) FEW_CASEBS(3) FEW_CASEBS(4) FEW_CASEBS(5) FEW_CASEBS(6)
+        FEW_CASEBS(7) FEW_CASEBS(8)
+#undef FEW_CASEBS
+    default:
+        set_error("batch mixed16 sum: %d column steps unsupported by the streamed variant", KB);
         return FEW_ERR_UNSUPPORTED;
     }
     FEW_HIP(hipGetLastError());
@@ -863,7 +1219,11 @@ int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs
     const int blocks = device_cus16() * (lds <= 80 * 1024 ? 2 : 1);
     const bool arith = B.uni.rows_arith != 0;
     const F16Sched sched = fact16_sched(variant);
-    if (variant == 4)
+    const int waves = fact16_waves(variant);
+    if (fact16_streamed(variant) && arith && KB >= 1 && KB <= F16S_MAX_KB)
+        return waves == 4 ? launch_batch_fact16s_w<4>(B, rs, p, tiles, tile_bytes, KB, blocks, lds, sched, stream)
+                          : launch_batch_fact16s_w<8>(B, rs, p, tiles, tile_bytes, KB, blocks, lds, sched, stream);
+    if (waves == 4)
         return arith ? launch_batch_fact16_w<4, true>(B, rs, p, tiles, tile_bytes, KB, blocks, lds, sched, stream)
                      : launch_batch_fact16_w<4, false>(B, rs, p, tiles, tile_bytes, KB, blocks, lds, sched, stream);
     return arith ? launch_batch_fact16_w<8, true>(B, rs, p, tiles, tile_bytes, KB, blocks, lds, sched, stream)

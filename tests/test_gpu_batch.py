@@ -82,7 +82,7 @@ def test_batch_autotune(gpu, tmp_path, monkeypatch):
     monkeypatch.setenv("FEW_AMD_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
     bp = BatchProblem(_small_batch(seed=21, count=3))
     best = autotune.tune_batch(bp, reps=1)
-    assert best["variant"] in (0, 4) and best["ms"] > 0
+    assert best["variant"] in autotune.BATCH_VARIANTS and best["ms"] > 0
     again = autotune.tune_batch(bp, reps=1)  # served from the cache
     assert again == best
     ref = oracle_waveform(bp.inputs[0])
