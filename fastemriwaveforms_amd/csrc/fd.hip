@@ -15,12 +15,16 @@
 //     with T = -fdot/|fddot| (2/sqrt 3) SPA(x) / sqrt(x),
 //     x = 2 pi fdot^3 / (3 fddot^2) and SPA(x) = i sqrt(x) e^{-ix} K_1/3(-ix).
 //
-// Layout of the work: the (mode, segment) items are flattened, their in-range
-// bin counts prefix-summed on the device, and a persistent grid walks the
-// flattened bins; consecutive bins of a thread mostly share an item, whose
-// coefficients stay in registers.  Contributions are added with FP64 atomics
-// (global_atomic_add_f64), as the reference's kernel adds them with atomic
-// CAS loops: the summation order of colliding bins is not fixed.
+// Layout of the work (summed output): bin-owned.  Each thread owns a bin and
+// its mirror about the zero bin, a workgroup a tile of such owners; the
+// segments that reach the tile are found per mode by binary search over the
+// mode's segments sorted by first bin, and each thread adds every
+// contribution to its bins itself, in a fixed order (mode, sorted segment,
+// root): no atomics, and the result is the same on every run (the
+// reference's kernel adds with atomic CAS loops in no fixed order).  With
+// separate_modes every (mode, bin) has one writer already: the (mode,
+// segment) items are flattened, their in-range bin counts prefix-summed on
+// the device, and a persistent grid walks the flattened bins.
 //
 // Complex arithmetic follows gcmplx::complex (src/few/cutils/cuda_complex.hpp,
 // the libc++ <complex> algorithms) operation for operation, on the device's
@@ -28,6 +32,8 @@
 // C99 Annex G NaN recovery.  cos and sin of one argument are taken with one
 // sincos call, as an optimising compiler emits for gcmplx::exp / polar.
 #include <math.h>
+
+#include <algorithm>
 
 #include "fd.hpp"
 #include "few_common.hpp"
@@ -464,9 +470,9 @@ __global__ __launch_bounds__(FD_SCAN_BLOCK) void fd_scan3_kernel(const long *loc
 
 #define FD_THREADS 256
 
-// Persistent walk over the flattened bins: workgroup b owns a contiguous
-// range; each thread strides through it, reloading the item only when its
-// next bin belongs to another (mode, segment).  Capped at 256 registers, two
+// separate_modes: persistent walk over the flattened bins: workgroup b owns a
+// contiguous range; each thread strides through it, reloading the item only
+// when its next bin belongs to another (mode, segment).  Capped at 256 registers, two
 // waves per SIMD: the per-bin chains of FP64 libm calls are latency-bound,
 // and a second wave hides more of that than the few spills it costs
 // (measured on full1-sized FD sums: 133 ms against 164 ms at one wave).
@@ -502,35 +508,265 @@ void fd_sum_kernel(FdArgs a, long nitems)
         const int i = lo + (int)(w - item_begin);
         const double f = a.freqs[i];
         const int q = (int)(item / a.nseg);
-        const bool conj_term = (it.mm != 0.0) && a.include_minus_m;
-        const int mi = fd_minus_index(a.zero_index, i);
-        double2 *wave = a.wave;
-        const int sep = a.separate_modes;
-        const long sep_o = (long)q * a.data_length + i;
-        const int nf = a.data_length;
-        fd_bin(it, f, xi, kc, [&](fcx plus, fcx minus) {
-            if (sep) {
-                wave[sep_o] = make_double2(plus.re, plus.im);
-            } else {
-                double *p = reinterpret_cast<double *>(wave + i);
-                unsafeAtomicAdd(p, plus.re);
-                unsafeAtomicAdd(p + 1, plus.im);
-                if (conj_term && mi >= 0 && mi < nf) {
-                    double *pm = reinterpret_cast<double *>(wave + mi);
-                    unsafeAtomicAdd(pm, minus.re);
-                    unsafeAtomicAdd(pm + 1, minus.im);
-                }
-            }
-        });
+        double2 *out = a.wave + (long)q * a.data_length + i;
+        // one writer per (mode, bin); the -f terms are not part of this output
+        fd_bin(it, f, xi, kc, [&](fcx plus, fcx) { *out = make_double2(plus.re, plus.im); });
     }
 }
 
-size_t fd_workspace_bytes(int K, int nseg)
+// ------------------------------------------------ bin-owned accumulation
+// Each output bin b is written by exactly one thread, which adds every
+// contribution to it in a fixed order (no atomics, run-to-run identical).
+// The +f and -f terms of one (item, bin) evaluation go to bins i and 2z - i
+// (z = zero_index), so a thread owns a bin pair: owner o = b <= z with its
+// mirror b' = 2z - b ("low" owners), or a bin b > 2z with no mirror partner
+// ("high" owners).  A workgroup takes FD_OWN_TILE owners; for every mode it
+// finds the segments whose bin range meets the tile's bins or their mirrors
+// (each mode's segments sorted by first bin, with the running maximum of
+// their last bins, so two binary searches bound the candidates), and every
+// (item, bin) evaluation happens once, in the thread that owns the bin.
+
+#define FD_OWN_TILE 256  // threads per workgroup
+#define FD_OWN_BPT 8     // owners per thread
+
+struct FdOwnIndex {
+    int *cnt;   // (K,) segments with in-range bins
+    int *slo;   // (K, nseg) their first bins, ascending (ties by segment)
+    int *sj;    // (K, nseg) the segment of each sorted position
+    int *pmx;   // (K, nseg) running maximum of the last bins in that order
+};
+
+__host__ __device__ inline size_t fd_align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// One thread per mode: insertion sort of its non-empty segments by first bin.
+__global__ __launch_bounds__(256) void fd_mode_index_kernel(FdArgs a, FdOwnIndex ix)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= a.K) return;
+    const long base = (long)q * a.nseg;
+    int c = 0;
+    for (int j = 0; j < a.nseg; j++) {
+        int lo, hi;
+        fd_item_range(a, base + j, lo, hi);
+        if (hi < lo) continue;
+        int p = c;
+        while (p > 0 && ix.slo[base + p - 1] > lo) {
+            ix.slo[base + p] = ix.slo[base + p - 1];
+            ix.sj[base + p] = ix.sj[base + p - 1];
+            p--;
+        }
+        ix.slo[base + p] = lo;
+        ix.sj[base + p] = j;
+        c++;
+    }
+    int mx = -1;
+    for (int p = 0; p < c; p++) {
+        int lo, hi;
+        fd_item_range(a, base + ix.sj[base + p], lo, hi);
+        mx = max(mx, hi);
+        ix.pmx[base + p] = mx;
+    }
+    ix.cnt[q] = c;
+}
+
+// Sorted positions [s, e) of mode q that can meet bins [w0, w1]: first bin
+// <= w1 and running maximum of last bins >= w0 (e <= s: none).
+__device__ inline void fd_candidates(const FdOwnIndex &ix, long base, int c, int w0, int w1,
+                                     int &s, int &e)
+{
+    s = 0;
+    e = 0;
+    if (c == 0 || w1 < w0 || ix.slo[base] > w1 || ix.pmx[base + c - 1] < w0) return;
+    int l = 0, h = c;  // e: first position with slo > w1
+    while (l < h) {
+        const int mid = (l + h) >> 1;
+        if (ix.slo[base + mid] <= w1) l = mid + 1; else h = mid;
+    }
+    e = l;
+    l = 0;
+    h = e;  // s: first position with pmx >= w0
+    while (l < h) {
+        const int mid = (l + h) >> 1;
+        if (ix.pmx[base + mid] >= w0) h = mid; else l = mid + 1;
+    }
+    s = l;
+}
+
+__global__ __launch_bounds__(FD_OWN_TILE) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void fd_own_kernel(FdArgs a, FdOwnIndex ix, int n_low, int hb0, int tiles_low)
+{
+    constexpr int SPAN = FD_OWN_TILE * FD_OWN_BPT;  // owners per tile
+    __shared__ int s_q[2 * FD_OWN_TILE], s_s[2 * FD_OWN_TILE], s_e[2 * FD_OWN_TILE];
+    __shared__ int s_scan[FD_OWN_TILE];
+    // the thread's accumulators: [0] its bins, [1] their mirrors
+    __shared__ double2 s_acc[2][FD_OWN_BPT][FD_OWN_TILE];
+    const int tid = threadIdx.x;
+    const int z = a.zero_index, nf = a.data_length;
+    // Low tiles in descending order: the frequencies nearest zero, where most
+    // modes' bands overlap, carry the most work and start first.
+    const bool low = (int)blockIdx.x < tiles_low;
+    const int tile = low ? tiles_low - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    // this tile's bins [w0, w1] and their mirrors [m0, m1] (empty when m1 < m0)
+    int w0, w1, m0 = 0, m1 = -1;
+    if (low) {
+        w0 = tile * SPAN;
+        w1 = min(w0 + SPAN, n_low) - 1;
+        m0 = max(2 * z - w1, 0);
+        m1 = min(2 * z - w0, nf - 1);
+    } else {
+        w0 = hb0 + (tile - tiles_low) * SPAN;
+        w1 = min(w0 + SPAN, nf) - 1;
+    }
+    // owner k of this thread: bin w0 + k * FD_OWN_TILE + tid and (low tiles) its mirror
+#pragma unroll
+    for (int k = 0; k < FD_OWN_BPT; k++) {
+        s_acc[0][k][tid] = make_double2(0.0, 0.0);
+        s_acc[1][k][tid] = make_double2(0.0, 0.0);
+    }
+    const fcx xi = fd_xi();
+    const FdSpaConst kc = fd_spa_const();
+    const bool minus_on = a.include_minus_m != 0;
+    for (int q0 = 0; q0 < a.K; q0 += FD_OWN_TILE) {
+        // candidate segment ranges of mode q0 + tid, compacted in mode order
+        const int q = q0 + tid;
+        int sa = 0, ea = 0, sb = 0, eb = 0;
+        if (q < a.K) {
+            const long base = (long)q * a.nseg;
+            const int c = ix.cnt[q];
+            fd_candidates(ix, base, c, w0, w1, sa, ea);
+            fd_candidates(ix, base, c, m0, m1, sb, eb);
+            if (ea > sa && eb > sb && sb <= ea && sa <= eb) {  // overlapping: one range
+                sa = min(sa, sb);
+                ea = max(ea, eb);
+                eb = sb;
+            }
+        }
+        const int na = (ea > sa ? 1 : 0) + (eb > sb ? 1 : 0);
+        s_scan[tid] = na;
+        __syncthreads();
+        for (int off = 1; off < FD_OWN_TILE; off <<= 1) {
+            const int v = tid >= off ? s_scan[tid - off] : 0;
+            __syncthreads();
+            s_scan[tid] += v;
+            __syncthreads();
+        }
+        int pos = s_scan[tid] - na;
+        const int nent = s_scan[FD_OWN_TILE - 1];
+        if (ea > sa) {
+            s_q[pos] = q;
+            s_s[pos] = sa;
+            s_e[pos] = ea;
+            pos++;
+        }
+        if (eb > sb) {
+            s_q[pos] = q;
+            s_s[pos] = sb;
+            s_e[pos] = eb;
+        }
+        __syncthreads();
+        // The items of the list in order; each wave skips the items none of
+        // its bins meet.  A thread evaluates up to 2 FD_OWN_BPT bins per item
+        // (its bins and their mirrors), so one item load serves several
+        // evaluations, all through one fd_bin body (a second inlined copy of
+        // the large evaluation costs instruction-cache misses).
+        for (int en = 0; en < nent; en++) {
+            const int qe = __builtin_amdgcn_readfirstlane(s_q[en]);
+            const int ps = __builtin_amdgcn_readfirstlane(s_s[en]);
+            const int pe = __builtin_amdgcn_readfirstlane(s_e[en]);
+            const long base = (long)qe * a.nseg;
+            const bool conj_term = minus_on && a.m[qe] != 0;
+            for (int p = ps; p < pe; p++) {
+                const int j = __builtin_amdgcn_readfirstlane(ix.sj[base + p]);
+                int lo, hi;
+                fd_item_range(a, base + j, lo, hi);
+                unsigned mask = 0;  // bit 2k: bin of owner k, bit 2k + 1: its mirror
+#pragma unroll
+                for (int k = 0; k < FD_OWN_BPT; k++) {
+                    const int bk = w0 + k * FD_OWN_TILE + tid;
+                    if (bk <= w1) {
+                        if (bk >= lo && bk <= hi) mask |= 1u << (2 * k);
+                        const int mk = 2 * z - bk;
+                        if (low && mk != bk && mk < nf && mk >= lo && mk <= hi)
+                            mask |= 2u << (2 * k);
+                    }
+                }
+                if (!__any(mask != 0)) continue;
+                FdItem it;
+                fd_load_item(a, qe, j, it);
+#pragma nounroll
+                for (int slot = 0; slot < 2 * FD_OWN_BPT; slot++) {
+                    const bool act = (mask >> slot) & 1u;
+                    if (!__any(act)) continue;
+                    if (act) {
+                        const int k = slot >> 1;
+                        const bool mirror = slot & 1;
+                        const int bk = w0 + k * FD_OWN_TILE + tid;
+                        const int mk = 2 * z - bk;
+                        // mirror == 0: +f to bk, -f to 2z - bk; mirror == 1: +f to the
+                        // mirror bin, -f back to bk
+                        fd_bin(it, a.freqs[mirror ? mk : bk], xi, kc, [&](fcx plus, fcx minus) {
+                            double2 &tp = s_acc[mirror ? 1 : 0][k][tid];
+                            tp.x += plus.re;
+                            tp.y += plus.im;
+                            if (conj_term) {
+                                if (mirror || mk == bk) {
+                                    double2 &tm = s_acc[0][k][tid];
+                                    tm.x += minus.re;
+                                    tm.y += minus.im;
+                                } else if (low && mk >= 0 && mk < nf) {
+                                    double2 &tm = s_acc[1][k][tid];
+                                    tm.x += minus.re;
+                                    tm.y += minus.im;
+                                }
+                            }
+                        });
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < FD_OWN_BPT; k++) {
+        const int bk = w0 + k * FD_OWN_TILE + tid;
+        if (bk > w1) continue;
+        const double2 ab = s_acc[0][k][tid];
+        const double2 w = a.wave[bk];
+        a.wave[bk] = make_double2(w.x + ab.x, w.y + ab.y);
+        const int mk = 2 * z - bk;
+        if (low && mk != bk && mk < nf) {
+            const double2 am = s_acc[1][k][tid];
+            const double2 wm = a.wave[mk];
+            a.wave[mk] = make_double2(wm.x + am.x, wm.y + am.y);
+        }
+    }
+}
+
+// Owners and tiles of the bin-owned sum.
+inline void fd_owner_split(int z, int nf, int &n_low, int &hb0, int &tiles_low, int &tiles)
+{
+    n_low = z >= 0 ? min(z, nf - 1) + 1 : 0;
+    hb0 = (int)std::max(2L * z + 1, 0L);
+    const int n_high = hb0 < nf ? nf - hb0 : 0;
+    const int span = FD_OWN_TILE * FD_OWN_BPT;
+    tiles_low = (n_low + span - 1) / span;
+    tiles = tiles_low + (n_high + span - 1) / span;
+}
+
+static size_t fd_scan_ws_bytes(int K, int nseg)
 {
     const long nitems = (long)K * nseg;
     const long nb = (nitems + FD_SCAN_BLOCK - 1) / FD_SCAN_BLOCK;
     // local (nitems) + prefix (nitems + 1) + block sums (nb) + total
     return sizeof(long) * (size_t)(2 * nitems + 1 + nb + 1);
+}
+
+size_t fd_workspace_bytes(int K, int nseg)
+{
+    const size_t nitems = (size_t)K * nseg;
+    // the item-walk scan (separate_modes) and the bin-owned index share nothing
+    return fd_align256(fd_scan_ws_bytes(K, nseg)) + fd_align256(sizeof(int) * (size_t)K) +
+           3 * fd_align256(sizeof(int) * nitems);
 }
 
 int launch_fd_sum(FdArgs a, void *workspace, size_t workspace_bytes, hipStream_t stream)
@@ -546,6 +782,29 @@ int launch_fd_sum(FdArgs a, void *workspace, size_t workspace_bytes, hipStream_t
     if (nb > 0x7fffffffL) {
         set_error("fd_sum: too many (mode, segment) items (%ld)", nitems);
         return FEW_ERR_LIMIT;
+    }
+    if (!a.separate_modes) {
+        // bin-owned: every bin written once, contributions in a fixed order
+        char *w = static_cast<char *>(workspace) + fd_align256(fd_scan_ws_bytes(a.K, a.nseg));
+        FdOwnIndex ix;
+        ix.cnt = reinterpret_cast<int *>(w);
+        w += fd_align256(sizeof(int) * (size_t)a.K);
+        ix.slo = reinterpret_cast<int *>(w);
+        w += fd_align256(sizeof(int) * (size_t)nitems);
+        ix.sj = reinterpret_cast<int *>(w);
+        w += fd_align256(sizeof(int) * (size_t)nitems);
+        ix.pmx = reinterpret_cast<int *>(w);
+        hipLaunchKernelGGL(fd_mode_index_kernel, dim3((a.K + 255) / 256), dim3(256), 0, stream, a,
+                           ix);
+        FEW_HIP(hipGetLastError());
+        int n_low, hb0, tiles_low, tiles;
+        fd_owner_split(a.zero_index, a.data_length, n_low, hb0, tiles_low, tiles);
+        if (tiles > 0) {
+            hipLaunchKernelGGL(fd_own_kernel, dim3(tiles), dim3(FD_OWN_TILE), 0, stream, a, ix,
+                               n_low, hb0, tiles_low);
+            FEW_HIP(hipGetLastError());
+        }
+        return FEW_OK;
     }
     long *local = static_cast<long *>(workspace);
     long *prefix = local + nitems;

@@ -184,6 +184,93 @@ __device__ void fact16_split_block(const FactPlanDev &P, const double *interp, c
 #define F16_PREP_THREADS 512
 #define F16_PREP_WHOLE_MIN 1024  // intervals from which a launch takes this form
 
+// The split of one cell's powers into the two planes (rows 2R: (Re, -Im),
+// 2R + 1: (Im, Re) of R = 4 r + p).
+__device__ __forceinline__ void f16_store_cell(_Float16 *hi, _Float16 *lo, int RB, int r, int col,
+                                               const double (&cre)[4], const double (&cim)[4],
+                                               const double (&sigma)[4])
+{
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int R = 4 * r + p;
+        const float xr = f32_opaque((float)(cre[p] * sigma[p]));
+        const float xi = f32_opaque((float)(cim[p] * sigma[p]));
+        const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+        const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
+        const long o0 = f16_tile_off(2 * R, 2 * col, RB);
+        const long o1 = f16_tile_off(2 * R + 1, 2 * col, RB);
+        hi[o0] = rh; hi[o0 + 1] = -ih; lo[o0] = rl; lo[o0 + 1] = -il;
+        hi[o1] = ih; hi[o1 + 1] = rh;  lo[o1] = il; lo[o1 + 1] = rl;
+    }
+}
+
+// The interval's scales from the per-wave maxima in s_red; writes the inverse
+// scales after the planes (thread 0).
+__device__ __forceinline__ void f16_interval_sigma(const double (*s_red)[4], int nwaves,
+                                                   float *inv_out, double (&sigma)[4])
+{
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        double m = 0.0;
+        for (int w = 0; w < nwaves; w++) m = fmax(m, s_red[w][p]);
+        int ex = 0;
+        if (m > 0.0) {
+            frexp(m, &ex);
+            ex = 14 - ex;
+        }
+        sigma[p] = ldexp(1.0, ex);
+        if (threadIdx.x == 0) inv_out[p] = (float)ldexp(1.0, -ex);
+    }
+}
+
+// One workgroup per interval with the cells gathered once: each thread keeps
+// the CPT cells it gathered (4 powers, FP64) in registers through the maxima
+// reduction and splits them from there (the two-pass form gathers every cell
+// twice; the gathers are the kernel's HBM traffic).
+template <int CPT>
+__device__ void fact16_prep_interval_cached(const FactPlanDev &P, const double *interp,
+                                            const double2 *ylm, const double *h_t, int L, int j,
+                                            unsigned char *tile, int KB, int RB)
+{
+    __shared__ double s_red[F16_PREP_THREADS / 64][4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int ncolc = 16 * KB;
+    const int ncell = 2 * RB * ncolc;
+    const double hw = h_t[j + 1] - h_t[j];
+    double cre[CPT][4], cim[CPT][4];
+    double mx[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const int ci = tid + k * F16_PREP_THREADS;
+        const int r = ci / ncolc, col = ci - r * ncolc;
+        if (ci < ncell) {
+            f16_cell(P, interp, ylm, L, j, hw, r, col, cre[k], cim[k]);
+        } else {
+#pragma unroll
+            for (int p = 0; p < 4; p++) cre[k][p] = cim[k][p] = 0.0;
+        }
+#pragma unroll
+        for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], fmax(fabs(cre[k][p]), fabs(cim[k][p])));
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx[p] = fmax(mx[p], __shfl_xor(mx[p], off, 64));
+        if (lane == 0) s_red[wv][p] = mx[p];
+    }
+    __syncthreads();
+    const long plane = (long)KB * RB * F16_BLK;
+    _Float16 *hi = reinterpret_cast<_Float16 *>(tile);
+    _Float16 *lo = hi + plane;
+    double sigma[4];
+    f16_interval_sigma(s_red, F16_PREP_THREADS / 64, reinterpret_cast<float *>(lo + plane), sigma);
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const int ci = tid + k * F16_PREP_THREADS;
+        if (ci < ncell) f16_store_cell(hi, lo, RB, ci / ncolc, ci % ncolc, cre[k], cim[k], sigma);
+    }
+}
+
 __device__ void fact16_prep_interval(const FactPlanDev &P, const double *interp, const double2 *ylm,
                                      const double *h_t, int L, int j, unsigned char *tile, int KB,
                                      int RB)
@@ -192,6 +279,11 @@ __device__ void fact16_prep_interval(const FactPlanDev &P, const double *interp,
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int ncolc = 16 * KB;         // complex columns incl. padding
     const int ncell = 2 * RB * ncolc;  // complex rows r incl. padding x columns
+    const int cpt = (ncell + F16_PREP_THREADS - 1) / F16_PREP_THREADS;  // uniform
+    if (cpt <= 1) return fact16_prep_interval_cached<1>(P, interp, ylm, h_t, L, j, tile, KB, RB);
+    if (cpt == 2) return fact16_prep_interval_cached<2>(P, interp, ylm, h_t, L, j, tile, KB, RB);
+    if (cpt == 3) return fact16_prep_interval_cached<3>(P, interp, ylm, h_t, L, j, tile, KB, RB);
+    if (cpt == 4) return fact16_prep_interval_cached<4>(P, interp, ylm, h_t, L, j, tile, KB, RB);
     const double hw = h_t[j + 1] - h_t[j];
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
     for (int ci = tid; ci < ncell; ci += F16_PREP_THREADS) {
@@ -212,34 +304,12 @@ __device__ void fact16_prep_interval(const FactPlanDev &P, const double *interp,
     _Float16 *hi = reinterpret_cast<_Float16 *>(tile);
     _Float16 *lo = hi + plane;
     double sigma[4];
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        double m = 0.0;
-        for (int w = 0; w < F16_PREP_THREADS / 64; w++) m = fmax(m, s_red[w][p]);
-        int ex = 0;
-        if (m > 0.0) {
-            frexp(m, &ex);
-            ex = 14 - ex;
-        }
-        sigma[p] = ldexp(1.0, ex);
-        if (tid == 0) reinterpret_cast<float *>(lo + plane)[p] = (float)ldexp(1.0, -ex);
-    }
+    f16_interval_sigma(s_red, F16_PREP_THREADS / 64, reinterpret_cast<float *>(lo + plane), sigma);
     for (int ci = tid; ci < ncell; ci += F16_PREP_THREADS) {
         const int r = ci / ncolc, col = ci - r * ncolc;
         double cre[4], cim[4];
         f16_cell(P, interp, ylm, L, j, hw, r, col, cre, cim);
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-            const int R = 4 * r + p;
-            const float xr = f32_opaque((float)(cre[p] * sigma[p]));
-            const float xi = f32_opaque((float)(cim[p] * sigma[p]));
-            const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
-            const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
-            const long o0 = f16_tile_off(2 * R, 2 * col, RB);
-            const long o1 = f16_tile_off(2 * R + 1, 2 * col, RB);
-            hi[o0] = rh; hi[o0 + 1] = -ih; lo[o0] = rl; lo[o0 + 1] = -il;
-            hi[o1] = ih; hi[o1 + 1] = rh;  lo[o1] = il; lo[o1 + 1] = rl;
-        }
+        f16_store_cell(hi, lo, RB, r, col, cre, cim, sigma);
     }
 }
 

@@ -8,8 +8,8 @@ Bars:
   same inputs.  The operations are the oracle's (contraction off); what
   differs is the device libm (hypot, atan2, log, exp, sincos) against glibc in
   the last bit, which Cardano's cube root and the phase exponential carry
-  into the result, and the order in which colliding bins are added (FP64
-  atomics).
+  into the result.  The summed output is bin-owned (each bin written by one
+  thread in a fixed order), so it is also identical from run to run.
 """
 
 import numpy as np
@@ -110,6 +110,37 @@ def test_fd_kernel_empty_and_out_of_range(gpu):
                              inp.ylms, zero)
     got = _run_capi(inp, freq, s2, e2, zero)
     assert _rel_l2(got, ref) < FD_TOL
+
+
+def test_fd_bin_owned_deterministic(gpu):
+    """Two runs of the summed output are identical bit for bit (no atomics)."""
+    inp = _inp(T=0.3, L=80, K=24, seed=21)
+    freq = _grid(inp)
+    ref, (start, end, zero) = oracle_fd(inp, freq)
+    a = _run_capi(inp, freq, start, end, zero)
+    b = _run_capi(inp, freq, start, end, zero)
+    assert np.array_equal(a, b)
+    assert _rel_l2(a, ref) < FD_TOL
+
+
+@pytest.mark.parametrize("zero_shift", [-7, 0, 5, "first", "last"])
+def test_fd_bin_owned_zero_index_and_grid(gpu, zero_shift):
+    """Bin ownership pairs bin b with its mirror 2z - b: grids and zero indices
+    where some bins have no partner (zero index off-centre, at the first or the
+    last bin) give the oracle's sum, whose -f terms land at 2z - i when inside."""
+    inp = _inp(T=0.02, L=25, K=8, seed=3)
+    freq = _grid(inp)
+    _, (start, end, zero) = oracle_fd(inp, freq)
+    z = {"first": 0, "last": len(freq) - 1}.get(zero_shift, None)
+    if z is None:
+        z = zero + zero_shift
+    f_phi, f_theta, f_r = fd_frequencies(inp)
+    interp, pcoef = oracle_fd_inputs(inp)
+    ref = oracle.fd_waveform(interp, inp.t, pcoef, inp.m, inp.k, inp.n, inp.t, freq, start, end,
+                             inp.ylms, z)
+    got = _run_capi(inp, freq, start, end, z)
+    assert np.abs(ref).max() > 0
+    assert _rel_l2(got, ref) < FD_TOL, zero_shift
 
 
 def _turnover_freqs(inp, bump):
