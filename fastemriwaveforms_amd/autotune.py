@@ -14,12 +14,15 @@ import os
 
 import torch
 
-# mixed16 variants: 0 / 4 = 8 / 4 waves per workgroup; 16 / 20 = the same with
-# the row-streamed contraction (taken where the rows are arithmetic and the
-# plan has at most 8 column steps, else the regular kernel runs)
-CANDIDATES = [("mixed16", 0), ("mixed16", 4), ("mixed16", 16), ("mixed16", 20), ("mixed", 0),
-              ("mixed", 4)]
-BATCH_VARIANTS = (0, 4, 16, 20)
+# mixed16 variants are a bit set (include/few_rocm.h): 4 = 4 waves per
+# workgroup instead of 8, 8 = no raised wave priority over the MFMA phase,
+# 16 = the row-streamed contraction (taken where the rows are arithmetic and
+# the plan has at most 8 column steps, else the regular kernel runs), 32 =
+# twice as many persistent workgroups as fit at once.  All 16 are timed.
+MIXED16_VARIANTS = tuple(w | p | s | g for w in (0, 4) for p in (0, 8) for s in (0, 16)
+                         for g in (0, 32))
+CANDIDATES = [("mixed16", v) for v in MIXED16_VARIANTS] + [("mixed", 0), ("mixed", 4)]
+BATCH_VARIANTS = MIXED16_VARIANTS
 
 
 def cache_path() -> str:

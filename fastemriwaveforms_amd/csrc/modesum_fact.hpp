@@ -133,13 +133,18 @@ inline size_t fact16_max_bytes(const FactPlanHost &p, long intervals)
 struct F16Sched {
     int prio;
 };
-// variant: 0 = 8 waves with priority, 4 = 4 waves with priority,
-// 8 = 8 waves without; 16 / 20 = the row-streamed contraction (arithmetic
-// rows and at most F16S_MAX_KB column steps; otherwise variant 0 / 4) with
-// 8 / 4 waves, with priority
-inline F16Sched fact16_sched(int variant) { return F16Sched{variant == 8 ? 0 : 1}; }
-inline int fact16_waves(int variant) { return variant == 4 || variant == 20 ? 4 : 8; }
-inline bool fact16_streamed(int variant) { return variant == 16 || variant == 20; }
+// variant of the split-FP16 sum, a bit set:
+//   4  -- 4 waves per workgroup (one per SIMD) instead of 8;
+//   8  -- no raised wave priority over the contraction;
+//   16 -- the row-streamed contraction (taken for arithmetic rows and at most
+//         F16S_MAX_KB column steps; otherwise the regular one runs);
+//   32 -- twice as many persistent workgroups as fit at once (shorter
+//         contiguous tile ranges, balanced by the hardware scheduler).
+constexpr int F16_VARIANT_BITS = 4 | 8 | 16 | 32;
+inline F16Sched fact16_sched(int variant) { return F16Sched{(variant & 8) ? 0 : 1}; }
+inline int fact16_waves(int variant) { return (variant & 4) ? 4 : 8; }
+inline bool fact16_streamed(int variant) { return (variant & 16) != 0; }
+inline int fact16_grid_factor(int variant) { return (variant & 32) ? 2 : 1; }
 constexpr int F16S_MAX_KB = 8;
 
 struct Fact16Args {

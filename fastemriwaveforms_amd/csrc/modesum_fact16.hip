@@ -1194,7 +1194,11 @@ int launch_fact16_sum(const Fact16Args &a_in, int RB, int max_blocks, int varian
         set_error("mixed16 sum: coefficient tile of %zu bytes exceeds LDS", lds);
         return FEW_ERR_UNSUPPORTED;
     }
-    int blocks = device_cus16() * (lds <= 80 * 1024 ? 2 : 1);
+    if (variant & ~F16_VARIANT_BITS) {
+        set_error("mixed16 sum: unknown variant %d", variant);
+        return FEW_ERR_ARG;
+    }
+    int blocks = device_cus16() * (lds <= 80 * 1024 ? 2 : 1) * fact16_grid_factor(variant);
     if (blocks > max_blocks) blocks = max_blocks;
     const bool arith = a.plan.rows_arith != 0;
     const int waves = fact16_waves(variant);
@@ -1284,9 +1288,13 @@ int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs
         set_error("batch mixed16 sum: coefficient tile of %zu bytes exceeds LDS", lds);
         return FEW_ERR_UNSUPPORTED;
     }
+    if (variant & ~F16_VARIANT_BITS) {
+        set_error("batch mixed16 sum: unknown variant %d", variant);
+        return FEW_ERR_ARG;
+    }
     BatchRunDev rs = r;
     rs.overwrite = (stage & 4) ? 1 : 0;
-    const int blocks = device_cus16() * (lds <= 80 * 1024 ? 2 : 1);
+    const int blocks = device_cus16() * (lds <= 80 * 1024 ? 2 : 1) * fact16_grid_factor(variant);
     const bool arith = B.uni.rows_arith != 0;
     const F16Sched sched = fact16_sched(variant);
     const int waves = fact16_waves(variant);

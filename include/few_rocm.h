@@ -137,10 +137,14 @@ size_t few_plan_workspace_bytes(const void *plan, int init_len, int out_len, int
  * precision choice, asynchronous on device-resident time arrays.
  * FEW_PRECISION_FP64 runs the reference arithmetic; FEW_PRECISION_MIXED the
  * factorised MFMA sum (rel-L2 ~1e-7 against the FP64 result).  `variant`
- * selects the tile shape (0 = default: 32 samples per wave, 2 waves per
- * SIMD; 4 = 64 samples per wave, 1 wave per SIMD; for FEW_PRECISION_MIXED16
- * 8 = the default shape without the raised wave priority over the MFMA
- * phase).  `stage` is a bitmask:
+ * selects the kernel shape (0 = default).  FEW_PRECISION_MIXED: 0 = 32
+ * samples per wave, 2 waves per SIMD; 4 = 64 samples per wave, 1 wave per
+ * SIMD.  FEW_PRECISION_MIXED16, a bit set: 4 = 4 waves per workgroup (one
+ * per SIMD) instead of 8; 8 = no raised wave priority over the MFMA phase;
+ * 16 = the row-streamed contraction (same results bit for bit); 32 = twice
+ * as many persistent workgroups as fit at once; other bits are refused
+ * (FEW_ERR_ARG).  The autotuner (fastemriwaveforms_amd.autotune) times them.
+ * `stage` is a bitmask:
  * 1 = prepare (interval planning, coefficient tiles), 2 = sum (the output
  * kernel), 3 = both; stage 2 reuses what the last stage-1 call on the same
  * workspace prepared, and is refused (FEW_ERR_ARG) unless that call had the

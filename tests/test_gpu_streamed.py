@@ -95,3 +95,29 @@ def test_streamed_full1_vs_fp64(gpu):
     err = oracle.rel_l2(h16.cpu().numpy(), h64.cpu().numpy())
     print(f"full1 streamed rel-L2 vs fp64 {err:.3e}")
     assert err < 1e-6
+
+
+def test_every_mixed16_variant_same_bits(gpu):
+    """All 16 variants of the split-FP16 sum (waves per workgroup, wave
+    priority, contraction form, persistent grid size) give the same samples
+    bit for bit, single waveform and batch; an unknown variant bit is refused."""
+    from fastemriwaveforms_amd import autotune, synthetic
+    from fastemriwaveforms_amd._lib import FewRocmError
+    from fastemriwaveforms_amd.pipeline import BatchProblem
+    from test_gpu_batch import _small_batch
+    assert len(autotune.MIXED16_VARIANTS) == 16
+    inp = synthetic.make_workload("cpu0")
+    ref, _ = _single(inp, 0)
+    for v in autotune.MIXED16_VARIANTS:
+        got, _ = _single(inp, v)
+        assert np.array_equal(got, ref), v
+    bp = BatchProblem(_small_batch(seed=17, count=4))
+    bp.variant = 0
+    bp.run("mixed16")
+    bref = bp.wave.clone()
+    for v in autotune.MIXED16_VARIANTS:
+        bp.variant = v
+        bp.run("mixed16")
+        assert torch.equal(bp.wave, bref), v
+    with pytest.raises(FewRocmError):
+        _single(inp, 64)
