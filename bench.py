@@ -784,6 +784,12 @@ def single_leg(dev, args):
            "precision": prec, "ms_per_step": ms_step, "eager_ms_per_step": ms_eager,
            "kernel_ms": ms, "mode_samples_per_s": value, "waveforms_per_s": 1e3 / ms_step,
            "rel_l2_vs_oracle": None,
+           # spline build: 32 B per (array, knot) -- the knot value read, c1..c3 written
+           "spline_roofline": {"bound": "hbm", "unit": "GB/s", "peak": 8000.0,
+                               "achieved": 32.0 * (2 * K + 3) * inp.L / (ms["spline"] * 1e-3) / 1e9,
+                               "algorithmic": f"32 B x {2 * K + 3} arrays x {inp.L} knots",
+                               "bound_by": "the elimination and back-substitution chains "
+                                           "(latency), not bandwidth"},
            "roofline": fact16_roofline("fact16_kernel", prob.plan.rows, prob.plan.cols,
                                        prob.plan.row_blocks, N, ms["sum"],
                                        FACT16_FULL1_TRAFFIC,

@@ -158,8 +158,9 @@ __host__ __device__ inline double nr_jn(int n, double x)
 
 // One step k of nr_jn_miller for the chains LO..4 of nr_jn5 (each chain's
 // operations in nr_jn_miller's order: the step, the rescaling, the even-order
-// sum -- k odd, as every start is even -- and the capture at k = order).
-template <int LO>
+// sum -- on odd k, as every start is even -- and the capture at k = order,
+// which only the CAP steps test).
+template <int LO, bool ODD, bool CAP>
 __host__ __device__ inline void nr_miller_step(int k, double two_over_x, int nu0, double (&cur)[5],
                                                double (&prev)[5], double (&even_sum)[5],
                                                double (&at_n)[5])
@@ -184,13 +185,35 @@ __host__ __device__ inline void nr_miller_step(int k, double two_over_x, int nu0
             }
         }
     }
-    if (k & 1) {
+    if (ODD) {
 #pragma unroll
         for (int i = LO; i < 5; i++) even_sum[i] += cur[i];
     }
+    if (CAP) {
 #pragma unroll
-    for (int i = LO; i < 5; i++)
-        if (k == nu0 + i) at_n[i] = prev[i];
+        for (int i = LO; i < 5; i++)
+            if (k == nu0 + i) at_n[i] = prev[i];
+    }
+}
+
+// Steps k, k - 1, ..., stop + 1 of the chains LO..4, k and stop even: step
+// pairs (even k, odd k - 1), so the even-order sum needs no parity test and
+// the two steps share their registers; only the pairs that hold an order
+// nu0 .. nu0 + 4 test for the capture.
+template <int LO>
+__host__ __device__ inline void nr_miller_run(int &k, int stop, double two_over_x, int nu0,
+                                              double (&cur)[5], double (&prev)[5],
+                                              double (&even_sum)[5], double (&at_n)[5])
+{
+    for (; k > stop; k -= 2) {
+        if (k - 1 <= nu0 + 4 && k >= nu0) {
+            nr_miller_step<LO, false, true>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
+            nr_miller_step<LO, true, true>(k - 1, two_over_x, nu0, cur, prev, even_sum, at_n);
+        } else {
+            nr_miller_step<LO, false, false>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
+            nr_miller_step<LO, true, false>(k - 1, two_over_x, nu0, cur, prev, even_sum, at_n);
+        }
+    }
 }
 
 // nr_jn(nu0 + i, x), i = 0..4, for x >= 0 (the five Bessel values of one
@@ -215,12 +238,13 @@ __host__ __device__ inline void nr_jn5(int nu0, double x, double (&out)[5])
         cur[i] = 1.0;
         prev[i] = at_n[i] = even_sum[i] = 0.0;
     }
+    // every start is even (or 0), so every run spans whole step pairs
     int k = start[4];
-    for (; k > start[3]; k--) nr_miller_step<4>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
-    for (; k > start[2]; k--) nr_miller_step<3>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
-    for (; k > start[1]; k--) nr_miller_step<2>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
-    for (; k > start[0]; k--) nr_miller_step<1>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
-    for (; k > 0; k--) nr_miller_step<0>(k, two_over_x, nu0, cur, prev, even_sum, at_n);
+    nr_miller_run<4>(k, start[3], two_over_x, nu0, cur, prev, even_sum, at_n);
+    nr_miller_run<3>(k, start[2], two_over_x, nu0, cur, prev, even_sum, at_n);
+    nr_miller_run<2>(k, start[1], two_over_x, nu0, cur, prev, even_sum, at_n);
+    nr_miller_run<1>(k, start[0], two_over_x, nu0, cur, prev, even_sum, at_n);
+    nr_miller_run<0>(k, 0, two_over_x, nu0, cur, prev, even_sum, at_n);
 #pragma unroll
     for (int i = 0; i < 5; i++) {
         const int nu = nu0 + i;
