@@ -495,11 +495,15 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
     teuk = torch.from_numpy(np.ascontiguousarray(inp.teuk)).to(dev)
     pk = torch.from_numpy(np.ascontiguousarray(inp.phase_knots)).to(dev)
     rng = np.random.default_rng(2024)
-    # (forward, backward) pairs: "fp64" = the per-mode FP64 adjoint (the reference's
-    # arithmetic, differentiated); "fact64" = the factorised FP64 moments on the FP64
-    # matrix cores; "mixed16" = the split-FP16 moments (approximate phase gradients)
+    # (forward, backward) pairs.  Forward: "fp64" = the reference arithmetic (direct
+    # sum), "fact64" = the factorised sum in FP64 on the matrix cores, "mixed16" = the
+    # split-FP16 factorised sum.  Backward: "fp64" = the per-mode FP64 adjoint (the
+    # reference's arithmetic, differentiated); "fact64" = the factorised FP64 moments
+    # on the FP64 matrix cores; "mixed16" = the split-FP16 moments (approximate phase
+    # gradients)
     pairs = {"fp64": ("fp64", "fp64"), "fp64+fact64": ("fp64", "fact64"),
-             "mixed16+fact64": ("mixed16", "fact64"), "mixed16": ("mixed16", "mixed16")}
+             "fact64": ("fact64", "fact64"), "mixed16+fact64": ("mixed16", "fact64"),
+             "mixed16": ("mixed16", "mixed16")}
     engines = {name: DifferentiableModeSum(inp.t, inp.m, inp.k, inp.n, inp.ylms, inp.dt,
                                            inp.num_pts, precision=f, device=dev, backward=b)
                for name, (f, b) in pairs.items()}
@@ -554,8 +558,9 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
         return out
     base_t = torch.view_as_real(teuk).contiguous()
     amp_scale = float(torch.abs(teuk).max())
-    ad, fd, ad_f = [], [], []
+    ad, fd, ad_f, ad_64 = [], [], [], []
     g_tf, g_pf = grads["mixed16+fact64"]
+    g_t64, g_p64 = grads["fact64"]
     for c in range(fd_coords):
         if c < fd_coords * 3 // 4:   # amplitude knot (linear: step relative to max |A|)
             l, mo, part = int(rng.integers(inp.L)), int(rng.integers(inp.K)), int(rng.integers(2))
@@ -568,6 +573,7 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
             fd.append((loss_at(tp, pk) - loss_at(tm, pk)) / (2 * step))
             ad.append(float(g_t[l, mo, part]))
             ad_f.append(float(g_tf[l, mo, part]))
+            ad_64.append(float(g_t64[l, mo, part]))
         else:                        # phase knot (Phi_phi or Phi_r): Richardson of 5-point stencils
             q, l = int(rng.choice([0, 2])), int(rng.integers(inp.L))
 
@@ -582,7 +588,8 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
             fd.append((16 * d5(5e-4) - d5(1e-3)) / 15)
             ad.append(float(g_p[q, l]))
             ad_f.append(float(g_pf[q, l]))
-    ad, fd, ad_f = np.array(ad), np.array(fd), np.array(ad_f)
+            ad_64.append(float(g_p64[q, l]))
+    ad, fd, ad_f, ad_64 = np.array(ad), np.array(fd), np.array(ad_f), np.array(ad_64)
     per = np.abs(ad - fd) / np.maximum(np.abs(fd), 1e-300)
     n_amp = fd_coords * 3 // 4
     # per coordinate, relative to the scale of its gradient array (a single knot can
@@ -602,10 +609,12 @@ def adjoint_leg(dev, reps: int = 3, batch: int = 16, fd_coords: int = 64):
                                 "stencils at 1e-3 and 5e-4 rad (FP64 rounding of 1e5-1e7 rad "
                                 "phases floors a 1e-6 rad central difference at ~1e-3)",
                        "function": "FP64 GPU forward (rel-L2 ~4e-16 vs the oracle)",
-                       "mixed16_fact64_rel_l2": float(np.linalg.norm(ad_f - fd) / np.linalg.norm(fd))}
+                       "mixed16_fact64_rel_l2": float(np.linalg.norm(ad_f - fd) / np.linalg.norm(fd)),
+                       "fact64_rel_l2": float(np.linalg.norm(ad_64 - fd) / np.linalg.norm(fd))}
     log(f"[bench] adjoint FD check ({fd_coords} coords): rel-L2 {out['fd_check']['rel_l2']:.2e}, "
         f"max per-coordinate {per.max():.2e}, median {np.median(per):.2e}; mixed16+fact64 "
-        f"rel-L2 {out['fd_check']['mixed16_fact64_rel_l2']:.2e}")
+        f"rel-L2 {out['fd_check']['mixed16_fact64_rel_l2']:.2e}; fact64 pair "
+        f"rel-L2 {out['fd_check']['fact64_rel_l2']:.2e}")
     # gradient throughput: 16 full-size waveforms (own knots and amplitudes), fwd + bwd each
     if batch <= 0:
         return out
@@ -810,6 +819,23 @@ def single_leg(dev, args):
                                      "algorithmic": f"{DIRECT_FLOP} FLOP/mode-sample, sincos "
                                                     "counted 0"}}
         out["speedup_vs_fp64_plain_gpu"] = ms64_step / ms_step
+        if prob.plan.supported64:
+            # the factorised sum in FP64 on the matrix cores (near-FP64 results)
+            ms_f64_step, ms_f64 = time_path(prob, "fact64", 10, 2)
+            hf = prob.wave.cpu().numpy()
+            out["fact64"] = {
+                "ms_per_step": ms_f64_step, "kernel_ms": ms_f64,
+                "mode_samples_per_s": K * N / (ms_f64_step * 1e-3),
+                "rel_l2_vs_fp64_all_samples": float(np.linalg.norm(hf - h64) / np.linalg.norm(h64)),
+                "speedup_vs_fp64_plain_gpu": ms64_step / ms_f64_step,
+                "roofline": {"pipe": "FP64 MFMA", "peak": PEAK["fp64"], "unit": "TFLOP/s",
+                             "achieved": 32.0 * prob.plan.rows * prob.plan.cols * N /
+                             (ms_f64["sum"] * 1e-3) / 1e12,
+                             "algorithmic": "8 FLOP x 4 powers x rows x cols per sample"}}
+            out["fact64"]["roofline"]["frac"] = out["fact64"]["roofline"]["achieved"] / PEAK["fp64"]
+            log(f"[bench] full1 fact64: step {ms_f64_step:.3f} ms (sum {ms_f64['sum']:.3f}), "
+                f"{ms64_step / ms_f64_step:.1f}x the FP64 direct step, rel-L2 vs fp64 "
+                f"{out['fact64']['rel_l2_vs_fp64_all_samples']:.2e}")
         chk = (idx, h[idx].copy(), h64[idx].copy())
         log(f"[bench] full1 fp64: step {ms64_step:.3f} ms, speed-up {ms64_step / ms_step:.2f}x, "
             f"mixed16 rel-L2 vs fp64 {out['fp64_plain_gpu']['mixed16_rel_l2_vs_fp64_all_samples']:.3e}")

@@ -80,6 +80,8 @@ class DifferentiableModeSum:
             raise ValueError("this mode set does not fit the mixed16 tile; use 'mixed'")
         if precision == "mixed" and not self.plan.supported:
             raise ValueError("this mode set does not fit the factorised tile; use 'fp64'")
+        if precision == "fact64" and not self.plan.supported64:
+            raise ValueError("this mode set does not fit the FP64 factorised tile; use 'fp64'")
         self.interp = torch.empty((4, L, 2 * K), dtype=f64, device=dev)
         self.pcoef = torch.empty((8, L - 1, 3), dtype=f64, device=dev)
         self.scratch = torch.empty(4 * L * (2 * K + 3), dtype=f64, device=dev)

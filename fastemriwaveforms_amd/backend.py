@@ -124,7 +124,7 @@ class RocmBackend(Backend):
 
     @property
     def precision(self) -> str:
-        """Arithmetic of get_waveform_wrap ("fp64", "mixed16", "mixed"); the
+        """Arithmetic of get_waveform_wrap ("fp64", "fact64", "mixed16", "mixed"); the
         FEW_ROCM_PRECISION environment variable sets it at import."""
         from . import pyinterp
         return pyinterp.get_precision()

@@ -124,9 +124,9 @@ static size_t fact_coef_offset(int L) { return plan_layout(L).total; }
 static size_t fact_tiles_bytes(const FactPlanHost &p, int L)
 {
     const size_t n = (size_t)(L > 1 ? L - 1 : 1);
-    const size_t f32 = align256(sizeof(float) * fact_coef_floats_per_interval(p) * n);
+    const size_t f64 = align256(sizeof(double) * fact_coef_floats_per_interval(p) * n);
     const size_t f16 = align256(fact16_tile_bytes(p) * n);
-    return f32 > f16 ? f32 : f16;
+    return f64 > f16 ? f64 : f16;  // the FP32 tiles are half the FP64 ones
 }
 // plan layout | coefficient tiles | row maxima of the mixed16 tile prep
 static size_t fact_ws_bytes(const FactPlanHost &p, int L)
@@ -136,6 +136,11 @@ static size_t fact_ws_bytes(const FactPlanHost &p, int L)
 static bool fact16_supported(const FactPlanHost &p)
 {
     return p.ok && fact16_tile_bytes(p) + 24 * sizeof(double) <= 160 * 1024;
+}
+static bool fact64_supported(const FactPlanHost &p)
+{
+    return p.ok && fact_coef_floats_per_interval(p) * sizeof(double) + 24 * sizeof(double) <=
+                       160 * 1024;
 }
 
 struct DeviceGuard {
@@ -337,6 +342,51 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
         }
         FactArgs fa{wave, pcoef, segs, prefix, L - 1, dt, C, c_stride, plan->dev};
         return launch_fact_sum(fa, ph, (int)max_blocks, variant, stream);
+    }
+    if (precision == FEW_PRECISION_FACT64) {
+        if (!plan || !fact64_supported(plan->host)) {
+            set_error("get_waveform: fact64 precision needs a supported mode plan");
+            return FEW_ERR_UNSUPPORTED;
+        }
+        const FactPlanHost &ph = plan->host;
+        if (ph.K != K) {
+            set_error("get_waveform: plan built for %d modes, called with %d", ph.K, K);
+            return FEW_ERR_ARG;
+        }
+        if (ws_bytes < fact_ws_bytes(ph, L)) {
+            set_error("get_waveform: workspace of %zu bytes, need %zu", ws_bytes,
+                      fact_ws_bytes(ph, L));
+            return FEW_ERR_ARG;
+        }
+        double *C = reinterpret_cast<double *>(ws + fact_coef_offset(L));
+        const long c_stride = (long)fact_coef_floats_per_interval(ph);
+        if (variant != 0 && variant != 4) {
+            set_error("get_waveform: fact64 variant %d unknown (0 or 4)", variant);
+            return FEW_ERR_ARG;
+        }
+        const int tile = fact64_tile(variant);
+        const long max_blocks = ((long)out_len + tile - 1) / tile + (L - 1);
+        if (max_blocks > 0x7fffffffL) {
+            set_error("get_waveform: output too long (%d samples)", out_len);
+            return FEW_ERR_LIMIT;
+        }
+        const PrepKey key{ws, plan, precision, tile, L, out_len};
+        if (int rc = prep_stage(stage & 1, stage & 2, key, "get_waveform")) return rc;
+        if (stage & 1) {
+            PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
+            int rc = launch_plan(p, stream);
+            if (rc) return rc;
+            FactPrepArgs pa{interp, ylm, h_t, L, nullptr, c_stride, plan->dev};
+            pa.C64 = C;
+            rc = launch_fact_prep(pa, stream);
+            if (rc) return rc;
+        }
+        if (!(stage & 2)) return FEW_OK;
+        FactArgs fa{wave, pcoef, segs, prefix, L - 1, dt, nullptr, c_stride, plan->dev};
+        fa.C64 = C;
+        fa.overwrite = (stage & 4) ? 1 : 0;
+        fa.out_len = out_len;
+        return launch_fact64_sum(fa, ph, (int)max_blocks, variant, stream);
     }
     if (precision != FEW_PRECISION_FP64) {
         set_error("get_waveform: precision %d is not supported", precision);
@@ -666,7 +716,7 @@ int few_mode_plan_info(const void *plan, int *info7)
     info7[3] = h.n_lo;
     info7[4] = h.RB;
     info7[5] = h.KS;
-    info7[6] = (h.ok ? 1 : 0) | (fact16_supported(h) ? 2 : 0);
+    info7[6] = (h.ok ? 1 : 0) | (fact16_supported(h) ? 2 : 0) | (fact64_supported(h) ? 4 : 0);
     return FEW_OK;
 }
 
@@ -683,7 +733,8 @@ size_t few_plan_workspace_bytes(const void *plan, int init_len, int out_len, int
     (void)out_len;
     const int L = init_len < 2 ? 2 : init_len;
     size_t b = plan_layout(L).total;
-    if (plan && (precision == FEW_PRECISION_MIXED || precision == FEW_PRECISION_MIXED16)) {
+    if (plan && (precision == FEW_PRECISION_MIXED || precision == FEW_PRECISION_MIXED16 ||
+                 precision == FEW_PRECISION_FACT64)) {
         const FactPlanHost &h = static_cast<const FactPlan *>(plan)->host;
         if (h.ok) b = fact_ws_bytes(h, L);
     }

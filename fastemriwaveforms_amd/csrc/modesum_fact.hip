@@ -136,6 +136,16 @@ __global__ __launch_bounds__(256) void fact_prep_kernel(FactPrepArgs a)
             hp *= hw;
         }
     }
+    if (a.C64) {
+        double *dst = a.C64 + (long)j * a.c_stride;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const long o = ((long)(r * 4 + p) * P.CST + col) * 2;
+            dst[o] = cre[p];
+            dst[o + 1] = cim[p];
+        }
+        return;
+    }
     float *dst = a.C + (long)j * a.c_stride;
 #pragma unroll
     for (int p = 0; p < 4; p++) {
@@ -346,6 +356,198 @@ void fact_f32_kernel(FactArgs a)
     }
 }
 
+// ---------------------------------------------------------------- FP64 sum
+// fact_tile_body in FP64: the tile entries, the v_n and u_r recurrences and
+// the epilogue stay FP64, and the n-contraction runs on
+// v_mfma_f64_16x16x4f64 (FP64 products and accumulation).  That MFMA returns
+// D[g + 4 i][l & 15] to lane l (g = l >> 4) where the FP32 one returns
+// D[4 g + i]; the A rows are permuted so that a lane still receives
+// (p0 Re, p0 Im, p0+1 Re, p0+1 Im) of row r = 2 rb + (g >> 1): MFMA row rho
+// carries complex row r_local = (rho & 3) >> 1, power 2 (rho & 1) + (rho >> 3)
+// and part (rho >> 2) & 1.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <int RB, int NB, int WAVES>
+__device__ __forceinline__ void fact64_tile_body(const FactArgs &a, const FactPlanDev &P,
+                                                 const double *sC, const Segment &sg, int q,
+                                                 const double *sPh, int wv, int li, int g)
+{
+    // rounding fixed by the source, not by the compiler's fusion choices, so
+    // every variant (samples per wave, waves) gives the same bits
+#pragma clang fp contract(off)
+    const double *rp = sPh;
+    const double *rt = sPh + 8;
+    const double *rr = sPh + 16;
+    constexpr int TILE = 16 * NB * WAVES;
+    int idx[NB];
+    bool ok[NB];
+    double v_re[NB], v_im[NB], w_re[NB], w_im[NB], uph[NB], uth[NB], sa[NB];
+    const int n0 = P.n_lo + (g >> 1);
+#pragma unroll
+    for (int c = 0; c < NB; c++) {
+        idx[c] = sg.start + q * TILE + wv * (16 * NB) + c * 16 + li;
+        ok[c] = idx[c] < sg.end;
+        const double t = a.dt * (double)idx[c];
+        sa[c] = (t - sg.h_t) / sg.hw;
+        const double s = (t - sg.pt) / sg.pw;
+        const double s1 = 1.0 - s;
+        const double ur = dense_phase(rr, s, s1) * INV_2PI;
+        uph[c] = dense_phase(rp, s, s1) * INV_2PI;
+        uth[c] = P.has_k ? dense_phase(rt, s, s1) * INV_2PI : 0.0;
+        double sn, cs;
+        sincos_turns(reduce_turns((double)n0 * ur), sn, cs);
+        v_re[c] = cs;
+        v_im[c] = -sn;
+        sincos_turns(reduce_turns(2.0 * ur), sn, cs);
+        w_re[c] = cs;
+        w_im[c] = -sn;
+    }
+    v4d acc[RB][NB];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+        for (int c = 0; c < NB; c++) acc[rb][c] = v4d{0.0, 0.0, 0.0, 0.0};
+    const int f = g & 1;
+    const int r_loc = (li & 3) >> 1, p_row = 2 * (li & 1) + (li >> 3), e = (li >> 2) & 1;
+    const int pidx = e ^ f;
+    const double sgn = (e == 0 && f == 1) ? -1.0 : 1.0;
+    const int cst = P.CST;
+    const int KS = P.KS;
+    const int Nc = P.Nc;
+    for (int s = 0; s < KS; s++) {
+        const int acol = 2 * s + (g >> 1);
+        double bv[NB];
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+            bv[c] = (acol < Nc) ? (f ? v_im[c] : v_re[c]) : 0.0;
+            const double nr = v_re[c] * w_re[c] - v_im[c] * w_im[c];
+            const double ni = v_re[c] * w_im[c] + v_im[c] * w_re[c];
+            v_re[c] = nr;
+            v_im[c] = ni;
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const int R = (2 * rb + r_loc) * 4 + p_row;
+            const double av = sgn * sC[(R * cst + acol) * 2 + pidx];
+#pragma unroll
+            for (int c = 0; c < NB; c++)
+                acc[rb][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[c], acc[rb][c], 0, 0, 0);
+        }
+    }
+    const int p0 = 2 * f;
+#pragma unroll
+    for (int c = 0; c < NB; c++) {
+        const double s1 = sa[c], s2 = s1 * s1;
+        const double spa = p0 == 0 ? 1.0 : s2;
+        const double spb = p0 == 0 ? s1 : s2 * s1;
+        double hre = 0.0, him = 0.0;
+        double u_re = 0.0, u_im = 0.0, e_re = 0.0, e_im = 0.0;
+        if (P.rows_arith) {
+            double sn, cs;
+            const double m_first = (double)(P.m0 + (g >> 1));
+            sincos_turns(reduce_turns(m_first * uph[c] + (double)P.k0 * uth[c]), sn, cs);
+            u_re = cs;
+            u_im = -sn;
+            sincos_turns(reduce_turns(2.0 * uph[c]), sn, cs);
+            e_re = cs;
+            e_im = -sn;
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const int r = 2 * rb + (g >> 1);
+            double ur, ui;
+            if (P.rows_arith) {
+                ur = u_re;
+                ui = u_im;
+                const double nr = u_re * e_re - u_im * e_im;
+                const double ni = u_re * e_im + u_im * e_re;
+                u_re = nr;
+                u_im = ni;
+            } else {
+                const int2 mk = r < P.Nr ? P.row_mk[r] : make_int2(0, 0);
+                double sn, cs;
+                sincos_turns(reduce_turns((double)mk.x * uph[c] + (double)mk.y * uth[c]), sn, cs);
+                ur = cs;
+                ui = -sn;
+            }
+            if (r < P.Nr) {
+                const v4d W = acc[rb][c];
+                const double zr = spa * W[0] + spb * W[2];
+                const double zi = spa * W[1] + spb * W[3];
+                hre += ur * zr - ui * zi;
+                him += ur * zi + ui * zr;
+            }
+        }
+        hre += __shfl_xor(hre, 16);
+        him += __shfl_xor(him, 16);
+        hre += __shfl_xor(hre, 32);
+        him += __shfl_xor(him, 32);
+        if (g == 0 && ok[c]) {
+            if (a.overwrite) {
+                a.wave[idx[c]] = make_double2(hre, him);
+            } else {
+                double2 w = a.wave[idx[c]];
+                w.x += hre;
+                w.y += him;
+                a.wave[idx[c]] = w;
+            }
+        }
+    }
+}
+
+template <int RB, int NB, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
+void fact_f64_kernel(FactArgs a)
+{
+    constexpr int NT = 64 * WAVES;
+    extern __shared__ __attribute__((aligned(16))) double sC64[];
+    double *sPh = sC64 + a.c_stride;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = tid >> 6;
+    const int li = lane & 15;
+    const int g = lane >> 4;
+    const FactPlanDev P = a.plan;
+    const int total = a.prefix[a.nseg];
+    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * per;
+    const int t1 = min(total, t0 + per);
+    if (a.overwrite) {
+        // samples after the last interval: zero (the intervals cover [0, end))
+        const long end = a.nseg > 0 ? a.segs[a.nseg - 1].end : 0;
+        for (long i = end + (long)blockIdx.x * NT + tid; i < a.out_len; i += (long)gridDim.x * NT)
+            a.wave[i] = make_double2(0.0, 0.0);
+    }
+    if (t0 >= t1) return;
+    int j = fact_find_segment(a.prefix, a.nseg, t0);
+    int jnext_start = a.prefix[j + 1];
+    const long np = (long)a.nseg * 3;
+    bool stale = true;
+    for (int tile = t0; tile < t1; tile++) {
+        while (tile >= jnext_start) {
+            j++;
+            jnext_start = a.prefix[j + 1];
+            stale = true;
+        }
+        if (stale) {
+            __syncthreads();
+            const double2 *src = reinterpret_cast<const double2 *>(a.C64 + (long)j * a.c_stride);
+            double2 *dst = reinterpret_cast<double2 *>(sC64);
+            const int nv = (int)(a.c_stride >> 1);
+            for (int i = tid; i < nv; i += NT) dst[i] = src[i];
+            if (tid < 24) sPh[tid] = a.pcoef[(tid & 7) * np + (long)j * 3 + (tid >> 3)];
+            __syncthreads();
+            stale = false;
+        }
+        const Segment sg = a.segs[j];
+        fact64_tile_body<RB, NB, WAVES>(a, P, sC64, sg, tile - a.prefix[j], sPh, wv, li, g);
+    }
+}
+
+// variant 0: 8 waves x 16 samples (two waves per SIMD); 4: 4 waves x 32
+// samples (one wave per SIMD with the whole register file)
+int fact64_tile(int /*variant*/) { return 128; }
+
 int fact_tile(const FactPlanHost & /*p*/, int variant) { return 64 * (variant == 4 ? 4 : 2); }
 
 static int device_cus()
@@ -382,6 +584,48 @@ static int launch_fact_nb(const FactArgs &a, int RB, int blocks, size_t lds, hip
     }
     FEW_HIP(hipGetLastError());
     return FEW_OK;
+}
+
+template <int NB, int WAVES>
+static int launch_fact64_nb(const FactArgs &a, int RB, int blocks, size_t lds, hipStream_t stream)
+{
+    switch (RB) {
+#define FEW_CASE64(X)                                                                          \
+    case X:                                                                                    \
+        FEW_LDS((fact_f64_kernel<X, NB, WAVES>), lds);                                         \
+        hipLaunchKernelGGL((fact_f64_kernel<X, NB, WAVES>), dim3(blocks), dim3(64 * WAVES), lds, \
+                           stream, a);                                                         \
+        break;
+        FEW_CASE64(1) FEW_CASE64(2) FEW_CASE64(3) FEW_CASE64(4) FEW_CASE64(5) FEW_CASE64(6)
+        FEW_CASE64(7) FEW_CASE64(8) FEW_CASE64(9) FEW_CASE64(10) FEW_CASE64(11) FEW_CASE64(12)
+        FEW_CASE64(13) FEW_CASE64(14) FEW_CASE64(15) FEW_CASE64(16)
+#undef FEW_CASE64
+    default:
+        set_error("fact64 sum: %d row blocks unsupported", RB);
+        return FEW_ERR_UNSUPPORTED;
+    }
+    FEW_HIP(hipGetLastError());
+    return FEW_OK;
+}
+
+int launch_fact64_sum(const FactArgs &a, const FactPlanHost &p, int max_blocks, int variant,
+                      hipStream_t stream)
+{
+    if (max_blocks <= 0 && !a.overwrite) return FEW_OK;
+    if (variant != 0 && variant != 4) {
+        set_error("fact64 sum: unknown variant %d", variant);
+        return FEW_ERR_ARG;
+    }
+    const size_t lds = (size_t)fact_coef_floats_per_interval(p) * sizeof(double) + 24 * sizeof(double);
+    if (lds > 160 * 1024) {
+        set_error("fact64 sum: coefficient tile of %zu bytes exceeds LDS", lds);
+        return FEW_ERR_UNSUPPORTED;
+    }
+    int blocks = device_cus() * (lds <= 80 * 1024 ? 2 : 1);
+    if (blocks > max_blocks) blocks = max_blocks;
+    if (blocks < 1) blocks = 1;
+    if (variant == 4) return launch_fact64_nb<2, 4>(a, p.RB, blocks, lds, stream);
+    return launch_fact64_nb<1, 8>(a, p.RB, blocks, lds, stream);
 }
 
 int launch_fact_sum(const FactArgs &a, const FactPlanHost &p, int max_blocks, int variant,

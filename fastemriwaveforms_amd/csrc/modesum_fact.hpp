@@ -79,8 +79,11 @@ struct FactArgs {
     int nseg;
     double dt;
     const float *C;               // (L-1) x RB*8 x CST x 2
-    long c_stride;                // floats per interval
+    long c_stride;                // floats per interval (doubles for C64)
     FactPlanDev plan;
+    const double *C64 = nullptr;  // the FP64 tiles of the fact64 sum (same layout)
+    int overwrite = 0;            // fact64: write h instead of adding; zero the tail
+    int out_len = 0;
 };
 
 struct FactPrepArgs {
@@ -91,9 +94,17 @@ struct FactPrepArgs {
     float *C;
     long c_stride;
     FactPlanDev plan;
+    double *C64 = nullptr;        // when set: FP64 tiles instead of C
 };
 
 int launch_fact_prep(const FactPrepArgs &a, hipStream_t stream);
+
+// ---- FP64 factorised sum (modesum_fact.hip, precision FEW_PRECISION_FACT64):
+// the same tiles in FP64 and v_mfma_f64_16x16x4f64, FP64 phasors and epilogue.
+// variant 0: 8 waves x 16 samples per workgroup tile, 4: 4 waves x 32.
+int fact64_tile(int variant);
+int launch_fact64_sum(const FactArgs &a, const FactPlanHost &p, int max_blocks, int variant,
+                      hipStream_t stream);
 
 // ---- FP16 split-operand variant (modesum_fact16.hip)
 // Tile of one interval: two FP16 planes (hi, lo), each KB x RB blocks of

@@ -24,8 +24,10 @@ from . import _lib
 from ._lib import check, lib
 from .synthetic import ModeSumInput
 
-PRECISION_CODES = {"fp64": _lib.FEW_PRECISION_FP64, "mixed": _lib.FEW_PRECISION_MIXED,
-                   "mixed16": _lib.FEW_PRECISION_MIXED16}
+# fp64: the reference arithmetic (direct sum); fact64: the factorised sum in
+# FP64 on the matrix cores; mixed: FP32 MFMA; mixed16: split-FP16 MFMA
+PRECISION_CODES = {"fp64": _lib.FEW_PRECISION_FP64, "fact64": _lib.FEW_PRECISION_FACT64,
+                   "mixed": _lib.FEW_PRECISION_MIXED, "mixed16": _lib.FEW_PRECISION_MIXED16}
 
 
 class ModePlan:
@@ -48,6 +50,7 @@ class ModePlan:
         self.row_blocks, self.ksteps = int(info[4]), int(info[5])
         self.supported = bool(info[6] & 1)
         self.supported16 = bool(info[6] & 2)
+        self.supported64 = bool(info[6] & 4)
 
     def workspace_bytes(self, L, N, precision) -> int:
         return int(lib().few_plan_workspace_bytes(self.handle, L, N, PRECISION_CODES[precision]))

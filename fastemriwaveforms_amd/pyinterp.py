@@ -35,8 +35,8 @@ from ._lib import check, lib
 __all__ = ["interpolate_arrays_wrap", "get_waveform_wrap", "get_waveform_generic_fd_wrap",
            "interp2D", "pyWaveform", "set_precision", "get_precision"]
 
-_PRECISION_CODES = {"fp64": _lib.FEW_PRECISION_FP64, "mixed": _lib.FEW_PRECISION_MIXED,
-                    "mixed16": _lib.FEW_PRECISION_MIXED16}
+_PRECISION_CODES = {"fp64": _lib.FEW_PRECISION_FP64, "fact64": _lib.FEW_PRECISION_FACT64,
+                    "mixed": _lib.FEW_PRECISION_MIXED, "mixed16": _lib.FEW_PRECISION_MIXED16}
 
 
 def _check_precision(p: str) -> str:
@@ -49,7 +49,8 @@ _precision = _check_precision(os.environ.get("FEW_ROCM_PRECISION", "fp64"))
 
 
 def set_precision(precision: str) -> None:
-    """Arithmetic of get_waveform_wrap: "fp64" (the reference's), "mixed16" or "mixed"."""
+    """Arithmetic of get_waveform_wrap: "fp64" (the reference's), "fact64" (the
+    factorised sum in FP64, rel-L2 ~3e-10), "mixed16" or "mixed" (~1e-7)."""
     global _precision
     _precision = _check_precision(precision)
 
@@ -192,7 +193,8 @@ def get_waveform_wrap(*args, **kwargs):
     plan = None
     if prec != "fp64" and K > 0:
         plan = _mode_plan(d_m, d_k, d_n)
-        if not (plan.supported16 if prec == "mixed16" else plan.supported):
+        ok = {"mixed16": plan.supported16, "fact64": plan.supported64}.get(prec, plan.supported)
+        if not ok:
             logging.getLogger("fastemriwaveforms_amd").warning(
                 "get_waveform_wrap: the mode set's tiles do not fit the %s sum; running FP64", prec)
             prec = "fp64"

@@ -26,8 +26,8 @@ from .backend import ParallelModuleBase
 
 YRSID_SI = 31558149.763545595  # cutils/global.h:23
 
-PRECISIONS = {"fp64": _lib.FEW_PRECISION_FP64, "mixed": _lib.FEW_PRECISION_MIXED,
-              "mixed16": _lib.FEW_PRECISION_MIXED16}
+PRECISIONS = {"fp64": _lib.FEW_PRECISION_FP64, "fact64": _lib.FEW_PRECISION_FACT64,
+              "mixed": _lib.FEW_PRECISION_MIXED, "mixed16": _lib.FEW_PRECISION_MIXED16}
 
 
 def _device() -> torch.device:
@@ -221,9 +221,10 @@ class InterpolatedModeSum(SummationBase):
     """Interpolated mode summation (summation/interpolatedmodesum.py:260-375).
 
     ``precision`` (constructor or ``sum`` keyword) selects the arithmetic:
-    ``"fp64"`` is the reference's plain FP64 sum; ``"mixed"`` (FP64 phases,
-    FP32 MFMA contraction) and ``"mixed16"`` (FP64 phases, split-FP16 MFMA with
-    FP32 accumulation) are the factorised sums (rel-L2 ~1e-7).  The mixed paths
+    ``"fp64"`` is the reference's plain FP64 sum; ``"fact64"`` (the factorised
+    sum in FP64 on the FP64 matrix cores, rel-L2 ~3e-10), ``"mixed"`` (FP64
+    phases, FP32 MFMA contraction) and ``"mixed16"`` (FP64 phases, split-FP16
+    MFMA with FP32 accumulation, rel-L2 ~1e-7) are the factorised sums.  They
     need a mode plan, built once per mode set and cached; a mode set whose tile
     does not fit raises :class:`FewRocmError`.
     """

@@ -27,7 +27,7 @@ extern "C" {
 #define FEW_ROCM_ABI_VERSION 1
 
 #define FEW_PRECISION_FP64 0   /* plain FP64 direct sum (reference arithmetic) */
-#define FEW_PRECISION_FACT64 1 /* factorised sum in FP64 on the matrix cores (adjoint) */
+#define FEW_PRECISION_FACT64 1 /* factorised sum in FP64 on the matrix cores (sum and adjoint) */
 #define FEW_PRECISION_MIXED 2  /* factorised MFMA sum: FP64 phases, FP32 MFMA, FP64 output */
 #define FEW_PRECISION_MIXED16 3 /* factorised sum on FP16 MFMA with split (hi+lo) operands */
 
@@ -124,8 +124,8 @@ int few_prepare_modes_async(const double *teuk, const double *phase_knots, const
  * current device once; reused by every call with the same mode set.
  * Returns NULL on error.  info7 = {K, rows, cols, n_lo, row blocks,
  * k-steps, supported}; supported is a bit mask (1: FEW_PRECISION_MIXED,
- * 2: FEW_PRECISION_MIXED16); unsupported sets (too many rows/columns for the
- * MFMA tiles) can still use FEW_PRECISION_FP64.
+ * 2: FEW_PRECISION_MIXED16, 4: FEW_PRECISION_FACT64); unsupported sets (too
+ * many rows/columns for the MFMA tiles) can still use FEW_PRECISION_FP64.
  */
 void *few_mode_plan_create(const int *h_m, const int *h_k, const int *h_n, int num_modes);
 int few_mode_plan_info(const void *plan, int *info7);
@@ -135,8 +135,12 @@ size_t few_plan_workspace_bytes(const void *plan, int init_len, int out_len, int
 /*
  * get_waveform (src/few/cutils/interpolate.cu:726-797) with a mode plan and a
  * precision choice, asynchronous on device-resident time arrays.
- * FEW_PRECISION_FP64 runs the reference arithmetic; FEW_PRECISION_MIXED the
- * factorised MFMA sum (rel-L2 ~1e-7 against the FP64 result).  `variant`
+ * FEW_PRECISION_FP64 runs the reference arithmetic; FEW_PRECISION_MIXED and
+ * FEW_PRECISION_MIXED16 the factorised MFMA sum (rel-L2 ~1e-7 against the
+ * FP64 result); FEW_PRECISION_FACT64 the factorised sum in FP64 on
+ * v_mfma_f64_16x16x4f64 (FP64 tiles, phasors and accumulation; it differs from
+ * the reference arithmetic only by the rounding of the phase products,
+ * rel-L2 ~1e-10; variant 0 = 16 samples per wave, 4 = 32).  `variant`
  * selects the kernel shape (0 = default).  FEW_PRECISION_MIXED: 0 = 32
  * samples per wave, 2 waves per SIMD; 4 = 64 samples per wave, 1 wave per
  * SIMD.  FEW_PRECISION_MIXED16, a bit set: 4 = 4 waves per workgroup (one
