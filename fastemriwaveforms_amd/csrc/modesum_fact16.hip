@@ -500,8 +500,8 @@ __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sP
             //   g = 1:     Phi_phi           -> e = exp(-2 i Phi_phi)      (epilogue)
             //   g = 2, 3:  Phi_phi, Phi_theta -> u = exp(-i((m0 + g - 2) Phi_phi
             //                                               + k0 Phi_theta)) (epilogue)
-            const double p1 = dense_phase(sPh + (g == 0 ? 16 : 0), s, 1.0 - s) * INV_2PI16;
-            const double p2 = P.has_k ? dense_phase(sPh + 8, s, 1.0 - s) * INV_2PI16 : 0.0;
+            const double p1 = dense_phase_fused(sPh + (g == 0 ? 16 : 0), s, 1.0 - s) * INV_2PI16;
+            const double p2 = P.has_k ? dense_phase_fused(sPh + 8, s, 1.0 - s) * INV_2PI16 : 0.0;
             const double arg = g == 0 ? p1
                              : g == 1 ? 2.0 * p1
                                       : (double)(P.m0 + g - 2) * p1 + (double)P.k0 * p2;
@@ -513,7 +513,7 @@ __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sP
             zsn = __shfl(sn, li);
             zcs = __shfl(cs, li);
         } else {
-            ur = dense_phase(sPh + 16, s, 1.0 - s) * INV_2PI16;
+            ur = dense_phase_fused(sPh + 16, s, 1.0 - s) * INV_2PI16;
             sincos_turns_short(reduce_turns(ur), zsn, zcs);
         }
         // lane group g needs v_n for n = n_lo + 16 kb + 4 g + t, t = 0..3 (scaled by 256)
@@ -604,8 +604,11 @@ __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigne
             // summed by Horner's rule in FP64.
             // e from lane group 1, u_first from group 2 + (g >> 1) (set up at setup)
             const double e_re = __shfl(V.pc[c], li + 16), e_im = -__shfl(V.ps[c], li + 16);
+            // the last row block starts the sum (no multiply of a zero sum)
+            hre = (double)fmaf(ka, acc[RB - 1][c][0], kb2 * acc[RB - 1][c][2]);
+            him = (double)fmaf(ka, acc[RB - 1][c][1], kb2 * acc[RB - 1][c][3]);
 #pragma unroll
-            for (int rb = RB - 1; rb >= 0; rb--) {
+            for (int rb = RB - 2; rb >= 0; rb--) {
                 // rows r >= Nr hold zeros (W = 0)
                 const v4f W = acc[rb][c];
                 const double zr = (double)fmaf(ka, W[0], kb2 * W[2]);
@@ -623,8 +626,8 @@ __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigne
             him = ni;
         } else {
             const double s = (t - sg.pt) * inv_pw;
-            const double uph = dense_phase(sPh, s, 1.0 - s) * INV_2PI16;
-            const double uth = P.has_k ? dense_phase(sPh + 8, s, 1.0 - s) * INV_2PI16 : 0.0;
+            const double uph = dense_phase_fused(sPh, s, 1.0 - s) * INV_2PI16;
+            const double uth = P.has_k ? dense_phase_fused(sPh + 8, s, 1.0 - s) * INV_2PI16 : 0.0;
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) {
                 const int r = 2 * rb + (g >> 1);
@@ -725,6 +728,8 @@ struct F16Fold {
     double er[F16_NB], ei[F16_NB], hr[F16_NB], hi[F16_NB];
 };
 
+// FIRST: the block starts the sum (as in f16_epilogue, no step from zero).
+template <bool FIRST>
 __device__ __forceinline__ void f16_fold(const v4f (&acc)[F16_NB], F16Fold &F)
 {
 #pragma unroll
@@ -732,10 +737,15 @@ __device__ __forceinline__ void f16_fold(const v4f (&acc)[F16_NB], F16Fold &F)
         const v4f W = acc[c];
         const double zr = (double)fmaf(F.ka[c], W[0], F.kb2[c] * W[2]);
         const double zi = (double)fmaf(F.ka[c], W[1], F.kb2[c] * W[3]);
-        const double nr = fma(F.hr[c], F.er[c], fma(-F.hi[c], F.ei[c], zr));
-        const double ni = fma(F.hr[c], F.ei[c], fma(F.hi[c], F.er[c], zi));
-        F.hr[c] = nr;
-        F.hi[c] = ni;
+        if constexpr (FIRST) {
+            F.hr[c] = zr;
+            F.hi[c] = zi;
+        } else {
+            const double nr = fma(F.hr[c], F.er[c], fma(-F.hi[c], F.ei[c], zr));
+            const double ni = fma(F.hr[c], F.ei[c], fma(F.hi[c], F.er[c], zi));
+            F.hr[c] = nr;
+            F.hi[c] = ni;
+        }
     }
 }
 
@@ -788,8 +798,6 @@ __device__ __forceinline__ void f16s_tile(const FactPlanDev &P, const unsigned c
         F.kb2[c] = spb * sInv[p0 + 1];
         F.er[c] = __shfl(V.pc[c], li + 16);
         F.ei[c] = -__shfl(V.ps[c], li + 16);
-        F.hr[c] = 0.0;
-        F.hi[c] = 0.0;
     }
     F16Bfrag<KB> Bf;
     f16_split_all<KB>(V, Bf);
@@ -799,24 +807,34 @@ __device__ __forceinline__ void f16s_tile(const FactPlanDev &P, const unsigned c
     const int kstride = RB * F16_BLK;
     v4f accA[F16_NB], accB[F16_NB];
     if (prio) __builtin_amdgcn_s_setprio(1);
+    // row blocks from the last to the first; the fold of block rb runs
+    // under the MFMAs of block rb - 1
     int rb = RB - 1;
     f16_rows16<KB>(aH + rb * F16_BLK, aL + rb * F16_BLK, kstride, Bf, accA);
-    while (rb >= 2) {
-        f16_rows16<KB>(aH + (rb - 1) * F16_BLK, aL + (rb - 1) * F16_BLK, kstride, Bf, accB);
-        f16_fold(accA, F);
-        f16s_interleave<KB>();
-        f16_rows16<KB>(aH + (rb - 2) * F16_BLK, aL + (rb - 2) * F16_BLK, kstride, Bf, accA);
-        f16_fold(accB, F);
-        f16s_interleave<KB>();
-        rb -= 2;
-    }
-    if (rb == 1) {
-        f16_rows16<KB>(aH, aL, kstride, Bf, accB);
-        f16_fold(accA, F);
-        f16s_interleave<KB>();
-        f16_fold(accB, F);
+    if (rb == 0) {
+        f16_fold<true>(accA, F);
     } else {
-        f16_fold(accA, F);
+        f16_rows16<KB>(aH + (rb - 1) * F16_BLK, aL + (rb - 1) * F16_BLK, kstride, Bf, accB);
+        f16_fold<true>(accA, F);
+        f16s_interleave<KB>();
+        rb -= 1;  // accB holds block rb
+        while (rb >= 2) {
+            f16_rows16<KB>(aH + (rb - 1) * F16_BLK, aL + (rb - 1) * F16_BLK, kstride, Bf, accA);
+            f16_fold<false>(accB, F);
+            f16s_interleave<KB>();
+            f16_rows16<KB>(aH + (rb - 2) * F16_BLK, aL + (rb - 2) * F16_BLK, kstride, Bf, accB);
+            f16_fold<false>(accA, F);
+            f16s_interleave<KB>();
+            rb -= 2;
+        }
+        if (rb == 1) {
+            f16_rows16<KB>(aH, aL, kstride, Bf, accA);
+            f16_fold<false>(accB, F);
+            f16s_interleave<KB>();
+            f16_fold<false>(accA, F);
+        } else {
+            f16_fold<false>(accB, F);
+        }
     }
     if (prio) __builtin_amdgcn_s_setprio(0);
 #pragma unroll
