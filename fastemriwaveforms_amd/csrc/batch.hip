@@ -141,7 +141,7 @@ void *few_batch_create(int B, const int *h_L, const int *h_K, const int *h_N, co
         bp->tot[4] += 4L * L * 2 * K;
         bp->tot[5] += 8L * (L - 1) * 3;
         bp->tot[6] += 2L * N;
-        bp->tot[7] += 4L * L * (2 * K + 3);
+        bp->tot[7] += batch_spline_scratch(L, K);
         bp->total_intervals += L - 1;
         bp->total_tpref += L;
         bp->total_systems += 2L * K + 3;

@@ -29,8 +29,20 @@ struct WaveDesc {
     long interp_off;      // amplitude coefficients (4, L, 2K)
     long pc_off;          // dense-output phase coefficients (8, L-1, 3)
     long wave_off;        // output, complex (N) -> doubles
-    long sc_off;          // spline scratch (4 * L * (2K+3) doubles)
+    long sc_off;          // spline scratch (batch_spline_scratch(L, K) doubles)
 };
+
+// Spline scratch of one waveform: its knot matrix eliminated once (dgtsv's
+// piv | fact | d | du | dl, 5L doubles) and that elimination's INFO, padded
+// to 64 bytes, then four (L, 2K+3) planes.  Every system of the waveform
+// shares the matrix; with INFO = 0 the systems sweep their right-hand sides
+// in the first plane, otherwise each system runs dgtsv on its own in all four
+// (LAPACK's early return depends on the right-hand side).
+__host__ __device__ inline long batch_spline_head(long L) { return (5 * L + 1 + 7) & ~7L; }
+__host__ __device__ inline long batch_spline_scratch(long L, long K)
+{
+    return batch_spline_head(L) + 4 * L * (2 * K + 3);
+}
 
 // Device tables of a batch plan (one allocation).
 struct BatchDev {

@@ -383,89 +383,128 @@ __host__ __device__ inline int spline_factor(const double *x, int n, double *F)
     return dgtsv_factor(n, d, du, dl, piv, fact);
 }
 
-// One system on shared knots x with the recorded elimination F, in passes
-// whose iterations are independent wherever the algorithm allows it (so the
-// divisions of neighbouring rows overlap instead of forming one chain):
-//   spline_rhs_pass:   prep_row's right-hand side of every row -> b
-//   spline_solve_emit: dgtsv's forward sweep of b with the recorded
-//                      multipliers, back substitution (slopes into b), then
-//                      the coefficients of every interval and the final
-//                      derivatives.
-// y(i) = yv[i*ys], b(i) = bv[i*bs].  Same operations as spline_system_core
-// (bit-identical).
-__host__ __device__ inline void spline_rhs_pass(int n, const double *x, const double *yv, long ys,
-                                                double *bv, long bs)
+// One system on shared knots x with the recorded elimination F
+// (spline_factor's piv | fact | d | du | dl), its right-hand side in the
+// column bv (stride bs) as scratch:
+//   forward: prep_row's right-hand side of row r = i + 1 formed on the fly
+//            and swept with step i (dgtsv's no-interchange / interchange
+//            update), one store per row;
+//   back:    the back substitution, emitting interval ii's coefficients as
+//            soon as its left slope is known, one load per row.
+// The operations of spline_system_core in the same order for every value
+// (bit-identical), without its per-system elimination.  The knot, value and
+// elimination operands of the next PF steps are read ahead of the dependent
+// chains (PF changes the schedule, not the arithmetic).
+template <int PF>
+__host__ __device__ inline void spline_system_shared(const SplineGroup &G, int s, int n,
+                                                     const double *x, const double *F,
+                                                     const double *yv, long ys, double *bv,
+                                                     long bs)
 {
 #pragma clang fp contract(off)
-#pragma unroll 4
-    for (int r = 0; r < n; r++) {
-        const int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
-        double b_, u_, g_, l_;
-        prep_row(r, n, x[i0], x[i0 + 1], x[i0 + 2], yv[(long)i0 * ys], yv[(long)(i0 + 1) * ys],
-                 yv[(long)(i0 + 2) * ys], b_, u_, g_, l_);
-        bv[(long)r * bs] = b_;
-    }
-}
-
-__host__ __device__ inline void spline_solve_emit(const SplineGroup &G, int s, int n,
-                                                  const double *x, const double *F,
-                                                  const double *yv, long ys, double *bv, long bs)
-{
-#pragma clang fp contract(off)
-#define SY(i) yv[(long)(i) * ys]
 #define SB(i) bv[(long)(i) * bs]
+#define YV(i) yv[(long)(i) * ys]
     const double *piv = F, *fact = F + n, *d = F + 2 * n, *du = F + 3 * n, *dl = F + 4 * n;
-    double cur = SB(0);
-    for (int i = 0; i < n - 1; i++) {
-        const double nb = SB(i + 1);
-        if (piv[i] == 0.0) {
-            SB(i) = cur;
-            cur = nb - fact[i] * cur;
-        } else {
-            SB(i) = nb;
-            cur = cur - fact[i] * nb;
+    double xa = x[0], xb = x[1], xc = x[2];
+    double ya = YV(0), yb = YV(1), yc = YV(2);
+    double b_, u_, g_, l_;
+    prep_row(0, n, xa, xb, xc, ya, yb, yc, b_, u_, g_, l_);
+    double cur = b_;
+    // step i with row r = i + 1 and its knot window
+    auto step = [&](int i) {
+        prep_row(i + 1, n, xa, xb, xc, ya, yb, yc, b_, u_, g_, l_);
+        const bool keep = piv[i] == 0.0;
+        SB(i) = keep ? cur : b_;
+        cur = keep ? b_ - fact[i] * cur : cur - fact[i] * b_;
+    };
+    step(0);  // row 1 reads the window of row 0
+    // rows 2 .. n-2 read (x[r-1], x[r], x[r+1]): shift in knot i + 2 at step i
+    int i = 1;
+    for (; i + PF - 1 <= n - 3; i += PF) {
+        double px[PF], py[PF], pp[PF], pf[PF];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            px[u] = x[i + u + 2];
+            py[u] = YV(i + u + 2);
+            pp[u] = piv[i + u];
+            pf[u] = fact[i + u];
+        }
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            xa = xb; xb = xc; xc = px[u];
+            ya = yb; yb = yc; yc = py[u];
+            prep_row(i + u + 1, n, xa, xb, xc, ya, yb, yc, b_, u_, g_, l_);
+            const bool keep = pp[u] == 0.0;
+            SB(i + u) = keep ? cur : b_;
+            cur = keep ? b_ - pf[u] * cur : cur - pf[u] * b_;
         }
     }
+    for (; i <= n - 3; i++) {
+        xa = xb; xb = xc; xc = x[i + 2];
+        ya = yb; yb = yc; yc = YV(i + 2);
+        step(i);
+    }
+    step(n - 2);  // the last row reads the window of row n-2
+    // back substitution + coefficients; slope(n-1) = cur / d[n-1]
     double x_next = cur / d[n - 1];
-    SB(n - 1) = x_next;
-    double x_next2 = 0.0;
-    for (int ii = n - 2; ii >= 0; ii--) {
-        double xi;
-        if (ii == n - 2)
-            xi = (SB(ii) - du[ii] * x_next) / d[ii];
-        else
-            xi = (SB(ii) - du[ii] * x_next - dl[ii] * x_next2) / d[ii];
-        SB(ii) = xi;
+    double t1 = x[n - 1], y1 = YV(n - 1);
+    const double yl = y1;
+    double c1L = 0.0, c2L = 0.0, c3L = 0.0;
+    // row n-2 has no second super-diagonal term
+    double x_next2 = x_next;
+    {
+        const double t0 = x[n - 2], y0 = YV(n - 2);
+        const double xi = (SB(n - 2) - du[n - 2] * x_next) / d[n - 2];
+        spline_emit(G, s, n, n - 2, xi, x_next, t0, t1, y0, y1, c1L, c2L, c3L);
+        t1 = t0;
+        y1 = y0;
+        x_next = xi;
+    }
+    int ii = n - 3;
+    for (; ii - (PF - 1) >= 0; ii -= PF) {
+        double rb[PF], ru[PF], rl[PF], rd[PF], rx[PF],
+            ry[PF];
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            rb[u] = SB(ii - u);
+            ru[u] = du[ii - u];
+            rl[u] = dl[ii - u];
+            rd[u] = d[ii - u];
+            rx[u] = x[ii - u];
+            ry[u] = YV(ii - u);
+        }
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const double xi = (rb[u] - ru[u] * x_next - rl[u] * x_next2) / rd[u];
+            spline_emit(G, s, n, ii - u, xi, x_next, rx[u], t1, ry[u], y1, c1L, c2L, c3L);
+            t1 = rx[u];
+            y1 = ry[u];
+            x_next2 = x_next;
+            x_next = xi;
+        }
+    }
+    for (; ii >= 0; ii--) {
+        const double t0 = x[ii], y0 = YV(ii);
+        const double xi = (SB(ii) - du[ii] * x_next - dl[ii] * x_next2) / d[ii];
+        spline_emit(G, s, n, ii, xi, x_next, t0, t1, y0, y1, c1L, c2L, c3L);
+        t1 = t0;
+        y1 = y0;
         x_next2 = x_next;
         x_next = xi;
     }
-    double c1L = 0.0, c2L = 0.0, c3L = 0.0;
-#pragma unroll 4
-    for (int ii = 0; ii < n - 1; ii++)
-        spline_emit(G, s, n, ii, SB(ii), SB(ii + 1), x[ii], x[ii + 1], SY(ii), SY(ii + 1), c1L,
-                    c2L, c3L);
     if (!G.dense) {
         // fill_final_derivs (interpolate.cu:242-280)
         const double xx = x[n - 1] - x[n - 2];
         const double x2 = xx * xx;
         double *out = G.out;
         const long o = (long)s * G.o_sys + (long)(n - 1) * G.o_knot;
-        out[o] = SY(n - 1);
+        out[o] = yl;
         out[G.o_coef + o] = c1L + 2 * c2L * xx + 3 * c3L * x2;
         out[2 * G.o_coef + o] = (2. * c2L + 6. * c3L * xx) / 2.;
         out[3 * G.o_coef + o] = c3L;
     }
-#undef SY
 #undef SB
-}
-
-__host__ __device__ inline void spline_system_shared(const SplineGroup &G, int s, int n,
-                                                     const double *x, const double *F,
-                                                     const double *yv, long ys, double *bv,
-                                                     long bs)
-{
-    spline_rhs_pass(n, x, yv, ys, bv, bs);
-    spline_solve_emit(G, s, n, x, F, yv, ys, bv, bs);
+#undef YV
 }
 
 __host__ __device__ void spline_one_system(const SplineArgs &a, int gi, int s)
@@ -618,42 +657,108 @@ void fill_spline_args(const SplineGroup *groups, int ngroups, int length, double
 
 // Batched build: every system of every waveform (2K amplitude arrays, then
 // the 3 phase arrays in dense-output form unless the phases come as dense
-// output already, p.pk == nullptr), one lane per system.
+// output already, p.pk == nullptr).  All systems of a waveform share its knot
+// matrix, so it is eliminated once per waveform (batch_spline_factor_kernel)
+// and each system (one lane) only sweeps its right-hand side with the
+// recorded steps: one scratch plane per system instead of four, and no
+// division of the elimination per system.
+
+// One 64-lane block per waveform: the matrix rows (prep_row with y = 0, one
+// lane per row), then dgtsv's elimination on lane 0; INFO after the five
+// arrays.
+__global__ __launch_bounds__(64) void batch_spline_factor_kernel(BatchDev B, BatchPtrs p)
+{
+    const WaveDesc d = B.desc[blockIdx.x];
+    const int n = d.L;
+    const double *x = p.t + d.t_off;
+    double *F = p.scratch + d.sc_off;
+    double *piv = F, *fact = F + n, *dg = F + 2 * n, *du = F + 3 * n, *dl = F + 4 * n;
+    for (int r = threadIdx.x; r < n; r += 64) {
+        const int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
+        double b_, u_, g_, l_;
+        prep_row(r, n, x[i0], x[i0 + 1], x[i0 + 2], 0.0, 0.0, 0.0, b_, u_, g_, l_);
+        dg[r] = g_;
+        if (r < n - 1) du[r] = u_;
+        if (r > 0) dl[r - 1] = l_;
+    }
+    __syncthreads();  // the rows of every lane, for lane 0's chain
+    if (threadIdx.x == 0) {
+        const int info = dgtsv_factor_ahead<8>(n, dg, du, dl, piv, fact);
+        *reinterpret_cast<int *>(F + 5 * n) = info;
+    }
+}
+
+// System sg of the batch: its group (amplitude real / imaginary parts or
+// phases), index in the group and waveform.  False for phase systems when
+// the phases come as dense output (nothing to build).
+struct BatchSystem {
+    SplineGroup G;
+    int s;
+    long ls;
+    WaveDesc d;
+};
+
+__device__ inline bool batch_system(const BatchDev &B, const BatchPtrs &p, long sg, BatchSystem &S)
+{
+    const int b = upper_index(B.sys_prefix, B.B, sg);
+    S.d = B.desc[b];
+    S.ls = sg - B.sys_prefix[b];
+    const long K = S.d.K, L = S.d.L;
+    const double *teuk = p.teuk + S.d.teuk_off;
+    const double *t = p.t + S.d.t_off;
+    double *interp = p.interp + S.d.interp_off;
+    if (S.ls < K) {
+        S.G = SplineGroup{teuk, 2, 2 * K, t, 0, 1, interp, L * 2 * K, 1, 2 * K, (int)K, 0};
+        S.s = (int)S.ls;
+    } else if (S.ls < 2 * K) {
+        S.G = SplineGroup{teuk + 1, 2, 2 * K, t, 0, 1, interp + K, L * 2 * K, 1, 2 * K, (int)K, 0};
+        S.s = (int)(S.ls - K);
+    } else {
+        if (!p.pk) return false;
+        S.G = SplineGroup{p.pk + S.d.pk_off, L, 1, t, 0, 1, p.pcoef + S.d.pc_off, 0, 0, 0, 3, 1};
+        S.s = (int)(S.ls - 2 * K);
+    }
+    return true;
+}
+
+// One lane per system of a waveform whose matrix was eliminated (INFO = 0).
 __global__ __launch_bounds__(64) void batch_spline_kernel(BatchDev B, BatchPtrs p)
 {
     const long sg = (long)blockIdx.x * 64 + threadIdx.x;
-    if (sg >= B.total_systems) return;
-    const int b = upper_index(B.sys_prefix, B.B, sg);
-    const WaveDesc d = B.desc[b];
-    const long ls = sg - B.sys_prefix[b];
-    const long K = d.K, L = d.L;
-    const double *teuk = p.teuk + d.teuk_off;
-    const double *t = p.t + d.t_off;
-    double *interp = p.interp + d.interp_off;
-    SplineGroup G;
-    int s;
-    if (ls < K) {
-        G = SplineGroup{teuk, 2, 2 * K, t, 0, 1, interp, L * 2 * K, 1, 2 * K, (int)K, 0};
-        s = (int)ls;
-    } else if (ls < 2 * K) {
-        G = SplineGroup{teuk + 1, 2, 2 * K, t, 0, 1, interp + K, L * 2 * K, 1, 2 * K, (int)K, 0};
-        s = (int)(ls - K);
-    } else {
-        if (!p.pk) return;  // dense-output phases handed over in pcoef: nothing to build
-        G = SplineGroup{p.pk + d.pk_off, L, 1, t, 0, 1, p.pcoef + d.pc_off, 0, 0, 0, 3, 1};
-        s = (int)(ls - 2 * K);
-    }
-    const long f = L * (2 * K + 3);
-    double *sc = p.scratch + d.sc_off;
-    SplineScratch scr{sc, sc + f, sc + 2 * f, sc + 3 * f, nullptr};
-    spline_system_core(G, s, (int)L, ls, 2 * K + 3, scr);
+    BatchSystem S;
+    if (sg >= B.total_systems || !batch_system(B, p, sg, S)) return;
+    const long L = S.d.L;
+    const double *F = p.scratch + S.d.sc_off;
+    if (*reinterpret_cast<const int *>(F + 5 * L) != 0) return;
+    double *plane = p.scratch + S.d.sc_off + batch_spline_head(L);
+    spline_system_shared<2>(S.G, S.s, (int)L, S.G.t, F, S.G.y + (long)S.s * S.G.y_sys, S.G.y_knot,
+                            plane + S.ls, 2 * S.d.K + 3);
+}
+
+// The systems of waveforms whose matrix is singular: LAPACK returns early
+// at a step that the right-hand side's values decide, so each system runs
+// dgtsv on its own (a separate kernel keeps batch_spline_kernel's register
+// budget that of the swept path).
+__global__ __launch_bounds__(64) void batch_spline_single_kernel(BatchDev B, BatchPtrs p)
+{
+    const long sg = (long)blockIdx.x * 64 + threadIdx.x;
+    BatchSystem S;
+    if (sg >= B.total_systems || !batch_system(B, p, sg, S)) return;
+    const long L = S.d.L, ns = 2 * S.d.K + 3;
+    if (*reinterpret_cast<const int *>(p.scratch + S.d.sc_off + 5 * L) == 0) return;
+    double *planes = p.scratch + S.d.sc_off + batch_spline_head(L);
+    const long f = L * ns;
+    SplineScratch scr{planes, planes + f, planes + 2 * f, planes + 3 * f, nullptr};
+    spline_system_core(S.G, S.s, (int)L, S.ls, ns, scr);
 }
 
 int launch_batch_splines(const BatchDev &B, const BatchPtrs &p, hipStream_t stream)
 {
     if (B.total_systems <= 0) return FEW_OK;
     const long blocks = (B.total_systems + 63) / 64;
+    hipLaunchKernelGGL(batch_spline_factor_kernel, dim3((unsigned)B.B), dim3(64), 0, stream, B, p);
     hipLaunchKernelGGL(batch_spline_kernel, dim3((unsigned)blocks), dim3(64), 0, stream, B, p);
+    hipLaunchKernelGGL(batch_spline_single_kernel, dim3((unsigned)blocks), dim3(64), 0, stream, B, p);
     FEW_HIP(hipGetLastError());
     return FEW_OK;
 }

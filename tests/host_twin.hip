@@ -74,7 +74,7 @@ int twin_prepare_modes(const double *teuk, const double *phase_knots, const doub
 }
 
 // The shared-knot path of the spline build (spline_factor + spline_system_shared,
-// as spline_block_shared runs them), system by system on the host.
+// as the batched build runs them per waveform), system by system on the host.
 int twin_prepare_modes_shared(const double *teuk, const double *phase_knots, const double *t,
                               int length, int num_modes, double *interp_out, double *pcoef_out)
 {
@@ -88,7 +88,7 @@ int twin_prepare_modes_shared(const double *teuk, const double *phase_knots, con
     if (info) return info;
     for (int gi = 0; gi < 3; gi++)
         for (int s = 0; s < g[gi].nsys; s++)
-            spline_system_shared(g[gi], s, length, t, F.data(), g[gi].y + (long)s * g[gi].y_sys,
+            spline_system_shared<8>(g[gi], s, length, t, F.data(), g[gi].y + (long)s * g[gi].y_sys,
                                  g[gi].y_knot, b.data(), 1);
     return 0;
 }
@@ -339,11 +339,16 @@ void twin_spline_block_pieces(const double *x, const double *y, int n, double *F
     }
     info[1] = dgtsv_factor_ahead<8>(n, F_ahead + 2 * n, F_ahead + 3 * n, F_ahead + 4 * n, F_ahead,
                                     F_ahead + n);
-    spline_rhs_pass(n, x, y, 1, slopes, 1);
+    for (int r = 0; r < n; r++) {  // prep_row's right-hand sides
+        const int i0 = (r == 0) ? 0 : (r == n - 1 ? n - 3 : r - 1);
+        double b_, u_, g_, l_;
+        prep_row(r, n, x[i0], x[i0 + 1], x[i0 + 2], y[i0], y[i0 + 1], y[i0 + 2], b_, u_, g_, l_);
+        slopes[r] = b_;
+    }
     if (info[1] == 0) dgtsv_sweeps(n, F_ahead, slopes, 1);
     std::vector<double> out(4 * (size_t)n), b(n);
     SplineGroup G{y, 0, 1, x, 0, 1, out.data(), n, 0, 1, 1, 0};
-    if (info[0] == 0) spline_system_shared(G, 0, n, x, F_plain, y, 1, b.data(), 1);
+    if (info[0] == 0) spline_system_shared<4>(G, 0, n, x, F_plain, y, 1, b.data(), 1);
     for (int i = 0; i < n - 1; i++) c1[i] = out[n + i];
 }
 
