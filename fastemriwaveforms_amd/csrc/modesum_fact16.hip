@@ -128,6 +128,29 @@ __device__ void fact16_rowmax_block(const FactPlanDev &P, const double *interp, 
     if (lane < 4) maxes[r * 4 + lane] = lane == 0 ? mx[0] : lane == 1 ? mx[1] : lane == 2 ? mx[2] : mx[3];
 }
 
+// The split of one cell's powers into the two planes (rows 2R: (Re, -Im),
+// 2R + 1: (Im, Re) of R = 4 r + p).  The two halfs of a row's column pair are
+// adjacent (even offset): one 32-bit store each.
+__device__ __forceinline__ void f16_store_cell(_Float16 *hi, _Float16 *lo, int RB, int r, int col,
+                                               const double (&cre)[4], const double (&cim)[4],
+                                               const double (&sigma)[4])
+{
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int R = 4 * r + p;
+        const float xr = f32_opaque((float)(cre[p] * sigma[p]));
+        const float xi = f32_opaque((float)(cim[p] * sigma[p]));
+        const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
+        const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
+        const long o0 = f16_tile_off(2 * R, 2 * col, RB);
+        const long o1 = f16_tile_off(2 * R + 1, 2 * col, RB);
+        *reinterpret_cast<v2h *>(hi + o0) = v2h{rh, (_Float16)-ih};
+        *reinterpret_cast<v2h *>(lo + o0) = v2h{rl, (_Float16)-il};
+        *reinterpret_cast<v2h *>(hi + o1) = v2h{ih, rh};
+        *reinterpret_cast<v2h *>(lo + o1) = v2h{il, rl};
+    }
+}
+
 // Pass 2: the interval's scale per power from the row maxima, then the split
 // planes of the workgroup's rows.
 __device__ void fact16_split_block(const FactPlanDev &P, const double *interp, const double2 *ylm,
@@ -162,19 +185,7 @@ __device__ void fact16_split_block(const FactPlanDev &P, const double *interp, c
     for (int col = lane; col < ncolc; col += 64) {
         double cre[4], cim[4];
         f16_cell(P, interp, ylm, L, j, hw, r, col, cre, cim);
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-            const int R = 4 * r + p;
-            const float xr = f32_opaque((float)(cre[p] * sigma[p]));
-            const float xi = f32_opaque((float)(cim[p] * sigma[p]));
-            const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
-            const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
-            // row 2R:   (Re, -Im);  row 2R+1: (Im, Re)
-            const long o0 = f16_tile_off(2 * R, 2 * col, RB);
-            const long o1 = f16_tile_off(2 * R + 1, 2 * col, RB);
-            hi[o0] = rh; hi[o0 + 1] = -ih; lo[o0] = rl; lo[o0 + 1] = -il;
-            hi[o1] = ih; hi[o1 + 1] = rh;  lo[o1] = il; lo[o1 + 1] = rl;
-        }
+        f16_store_cell(hi, lo, RB, r, col, cre, cim, sigma);
     }
 }
 
@@ -183,26 +194,6 @@ __device__ void fact16_split_block(const FactPlanDev &P, const double *interp, c
 // where one workgroup per interval already fills the device.
 #define F16_PREP_THREADS 512
 #define F16_PREP_WHOLE_MIN 1024  // intervals from which a launch takes this form
-
-// The split of one cell's powers into the two planes (rows 2R: (Re, -Im),
-// 2R + 1: (Im, Re) of R = 4 r + p).
-__device__ __forceinline__ void f16_store_cell(_Float16 *hi, _Float16 *lo, int RB, int r, int col,
-                                               const double (&cre)[4], const double (&cim)[4],
-                                               const double (&sigma)[4])
-{
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const int R = 4 * r + p;
-        const float xr = f32_opaque((float)(cre[p] * sigma[p]));
-        const float xi = f32_opaque((float)(cim[p] * sigma[p]));
-        const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
-        const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
-        const long o0 = f16_tile_off(2 * R, 2 * col, RB);
-        const long o1 = f16_tile_off(2 * R + 1, 2 * col, RB);
-        hi[o0] = rh; hi[o0 + 1] = -ih; lo[o0] = rl; lo[o0 + 1] = -il;
-        hi[o1] = ih; hi[o1 + 1] = rh;  lo[o1] = il; lo[o1 + 1] = rl;
-    }
-}
 
 // The interval's scales from the per-wave maxima in s_red; writes the inverse
 // scales after the planes (thread 0).
