@@ -16,7 +16,7 @@ The headline path is the mixed-precision factorised sum (FP64 phases and
 spline build; the n-contraction on FP16 MFMA with hi/lo split operands and
 FP32 accumulation); the plain FP64 GPU sum (the reference's arithmetic) is
 timed beside it as the denominator of the speed-up.  Further legs: the single
-full-year waveform (config 1, all 3843 modes), config 2 (2 yr at 5 s, 2000
+full-year waveform (all 3843 modes), the 2 yr waveform at 5 s (2000
 modes with n to 55), the differentiation (forward + backward), the
 frequency-domain sum, AAK, interp2D.  The oracle (oracle/, test
 infrastructure) runs only in the CPU leg: as the CPU baseline and as the
@@ -45,7 +45,7 @@ FACT16_FULL1_TRAFFIC = 2.00 * 16244.8 * 1024 + 1.00 * 49313.2 * 1024
 # rocprofv3 PMC runs (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a 1 GiB
 # copy in the same process)
 BATCH_FACT16_TRAFFIC = 2.00 * 1391768.4 * 1024 + 1.00 * 13008271.3 * 1024
-BATCH_TRAFFIC_SOURCE = "profiles/batch_fact16_config3_hbm_traffic.md (rocprofv3 PMC)"
+BATCH_TRAFFIC_SOURCE = "profiles/batch_fact16_256wave_hbm_traffic.md (rocprofv3 PMC)"
 DIRECT_FLOP = 37  # algorithmic FLOP per mode-sample of the direct sum (sincos counted 0)
 
 
@@ -350,7 +350,7 @@ def physical_cores():
 
 
 def mixed2_leg(dev, steps: int = 4):
-    """Config 2, mixed precision at larger scale: T = 2 yr, dt = 5 s (12,623,260 samples),
+    """Mixed precision at larger scale: T = 2 yr, dt = 5 s (12,623,260 samples),
     200 knots, 2000 of the 12,987 modes (|m| <= l <= 10, n in [-55, 55]); the mixed16 step
     (same stages as the headline) timed, every sample compared with the plain FP64 GPU path
     (rel-L2 and mismatch), spread samples kept for the oracle in the CPU leg."""
@@ -367,7 +367,7 @@ def mixed2_leg(dev, steps: int = 4):
     h64 = prob.wave.cpu().numpy()
     fa, fb = np.fft.fft(h), np.fft.fft(h64)
     overlap = np.vdot(fa, fb) / np.sqrt(np.vdot(fa, fa) * np.vdot(fb, fb))
-    out = {"workload": f"config 2: T=2 yr, dt=5 s, N={N} samples, L={inp.L} knots, K={K} modes "
+    out = {"workload": f"T=2 yr, dt=5 s, N={N} samples, L={inp.L} knots, K={K} modes "
                        f"(n in [-55, 55]), plan rows={prob.plan.rows} cols={prob.plan.cols}",
            "precision": prec, "ms_per_step": ms_step, "kernel_ms": ms,
            "mode_samples_per_s": K * N / (ms_step * 1e-3),
@@ -771,7 +771,7 @@ def fact16_roofline(kernel, rows, cols, row_blocks, samples, sum_ms, traffic=Non
 
 
 def single_leg(dev, args):
-    """Config 1, one waveform: T = 1 yr, dt = 10 s (3,155,815 samples), 120 knots, all
+    """One full-year waveform: T = 1 yr, dt = 10 s (3,155,815 samples), 120 knots, all
     3843 modes; the step captured as a HIP graph and replayed; the plain FP64 GPU
     path beside it."""
     from fastemriwaveforms_amd import synthetic
@@ -788,7 +788,7 @@ def single_leg(dev, args):
         f"spline {ms['spline']:.3f} prepare {ms['prepare']:.3f} sum {ms['sum']:.3f} ms | "
         f"{value:.4g} mode-samples/s")
     idx = spread_indices(N, count=1500, seed=5)
-    out = {"workload": f"config 1: T={inp.T_years} yr, dt={inp.dt} s, N={N} samples, "
+    out = {"workload": f"T={inp.T_years} yr, dt={inp.dt} s, N={N} samples, "
                        f"L={inp.L} knots, K={K} modes",
            "precision": prec, "ms_per_step": ms_step, "eager_ms_per_step": ms_eager,
            "kernel_ms": ms, "mode_samples_per_s": value, "waveforms_per_s": 1e3 / ms_step,
@@ -852,9 +852,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--single-steps", type=int, default=100,
-                    help="timed steps of the single-waveform (config 1) leg")
+                    help="timed steps of the single full-year waveform leg")
     ap.add_argument("--single-warmup", type=int, default=50)
-    ap.add_argument("--no-single", action="store_true", help="skip the config-1 leg")
+    ap.add_argument("--no-single", action="store_true", help="skip the single full-year waveform leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 controls")
     ap.add_argument("--no-adjoint", action="store_true",
@@ -862,7 +862,7 @@ def main():
     ap.add_argument("--no-fd", action="store_true",
                     help="skip the frequency-domain sum leg")
     ap.add_argument("--no-mixed2", action="store_true",
-                    help="skip the config-2 leg (2 yr at 5 s, 200 knots, 2000 modes, n to 55)")
+                    help="skip the 2 yr waveform leg (5 s, 200 knots, 2000 modes, n to 55)")
     ap.add_argument("--no-aak", action="store_true",
                     help="skip the AAK and interp2D legs")
     ap.add_argument("--fd-coords", type=int, default=64,
@@ -982,7 +982,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64 phases + split-f16 MFMA (f32 acc)",
         "data": "synthetic, seeded (fastemriwaveforms_amd/synthetic.py)",
-        "config": {"workload": f"config 3: {batch['waveforms']} uneven waveforms, durations "
+        "config": {"workload": f"{batch['waveforms']} uneven waveforms, durations "
                                "U(0.1, 2) yr at dt = 10 s, knots U{50..200}, modes U{50..3000} "
                                f"of the 3843-mode set; {batch['samples']} samples, "
                                f"{batch['mode_samples']:.4g} mode-samples, "

@@ -349,7 +349,7 @@ def test_fact64_adjoint_vs_oracle(gpu, name):
 
 
 def test_adjoint_full1_accuracy_by_backward(gpu):
-    """Config 1 at full size with the differentiation workload's loss Re<d, h>,
+    """The full-year waveform (all 3843 modes) with the differentiation loss Re<d, h>,
     d = h + complex Gaussian noise (0.1 rms(h)): every backward against the FP64
     per-mode adjoint.  The FP64 factorised moments agree to FP64 level, phase
     knots included (bar 1e-6).  The split-FP16 moments give the amplitude

@@ -219,7 +219,7 @@ def test_batch_many_intervals(gpu):
         assert oracle.rel_l2(bp.waveform(b).cpu().numpy(), ref) < 1e-6
 
 
-def test_batch_config3_full_size(gpu):
+def test_batch_256_uneven_full_size(gpu):
     """The 256-waveform uneven batch at its real size (durations U(0.1, 2) yr at
     dt = 10 s, knots U{50..200}, U{50..3000} of the 3843 modes; 8.3e8 samples):
     mixed16 against the FP64 control on every sample, and 8 waveforms spread over
@@ -234,7 +234,7 @@ def test_batch_config3_full_size(gpu):
     h16 = bp.run("mixed16").clone()
     h64 = bp.run("fp64")
     err = float(torch.linalg.vector_norm(h16 - h64) / torch.linalg.vector_norm(h64))
-    print(f"config 3: {bp.samples} samples, mixed16 vs fp64 (all samples) {err:.3e}")
+    print(f"256 uneven waveforms: {bp.samples} samples, mixed16 vs fp64 (all samples) {err:.3e}")
     assert err < 1e-6
     for b in np.linspace(0, 255, 8).astype(int):
         x = inputs[b]

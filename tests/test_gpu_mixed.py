@@ -138,8 +138,8 @@ def test_overwrite_equals_zero_then_accumulate(gpu, prec, tail):
     assert torch.equal(p.wave, ref)
 
 
-def test_mixed16_config2_full_size(gpu):
-    """Config 2 at its real size (T = 2 yr at dt = 5 s: 12,623,260 samples; 200
+def test_mixed16_2yr_full_size(gpu):
+    """The 2 yr waveform at its real size (T = 2 yr at dt = 5 s: 12,623,260 samples; 200
     knots; 2000 modes with n in [-55, 55]): the split-FP16 sum against the FP64
     GPU path on every sample and against the oracle on spread samples."""
     from fastemriwaveforms_amd import synthetic
@@ -155,7 +155,7 @@ def test_mixed16_config2_full_size(gpu):
     ref = oracle_waveform(inp, idx)
     err_o = oracle.rel_l2(hm[idx], ref)
     err_64 = oracle.rel_l2(h64[idx], ref)
-    print(f"config 2 mixed16: rel-L2 vs fp64 GPU {err:.3e}, vs oracle {err_o:.3e}; fp64 vs oracle {err_64:.3e}")
+    print(f"2 yr at 5 s, 2000 modes, mixed16: rel-L2 vs fp64 GPU {err:.3e}, vs oracle {err_o:.3e}; fp64 vs oracle {err_64:.3e}")
     assert err < MIXED_TOL and err_o < MIXED_TOL and err_64 < 1e-12
 
 
