@@ -497,7 +497,7 @@ __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sP
                              : g == 1 ? 2.0 * p1
                                       : (double)(P.m0 + g - 2) * p1 + (double)P.k0 * p2;
             double sn, cs;
-            sincos_turns_short(reduce_turns(arg), sn, cs);
+            sincos_turns_circle(reduce_turns(arg), sn, cs);
             V.ps[c] = sn;
             V.pc[c] = cs;
             ur = __shfl(p1, li);
@@ -505,11 +505,11 @@ __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sP
             zcs = __shfl(cs, li);
         } else {
             ur = dense_phase_fused(sPh + 16, s, 1.0 - s) * INV_2PI16;
-            sincos_turns_short(reduce_turns(ur), zsn, zcs);
+            sincos_turns_circle(reduce_turns(ur), zsn, zcs);
         }
         // lane group g needs v_n for n = n_lo + 16 kb + 4 g + t, t = 0..3 (scaled by 256)
         double sn, cs;
-        sincos_turns_short(reduce_turns((double)(P.n_lo + 4 * g) * ur), sn, cs);
+        sincos_turns_circle(reduce_turns((double)(P.n_lo + 4 * g) * ur), sn, cs);
         const double zr = zcs, zi = -zsn;  // z = exp(-i Phi_r)
         double vr = 256.0 * cs, vi = -256.0 * sn;
         V.vr[c][0] = (f16v_t)vr;
@@ -624,7 +624,7 @@ __device__ __forceinline__ void f16_epilogue(const FactPlanDev &P, const unsigne
                 const int r = 2 * rb + (g >> 1);
                 const int2 mk = r < P.Nr ? P.row_mk[r] : make_int2(0, 0);
                 double sn, cs;
-                sincos_turns_short(reduce_turns((double)mk.x * uph + (double)mk.y * uth), sn, cs);
+                sincos_turns_circle(reduce_turns((double)mk.x * uph + (double)mk.y * uth), sn, cs);
                 const float ur = (float)cs, ui = (float)(-sn);
                 // rows r >= Nr hold zeros (W = 0)
                 const v4f W = acc[rb][c];

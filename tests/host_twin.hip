@@ -370,6 +370,11 @@ void twin_dgtsv_pair(int n, const double *d, const double *du, const double *dl,
     }
 }
 
+void twin_sincos_circle(const double *f, double *s, double *c, long n)
+{
+    for (long i = 0; i < n; i++) sincos_turns_circle(reduce_turns(f[i]), s[i], c[i]);
+}
+
 void twin_sincos_turns(const double *f, double *s, double *c, long n)
 {
     for (long i = 0; i < n; i++) sincos_turns(reduce_turns(f[i]), s[i], c[i]);
