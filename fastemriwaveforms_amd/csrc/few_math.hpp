@@ -86,15 +86,16 @@ __host__ __device__ inline void sincos_turns_short(double f, double &s, double &
     sincos_poly_short(r * 6.283185307179586476925286766559, (int)q, s, c);
 }
 
-// One fused multiply-add (a * b + c, rounded once, as fma()).  On the device
-// it is pinned to the three-address v_fma_f64: the compiler otherwise keeps
-// polynomial coefficients in VGPRs and spends a copy on each two-address
-// v_fmac_f64.
-__host__ __device__ __forceinline__ double fma3(double a, double b, double c)
+// One fused multiply-add a * b + c (rounded once, as fma()) whose addend is a
+// wave-uniform constant.  On the device it is pinned to the three-address
+// v_fma_f64 with the addend read from an SGPR pair: the compiler otherwise
+// copies polynomial coefficients into VGPRs and spends a move on each
+// two-address v_fmac_f64.
+__host__ __device__ __forceinline__ double fma_cs(double a, double b, double c)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     double r;
-    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(c));
     return r;
 #else
     return fma(a, b, c);
@@ -114,24 +115,24 @@ __host__ __device__ inline void sincos_turns_circle(double f, double &s, double 
     const double x = f * 6.283185307179586476925286766559;
     const double u = x * x;
     double ps = 2.459350323376714e-15;
-    ps = fma3(ps, u, -7.581087442893419e-13);
-    ps = fma3(ps, u, 1.6052202305314846e-10);
-    ps = fma3(ps, u, -2.505168645272782e-08);
-    ps = fma3(ps, u, 2.7557303499119424e-06);
-    ps = fma3(ps, u, -0.00019841269497515153);
-    ps = fma3(ps, u, 0.008333333329272535);
-    ps = fma3(ps, u, -0.16666666666449323);
-    ps = fma3(ps, u, 0.9999999999996599);
+    ps = fma_cs(ps, u, -7.581087442893419e-13);
+    ps = fma_cs(ps, u, 1.6052202305314846e-10);
+    ps = fma_cs(ps, u, -2.505168645272782e-08);
+    ps = fma_cs(ps, u, 2.7557303499119424e-06);
+    ps = fma_cs(ps, u, -0.00019841269497515153);
+    ps = fma_cs(ps, u, 0.008333333329272535);
+    ps = fma_cs(ps, u, -0.16666666666449323);
+    ps = fma_cs(ps, u, 0.9999999999996599);
     s = x * ps;
     double pc = 4.131222830361208e-14;
-    pc = fma3(pc, u, -1.1353068040299952e-11);
-    pc = fma3(pc, u, 2.0865037866218456e-09);
-    pc = fma3(pc, u, -2.7556630082695456e-07);
-    pc = fma3(pc, u, 2.4801563123496047e-05);
-    pc = fma3(pc, u, -0.0013888888401028148);
-    pc = fma3(pc, u, 0.041666666615207106);
-    pc = fma3(pc, u, -0.4999999999769597);
-    c = fma3(pc, u, 0.9999999999974767);
+    pc = fma_cs(pc, u, -1.1353068040299952e-11);
+    pc = fma_cs(pc, u, 2.0865037866218456e-09);
+    pc = fma_cs(pc, u, -2.7556630082695456e-07);
+    pc = fma_cs(pc, u, 2.4801563123496047e-05);
+    pc = fma_cs(pc, u, -0.0013888888401028148);
+    pc = fma_cs(pc, u, 0.041666666615207106);
+    pc = fma_cs(pc, u, -0.4999999999769597);
+    c = fma_cs(pc, u, 0.9999999999974767);
 }
 
 __host__ __device__ inline void sincos_turns(double f, double &s, double &c)
