@@ -52,8 +52,8 @@ __host__ __device__ inline A64Layout a64_layout(int RBM, int NBC)
 {
     A64Layout L;
     size_t o = 0;
-    L.a = o;   o = a64_align(o + sizeof(double) * 2 * A64_STEP * (size_t)RBM * 16);   // [k][row]
-    L.b = o;   o = a64_align(o + sizeof(double) * 2 * A64_STEP * (size_t)NBC * 16);   // [k][col]
+    L.a = o;   o = a64_align(o + sizeof(double) * 2 * A64_STEP * (size_t)(RBM * 16 + 1));  // [k][row]
+    L.b = o;   o = a64_align(o + sizeof(double) * 2 * A64_STEP * (size_t)(NBC * 16 + 1));  // [k][col]
     L.pp = o;  o = a64_align(o + sizeof(double) * A64_SUPER);
     L.pt = o;  o = a64_align(o + sizeof(double) * A64_SUPER);
     L.pr = o;  o = a64_align(o + sizeof(double) * A64_SUPER);
@@ -89,6 +89,10 @@ __global__ __launch_bounds__(512) void adjoint64_kernel(Adj16Args a)
     const int Nr = P.Nr;
     const int NRP = RBM * 16;       // padded real rows
     const int NCP = NBC * 16;       // padded columns
+    // LDS strides of the k-rows: one double past the rows (columns), so the
+    // 16 samples of a job group, which write one k-row each, fall on
+    // different banks (at a multiple of 128 bytes all 16 hit one bank)
+    const int SA = NRP + 1, SB = NCP + 1;
     const A64Layout Lay = a64_layout(RBM, NBC);
     double *sA = reinterpret_cast<double *>(smem + Lay.a);    // [k][row], k = 2 i + (re, im)
     double *sB = reinterpret_cast<double *>(smem + Lay.b);    // [k][col]
@@ -108,7 +112,7 @@ __global__ __launch_bounds__(512) void adjoint64_kernel(Adj16Args a)
     if (pad > 0) {
         for (int e = tid; e < 2 * A64_STEP * pad; e += NT) {
             const int k = e / pad, rr = e - k * pad;
-            sA[k * NRP + rows_used + rr] = 0.0;
+            sA[k * SA + rows_used + rr] = 0.0;
         }
     }
     const double inv_pw = 1.0 / sg.pw;
@@ -162,7 +166,7 @@ __global__ __launch_bounds__(512) void adjoint64_kernel(Adj16Args a)
                         const double2 cgv = sCg[i0 + ii];
                         double wx = cgv.x * ux - cgv.y * uy, wy = cgv.x * uy + cgv.y * ux;
                         const double sg1 = sSig[i0 + ii];
-                        double *k0 = sA + (2 * ii) * NRP, *k1 = k0 + NRP;
+                        double *k0 = sA + (2 * ii) * SA, *k1 = k0 + SA;
 #pragma unroll
                         for (int sm = 0; sm < A16_MOM; sm++) {
                             const int R2 = 2 * (A16_MOM * r + sm);
@@ -184,7 +188,7 @@ __global__ __launch_bounds__(512) void adjoint64_kernel(Adj16Args a)
                                      sn, cs);
                         double vr = cs, vi = -sn;
                         const double2 z = sZ[i0 + ii];
-                        double *k0 = sB + (2 * ii) * NCP, *k1 = k0 + NCP;
+                        double *k0 = sB + (2 * ii) * SB, *k1 = k0 + SB;
 #pragma unroll
                         for (int t2 = 0; t2 < A64_VG; t2++) {
                             const int col = A64_VG * grp + t2;
@@ -204,14 +208,14 @@ __global__ __launch_bounds__(512) void adjoint64_kernel(Adj16Args a)
             __syncthreads();
             double bf[2 * A64_STEP / 4];
 #pragma unroll
-            for (int kk = 0; kk < 2 * A64_STEP / 4; kk++) bf[kk] = sB[(4 * kk + g) * NCP + 16 * cb + li];
+            for (int kk = 0; kk < 2 * A64_STEP / 4; kk++) bf[kk] = sB[(4 * kk + g) * SB + 16 * cb + li];
 #pragma unroll
             for (int k = 0; k < RBH; k++) {
                 const int rb = part + NH * k;
                 if (RBM % NH == 0 || rb < RBM) {
 #pragma unroll
                     for (int kk = 0; kk < 2 * A64_STEP / 4; kk++) {
-                        const double av = sA[(4 * kk + g) * NRP + 16 * rb + li];
+                        const double av = sA[(4 * kk + g) * SA + 16 * rb + li];
                         acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bf[kk], acc[k], 0, 0, 0);
                     }
                 }
