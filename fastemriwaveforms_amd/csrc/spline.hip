@@ -462,8 +462,7 @@ __host__ __device__ inline void spline_system_shared(const SplineGroup &G, int s
     }
     int ii = n - 3;
     for (; ii - (PF - 1) >= 0; ii -= PF) {
-        double rb[PF], ru[PF], rl[PF], rd[PF], rx[PF],
-            ry[PF];
+        double rb[PF], ru[PF], rl[PF], rd[PF], rx[PF], ry[PF];
 #pragma unroll
         for (int u = 0; u < PF; u++) {
             rb[u] = SB(ii - u);
