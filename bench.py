@@ -249,7 +249,8 @@ def batch_leg(dev, steps: int, warmup: int, fp64: bool, count: int = 256, sum_re
            "wall_ms_per_step": wall_ms, "waveforms_per_s": count / (ms_graph * 1e-3),
            "mode_samples_per_s": bp.mode_samples / (ms_graph * 1e-3), "stage_ms": stage_ms,
            "rel_l2_vs_oracle_max_of_8": None, "plan_rows": int(bp.plan_rows),
-           "plan_cols": int(bp.plan_cols), "plan_row_blocks": int(bp.plan_row_blocks)}
+           "plan_cols": int(bp.plan_cols), "plan_cols16": int(bp.plan_cols16),
+           "plan_row_blocks": int(bp.plan_row_blocks)}
     log(f"[bench] batch mixed16 graph: {ms_graph:.2f} ms/step (wall {wall_ms:.2f}), "
         f"{out['waveforms_per_s']:.1f} waveforms/s, {out['mode_samples_per_s']:.4g} "
         f"mode-samples/s, stages {stage_ms}")
@@ -746,16 +747,20 @@ def ModeSumFunction_apply(eng, t_ri, p_k):
     return ModeSumFunction.apply(eng, t_ri, p_k)
 
 
-def fact16_roofline(kernel, rows, cols, row_blocks, samples, sum_ms, traffic=None,
+def fact16_roofline(kernel, rows, cols, cols16, row_blocks, samples, sum_ms, traffic=None,
                     traffic_source=None):
     """Roofline entry of a split-FP16 factorised sum launch: algorithmic FLOP = 8 FLOP x
     4 powers x rows x cols per output sample (the complex n-contraction of the (m, n)
-    cell grid) x samples; against the published dense FP16 MFMA peak.  The issue rate
-    of the split products (3 per FP32-accurate product, padded to the MFMA tiles) is
-    reported beside it."""
+    cell grid) x samples; against the published dense FP16 MFMA peak.  The tiles fold
+    column -n onto n (v_{-n} = conj(v_n)) when the n range is symmetric, so the kernel
+    executes the contraction over cols16 columns (8 FLOP each: four real products and
+    sums); that count and the issue rate of the split products (3 per FP32-accurate
+    product, padded to the MFMA tiles) are reported beside it."""
     flop_per_sample = 8 * 4 * rows * cols
     achieved = flop_per_sample * samples / (sum_ms * 1e-3) / 1e12
-    kb = (2 * cols + 31) // 32
+    executed_per_sample = 8 * 4 * rows * cols16
+    executed = executed_per_sample * samples / (sum_ms * 1e-3) / 1e12
+    kb = (2 * cols16 + 31) // 32
     issued_per_sample = 3 * 2 * (16 * row_blocks) * (32 * kb)
     issued = issued_per_sample * samples / (sum_ms * 1e-3) / 1e12
     return {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp16_mfma"], "unit": "TFLOP/s",
@@ -763,6 +768,10 @@ def fact16_roofline(kernel, rows, cols, row_blocks, samples, sum_ms, traffic=Non
             "algorithmic": f"8 FLOP x 4 powers x {rows} rows x {cols} cols = {flop_per_sample} "
                            f"FLOP/sample x {samples} samples per launch",
             "kernel_ms": sum_ms, "traffic_source": traffic_source,
+            "executed_contraction": {"tflops": executed, "frac_of_peak": executed / PEAK["fp16_mfma"],
+                                     "flop_per_sample": executed_per_sample,
+                                     "basis": f"{cols16} tile columns ({cols} n columns, "
+                                              f"{'-n folded onto n' if cols16 < cols else 'unfolded'})"},
             "split_product_issue": {"tflops": issued, "frac_of_peak": issued / PEAK["fp16_mfma"],
                                     "flop_per_sample": issued_per_sample,
                                     "basis": "3 FP16 products (lo*hi, hi*lo, hi*hi) per "
@@ -800,7 +809,7 @@ def single_leg(dev, args):
                                "bound_by": "the elimination and back-substitution chains "
                                            "(latency), not bandwidth"},
            "roofline": fact16_roofline("fact16_kernel", prob.plan.rows, prob.plan.cols,
-                                       prob.plan.row_blocks, N, ms["sum"],
+                                       prob.plan.cols16, prob.plan.row_blocks, N, ms["sum"],
                                        FACT16_FULL1_TRAFFIC,
                                        "profiles/fact16_full1_hbm_traffic.md (rocprofv3 PMC)")}
     chk = (idx, h[idx].copy())
@@ -897,11 +906,12 @@ def main():
     batch, batch_chk, batch_inputs = batch_leg(dev, args.steps, args.warmup, not args.no_fp64,
                                                args.batch_count)
     roofline = fact16_roofline("batch_fact16_kernel", batch["plan_rows"], batch["plan_cols"],
-                               batch["plan_row_blocks"], batch["samples"],
+                               batch["plan_cols16"], batch["plan_row_blocks"], batch["samples"],
                                batch["stage_ms"]["sum"], BATCH_FACT16_TRAFFIC,
                                BATCH_TRAFFIC_SOURCE)
     log(f"[bench] batch sum roofline: {roofline['achieved']:.1f} TFLOP/s = "
-        f"{roofline['frac']:.3f} of the dense FP16 peak (split-product issue "
+        f"{roofline['frac']:.3f} of the dense FP16 peak (executed folded contraction "
+        f"{roofline['executed_contraction']['frac_of_peak']:.3f}, split-product issue "
         f"{roofline['split_product_issue']['frac_of_peak']:.3f})")
 
     single = single_chk = inp1 = None

@@ -66,6 +66,14 @@ bool build_fact_plan(const int *m, const int *k, const int *n, int K, FactPlanHo
     p.RB = (8 * p.Nr + 15) / 16;
     p.KS = (2 * p.Nc + 3) / 4;
     p.CST = ((2 * p.KS + 15) / 16) * 16 + 2;
+    p.f16_nc = p.Nc;
+    p.f16_n0 = nlo;
+    const int nmax = std::max(-nlo, nhi);
+    if (nlo < 0 && (2 * (nmax + 1) + 31) / 32 < (2 * p.Nc + 31) / 32) {
+        p.f16_fold = 1;
+        p.f16_nc = nmax + 1;
+        p.f16_n0 = 0;
+    }
     const int ncell = p.Nr * p.Nc;
     std::vector<int> count(ncell + 1, 0);
     auto cell_of = [&](int mm, int kk, int nn) { return rows[{mm, kk}] * p.Nc + (nn - nlo); };

@@ -34,6 +34,10 @@ struct FactPlanHost {
     int rows_arith = 0;     // rows are m0, m0+1, ... with one k: phasors by recurrence
     int m0 = 0, k0 = 0;
     int has_k = 0;          // some mode has k != 0 (Phi_theta enters the phase)
+    // Split-FP16 tile columns.  v_{-n} = conj(v_n), so with f16_fold the
+    // coefficients of columns n and -n share tile column n (n = 0 ..
+    // f16_nc - 1) as real pairs; otherwise tile column c is n = f16_n0 + c.
+    int f16_fold = 0, f16_nc = 0, f16_n0 = 0;
     std::vector<int> row_m, row_k;        // (Nr,)
     std::vector<int> cell_start;          // (Nr*Nc + 1,)
     std::vector<int> entries;             // q*2 + conj, grouped by cell, q ascending
@@ -45,6 +49,7 @@ struct FactPlanHost {
 struct FactPlanDev {
     int K, Nr, Nc, n_lo, RB, KS, CST;
     int rows_arith, m0, k0, has_k;
+    int f16_fold, f16_nc, f16_n0;
     const int2 *row_mk;      // (Nr,)
     const int *cell_start;   // (Nr*Nc + 1,)
     const int *entries;
@@ -112,7 +117,7 @@ int launch_fact64_sum(const FactArgs &a, const FactPlanHost &p, int max_blocks, 
 // floats of inverse scales (one per power p), padded to 256 bytes.  The four 16-byte
 // chunks of a row are XOR-swizzled by (row >> 1) & 3, which puts each group of
 // eight lanes of a fragment read on distinct LDS banks without padding.
-inline int fact16_kb(const FactPlanHost &p) { return (2 * p.Nc + 31) / 32; }
+inline int fact16_kb(const FactPlanHost &p) { return (2 * p.f16_nc + 31) / 32; }
 inline size_t fact16_tile_bytes(const FactPlanHost &p)
 {
     const size_t b = 2 * (size_t)fact16_kb(p) * p.RB * 1024 + 4 * sizeof(float);

@@ -15,7 +15,11 @@
 // Tile layout (per knot interval, identical in global memory and LDS):
 //   plane hi, plane lo: RB*16 real rows x 32*KB real columns, real row
 //       2R+e, real column k = 2a+f, holding [[Re, -Im], [Im, Re]] of
-//       C[R, a]; stored as KB x RB blocks of 16 rows x 32 halfs (kb-major,
+//       C[R, n_a] (against B rows Re v_n, Im v_n of tile column a).  When the
+//       plan folds (n range about symmetric), tile column a is n = a >= 0 and
+//       also carries D = C[R, -a]: as v_{-n} = conj(v_n), the entries are
+//       [[Cr + Dr, Di - Ci], [Ci + Di, Cr - Dr]], which halves the columns;
+//       stored as KB x RB blocks of 16 rows x 32 halfs (kb-major,
 //       one KiB each) so that every fragment address inside the K loop is a
 //       lane base plus a compile-time offset; the four 16-byte chunks of a
 //       row are XOR-swizzled by (row >> 1) & 3.  A ds_read_b128 is served
@@ -65,19 +69,12 @@ __device__ __forceinline__ int f16_lane_off(int li, int g) { return li * 32 + ((
 // are gathered again rather than held).
 #define F16_PREP_ROWS 4  // complex rows r per prep workgroup, one wave each
 
-// C_p[r, col] * h^p of one cell (zero for padding rows and columns)
-__device__ __forceinline__ void f16_cell(const FactPlanDev &P, const double *interp,
-                                         const double2 *ylm, int L, int j, double hw, int r,
-                                         int col, double (&cre)[4], double (&cim)[4])
+// Adds C_p[r, col] (the entries of one cell, in list order) for the four powers.
+__device__ __forceinline__ void f16_gather(const FactPlanDev &P, const double *interp,
+                                           const double2 *ylm, int L, int j, int cell,
+                                           double (&cre)[4], double (&cim)[4])
 {
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        cre[p] = 0.0;
-        cim[p] = 0.0;
-    }
-    if (r >= P.Nr || col >= P.Nc) return;
     const long K2 = 2L * P.K;
-    const int cell = r * P.Nc + col;
     for (int e = P.cell_start[cell]; e < P.cell_start[cell + 1]; e++) {
         const int ent = P.entries[e];
         const int qm = ent >> 1;
@@ -92,13 +89,42 @@ __device__ __forceinline__ void f16_cell(const FactPlanDev &P, const double *int
             cim[p] += cr * Y.y + ci * Y.x;
         }
     }
+}
+
+// The four real tile entries of complex row r, tile column kc and each power p,
+// scaled by h^p: x[p] = {row 2R col 2kc, row 2R col 2kc + 1, row 2R + 1 col 2kc,
+// row 2R + 1 col 2kc + 1} of R = 4 r + p, against B rows (Re v_n, Im v_n).
+// With C = C_p[r, n] and D = C_p[r, -n] (D = 0 unless the tiles fold and n > 0),
+//   C v_n + D conj(v_n) = (Cr + Dr) Re v - (Ci - Di) Im v
+//                       + i [(Ci + Di) Re v + (Cr - Dr) Im v].
+// Padding rows and columns are zero.
+__device__ __forceinline__ void f16_cell(const FactPlanDev &P, const double *interp,
+                                         const double2 *ylm, int L, int j, double hw, int r,
+                                         int kc, double (&x)[4][4])
+{
+    double cre[4] = {0.0, 0.0, 0.0, 0.0}, cim[4] = {0.0, 0.0, 0.0, 0.0};
+    double dre[4] = {0.0, 0.0, 0.0, 0.0}, dim[4] = {0.0, 0.0, 0.0, 0.0};
+    if (r < P.Nr && kc < P.f16_nc) {
+        const int n = P.f16_n0 + kc;
+        const int col = n - P.n_lo;
+        if (col < P.Nc) f16_gather(P, interp, ylm, L, j, r * P.Nc + col, cre, cim);
+        const int colm = -n - P.n_lo;
+        if (P.f16_fold && n > 0 && colm >= 0) f16_gather(P, interp, ylm, L, j, r * P.Nc + colm, dre, dim);
+    }
     double hp = 1.0;
 #pragma unroll
     for (int p = 0; p < 4; p++) {
-        cre[p] *= hp;
-        cim[p] *= hp;
+        x[p][0] = (cre[p] + dre[p]) * hp;
+        x[p][1] = (dim[p] - cim[p]) * hp;
+        x[p][2] = (cim[p] + dim[p]) * hp;
+        x[p][3] = (cre[p] - dre[p]) * hp;
         hp *= hw;
     }
+}
+
+__device__ __forceinline__ double f16_cell_max(const double (&x)[4])
+{
+    return fmax(fmax(fabs(x[0]), fabs(x[1])), fmax(fabs(x[2]), fabs(x[3])));
 }
 
 // Pass 1: workgroup (j, row group) -- one wave per complex row r, lanes over
@@ -115,10 +141,10 @@ __device__ void fact16_rowmax_block(const FactPlanDev &P, const double *interp, 
     const double hw = h_t[j + 1] - h_t[j];
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
     for (int col = lane; col < ncolc; col += 64) {
-        double cre[4], cim[4];
-        f16_cell(P, interp, ylm, L, j, hw, r, col, cre, cim);
+        double x[4][4];
+        f16_cell(P, interp, ylm, L, j, hw, r, col, x);
 #pragma unroll
-        for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], fmax(fabs(cre[p]), fabs(cim[p])));
+        for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], f16_cell_max(x[p]));
     }
 #pragma unroll
     for (int p = 0; p < 4; p++) {
@@ -128,26 +154,24 @@ __device__ void fact16_rowmax_block(const FactPlanDev &P, const double *interp, 
     if (lane < 4) maxes[r * 4 + lane] = lane == 0 ? mx[0] : lane == 1 ? mx[1] : lane == 2 ? mx[2] : mx[3];
 }
 
-// The split of one cell's powers into the two planes (rows 2R: (Re, -Im),
-// 2R + 1: (Im, Re) of R = 4 r + p).  The two halfs of a row's column pair are
-// adjacent (even offset): one 32-bit store each.
-__device__ __forceinline__ void f16_store_cell(_Float16 *hi, _Float16 *lo, int RB, int r, int col,
-                                               const double (&cre)[4], const double (&cim)[4],
-                                               const double (&sigma)[4])
+// The split of one cell's powers into the two planes (rows 2R and 2R + 1 of
+// R = 4 r + p, columns 2kc and 2kc + 1).  The two halfs of a row's column pair
+// are adjacent (even offset): one 32-bit store each.
+__device__ __forceinline__ void f16_store_cell(_Float16 *hi, _Float16 *lo, int RB, int r, int kc,
+                                               const double (&x)[4][4], const double (&sigma)[4])
 {
 #pragma unroll
     for (int p = 0; p < 4; p++) {
         const int R = 4 * r + p;
-        const float xr = f32_opaque((float)(cre[p] * sigma[p]));
-        const float xi = f32_opaque((float)(cim[p] * sigma[p]));
-        const _Float16 rh = (_Float16)xr, ih = (_Float16)xi;
-        const _Float16 rl = (_Float16)(xr - (float)rh), il = (_Float16)(xi - (float)ih);
-        const long o0 = f16_tile_off(2 * R, 2 * col, RB);
-        const long o1 = f16_tile_off(2 * R + 1, 2 * col, RB);
-        *reinterpret_cast<v2h *>(hi + o0) = v2h{rh, (_Float16)-ih};
-        *reinterpret_cast<v2h *>(lo + o0) = v2h{rl, (_Float16)-il};
-        *reinterpret_cast<v2h *>(hi + o1) = v2h{ih, rh};
-        *reinterpret_cast<v2h *>(lo + o1) = v2h{il, rl};
+        const v2f a = {f32_opaque((float)(x[p][0] * sigma[p])), f32_opaque((float)(x[p][1] * sigma[p]))};
+        const v2f b = {f32_opaque((float)(x[p][2] * sigma[p])), f32_opaque((float)(x[p][3] * sigma[p]))};
+        const v2h ah = __builtin_convertvector(a, v2h), bh = __builtin_convertvector(b, v2h);
+        const long o0 = f16_tile_off(2 * R, 2 * kc, RB);
+        const long o1 = f16_tile_off(2 * R + 1, 2 * kc, RB);
+        *reinterpret_cast<v2h *>(hi + o0) = ah;
+        *reinterpret_cast<v2h *>(lo + o0) = f16_split_lo(ah, a);
+        *reinterpret_cast<v2h *>(hi + o1) = bh;
+        *reinterpret_cast<v2h *>(lo + o1) = f16_split_lo(bh, b);
     }
 }
 
@@ -183,9 +207,9 @@ __device__ void fact16_split_block(const FactPlanDev &P, const double *interp, c
 #pragma unroll
     for (int p = 0; p < 4; p++) sigma[p] = s_sigma[p];
     for (int col = lane; col < ncolc; col += 64) {
-        double cre[4], cim[4];
-        f16_cell(P, interp, ylm, L, j, hw, r, col, cre, cim);
-        f16_store_cell(hi, lo, RB, r, col, cre, cim, sigma);
+        double x[4][4];
+        f16_cell(P, interp, ylm, L, j, hw, r, col, x);
+        f16_store_cell(hi, lo, RB, r, col, x, sigma);
     }
 }
 
@@ -228,20 +252,20 @@ __device__ void fact16_prep_interval_cached(const FactPlanDev &P, const double *
     const int ncolc = 16 * KB;
     const int ncell = 2 * RB * ncolc;
     const double hw = h_t[j + 1] - h_t[j];
-    double cre[CPT][4], cim[CPT][4];
+    double x[CPT][4][4];
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < CPT; k++) {
         const int ci = tid + k * F16_PREP_THREADS;
         const int r = ci / ncolc, col = ci - r * ncolc;
         if (ci < ncell) {
-            f16_cell(P, interp, ylm, L, j, hw, r, col, cre[k], cim[k]);
+            f16_cell(P, interp, ylm, L, j, hw, r, col, x[k]);
         } else {
 #pragma unroll
-            for (int p = 0; p < 4; p++) cre[k][p] = cim[k][p] = 0.0;
+            for (int p = 0; p < 4; p++) x[k][p][0] = x[k][p][1] = x[k][p][2] = x[k][p][3] = 0.0;
         }
 #pragma unroll
-        for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], fmax(fabs(cre[k][p]), fabs(cim[k][p])));
+        for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], f16_cell_max(x[k][p]));
     }
 #pragma unroll
     for (int p = 0; p < 4; p++) {
@@ -258,7 +282,7 @@ __device__ void fact16_prep_interval_cached(const FactPlanDev &P, const double *
 #pragma unroll
     for (int k = 0; k < CPT; k++) {
         const int ci = tid + k * F16_PREP_THREADS;
-        if (ci < ncell) f16_store_cell(hi, lo, RB, ci / ncolc, ci % ncolc, cre[k], cim[k], sigma);
+        if (ci < ncell) f16_store_cell(hi, lo, RB, ci / ncolc, ci % ncolc, x[k], sigma);
     }
 }
 
@@ -279,10 +303,10 @@ __device__ void fact16_prep_interval(const FactPlanDev &P, const double *interp,
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
     for (int ci = tid; ci < ncell; ci += F16_PREP_THREADS) {
         const int r = ci / ncolc, col = ci - r * ncolc;
-        double cre[4], cim[4];
-        f16_cell(P, interp, ylm, L, j, hw, r, col, cre, cim);
+        double x[4][4];
+        f16_cell(P, interp, ylm, L, j, hw, r, col, x);
 #pragma unroll
-        for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], fmax(fabs(cre[p]), fabs(cim[p])));
+        for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], f16_cell_max(x[p]));
     }
 #pragma unroll
     for (int p = 0; p < 4; p++) {
@@ -298,9 +322,9 @@ __device__ void fact16_prep_interval(const FactPlanDev &P, const double *interp,
     f16_interval_sigma(s_red, F16_PREP_THREADS / 64, reinterpret_cast<float *>(lo + plane), sigma);
     for (int ci = tid; ci < ncell; ci += F16_PREP_THREADS) {
         const int r = ci / ncolc, col = ci - r * ncolc;
-        double cre[4], cim[4];
-        f16_cell(P, interp, ylm, L, j, hw, r, col, cre, cim);
-        f16_store_cell(hi, lo, RB, r, col, cre, cim, sigma);
+        double x[4][4];
+        f16_cell(P, interp, ylm, L, j, hw, r, col, x);
+        f16_store_cell(hi, lo, RB, r, col, x, sigma);
     }
 }
 
@@ -473,7 +497,7 @@ __device__ __forceinline__ void f16_prefetch(const Segment &sg, int base_i, cons
 }
 
 // Phi_r (FP64) and the v_n recurrence state: lane group g needs v_n for
-// n = n_lo + 16 kb + 4 g + t, t = 0..3, scaled by 256.
+// n = n0 + 16 kb + 4 g + t, t = 0..3, scaled by 256 (n0: the n of tile column 0).
 template <bool ARITH>
 __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sPh, const Segment &sg,
                                           double inv_pw, int base_i, double dt, F16V &V)
@@ -507,9 +531,9 @@ __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sP
             ur = dense_phase_fused(sPh + 16, s, 1.0 - s) * INV_2PI16;
             sincos_turns_circle(reduce_turns(ur), zsn, zcs);
         }
-        // lane group g needs v_n for n = n_lo + 16 kb + 4 g + t, t = 0..3 (scaled by 256)
+        // lane group g needs v_n for n = n0 + 16 kb + 4 g + t, t = 0..3 (scaled by 256)
         double sn, cs;
-        sincos_turns_circle(reduce_turns((double)(P.n_lo + 4 * g) * ur), sn, cs);
+        sincos_turns_circle(reduce_turns((double)(P.f16_n0 + 4 * g) * ur), sn, cs);
         const double zr = zcs, zi = -zsn;  // z = exp(-i Phi_r)
         double vr = 256.0 * cs, vi = -256.0 * sn;
         V.vr[c][0] = (f16v_t)vr;

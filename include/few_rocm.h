@@ -123,9 +123,11 @@ int few_prepare_modes_async(const double *teuk, const double *phase_knots, const
  * on the host from HOST copies of the int32 mode indices and uploaded to the
  * current device once; reused by every call with the same mode set.
  * Returns NULL on error.  info7 = {K, rows, cols, n_lo, row blocks,
- * k-steps, supported}; supported is a bit mask (1: FEW_PRECISION_MIXED,
- * 2: FEW_PRECISION_MIXED16, 4: FEW_PRECISION_FACT64); unsupported sets (too
- * many rows/columns for the MFMA tiles) can still use FEW_PRECISION_FP64.
+ * k-steps, flags}; flags is a bit mask (1: FEW_PRECISION_MIXED supported,
+ * 2: FEW_PRECISION_MIXED16 supported, 4: FEW_PRECISION_FACT64 supported,
+ * 8: the MIXED16 tiles fold column -n onto n, max|n| + 1 tile columns);
+ * unsupported sets (too many rows/columns for the MFMA tiles) can still use
+ * FEW_PRECISION_FP64.
  */
 void *few_mode_plan_create(const int *h_m, const int *h_k, const int *h_n, int num_modes);
 int few_mode_plan_info(const void *plan, int *info7);

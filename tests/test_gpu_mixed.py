@@ -77,11 +77,45 @@ def test_mixed_explicit_pm_modes(gpu, prec):
     h, p = _run(inp, prec)
     assert p.plan.supported16
     assert p.plan.cols == 111
+    assert p.plan.folded16 and p.plan.cols16 == 56  # columns n and -n share a tile column
     idx = spread_indices(inp.num_pts, count=1500)
     ref = oracle_waveform(inp, idx)
     err = oracle.rel_l2(h[idx], ref)
     print(f"mixed2-shape {prec} rel-L2 {err:.3e}")
     assert err < MIXED_TOL
+
+
+@pytest.mark.parametrize("case", ["asymmetric_folded", "narrow_unfolded"])
+def test_mixed16_column_folding(gpu, case):
+    """The split-FP16 tiles fold column -n onto n when that needs fewer 32-column
+    steps.  asymmetric_folded: m = 0 modes with n in [-30, 10] beside m != 0 modes
+    with |n| <= 5 (n range [-30, 10]; tile columns 11..30 hold only their -n
+    partner); narrow_unfolded: |n| <= 7 (15 columns, one step either way, no fold).
+    Against the oracle and the FP64 GPU path."""
+    from fastemriwaveforms_amd import synthetic
+    inp = synthetic.small_input(seed=37, L=16, K=60, T_years=0.004)
+    rng = np.random.default_rng(5)
+    K = len(inp.m)
+    if case == "asymmetric_folded":
+        zero = np.arange(K) < K // 2
+        inp.m = np.where(zero, 0, inp.m).astype(inp.m.dtype)
+        inp.n = np.where(zero, rng.integers(-30, 11, K), rng.integers(-5, 6, K)).astype(inp.n.dtype)
+        inp.n[0] = -30
+    else:
+        inp.n = rng.integers(-7, 8, K).astype(inp.n.dtype)
+    h, p = _run(inp, "mixed16")
+    assert p.plan.supported16
+    if case == "asymmetric_folded":
+        assert p.plan.n_lo == -30 and p.plan.cols == 41 and p.plan.folded16 and p.plan.cols16 == 31
+    else:
+        assert not p.plan.folded16 and p.plan.cols16 == p.plan.cols
+    ref = oracle_waveform(inp)
+    err = oracle.rel_l2(h, ref)
+    h64, _ = _run(inp, "fp64")
+    err64 = oracle.rel_l2(h, h64)
+    print(f"{case}: cols {p.plan.cols} -> {p.plan.cols16}, rel-L2 vs oracle {err:.3e}, "
+          f"vs fp64 GPU {err64:.3e}")
+    assert err < MIXED_TOL and err64 < MIXED_TOL
 
 
 @pytest.mark.parametrize("prec", PRECS)

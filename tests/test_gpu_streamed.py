@@ -37,8 +37,8 @@ def _inputs():
     long = synthetic.small_input(seed=30, L=14, K=30, T_years=0.002)
     long.num_pts += 3000
     return {
-        "cpu0": synthetic.make_workload("cpu0"),                                   # 4 column steps
-        "n55": synthetic.make_input(0.003, 10.0, 20, "m_all_55", 400, 41, name="n55"),  # 7
+        "cpu0": synthetic.make_workload("cpu0"),                                   # 2 column steps
+        "n55": synthetic.make_input(0.003, 10.0, 20, "m_all_55", 400, 41, name="n55"),  # 4
         "k1_20": synthetic.make_input(0.003, 10.0, 20, "k1_20", 300, 42, name="k1_20"),
         "one_mode": synthetic.make_input(0.002, 10.0, 10, "m_nonneg_30", 1, 43, name="one"),
         "output_cut": cut,
