@@ -212,9 +212,9 @@ void *few_batch_create(int B, const int *h_L, const int *h_K, const int *h_N, co
     U.m0 = u.m0;
     U.k0 = u.k0;
     U.has_k = u.has_k;
-    U.f16_fold = u.f16_fold;
-    U.f16_nc = u.f16_nc;
-    U.f16_n0 = u.f16_n0;
+    U.fold = u.fold;
+    U.tcols = u.tcols;
+    U.tn0 = u.tn0;
     U.row_mk = reinterpret_cast<const int2 *>(base + o_row);
     U.cell_start = nullptr;
     U.entries = nullptr;

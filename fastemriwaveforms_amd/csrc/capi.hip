@@ -673,9 +673,9 @@ void *few_mode_plan_create(const int *h_m, const int *h_k, const int *h_n, int n
     pl->dev.m0 = h.m0;
     pl->dev.k0 = h.k0;
     pl->dev.has_k = h.has_k;
-    pl->dev.f16_fold = h.f16_fold;
-    pl->dev.f16_nc = h.f16_nc;
-    pl->dev.f16_n0 = h.f16_n0;
+    pl->dev.fold = h.fold;
+    pl->dev.tcols = h.tcols;
+    pl->dev.tn0 = h.tn0;
     if (!h.ok) return pl;  // usable for the direct paths; mixed reports unsupported
     std::vector<int> buf;
     for (int r = 0; r < h.Nr; r++) {
@@ -720,7 +720,7 @@ int few_mode_plan_info(const void *plan, int *info7)
     info7[4] = h.RB;
     info7[5] = h.KS;
     info7[6] = (h.ok ? 1 : 0) | (fact16_supported(h) ? 2 : 0) | (fact64_supported(h) ? 4 : 0) |
-               (h.f16_fold ? 8 : 0);
+               (h.fold ? 8 : 0);
     return FEW_OK;
 }
 

@@ -64,16 +64,16 @@ bool build_fact_plan(const int *m, const int *k, const int *n, int K, FactPlanHo
     p.n_lo = nlo;
     p.Nc = nhi - nlo + 1;
     p.RB = (8 * p.Nr + 15) / 16;
-    p.KS = (2 * p.Nc + 3) / 4;
-    p.CST = ((2 * p.KS + 15) / 16) * 16 + 2;
-    p.f16_nc = p.Nc;
-    p.f16_n0 = nlo;
+    p.tcols = p.Nc;
+    p.tn0 = nlo;
     const int nmax = std::max(-nlo, nhi);
-    if (nlo < 0 && (2 * (nmax + 1) + 31) / 32 < (2 * p.Nc + 31) / 32) {
-        p.f16_fold = 1;
-        p.f16_nc = nmax + 1;
-        p.f16_n0 = 0;
+    if (nlo < 0 && nmax + 1 < p.Nc) {
+        p.fold = 1;
+        p.tcols = nmax + 1;
+        p.tn0 = 0;
     }
+    p.KS = (2 * p.tcols + 3) / 4;
+    p.CST = (((p.fold ? 4 : 2) * p.KS + 15) / 16) * 16 + 2;
     const int ncell = p.Nr * p.Nc;
     std::vector<int> count(ncell + 1, 0);
     auto cell_of = [&](int mm, int kk, int nn) { return rows[{mm, kk}] * p.Nc + (nn - nlo); };
@@ -117,40 +117,25 @@ __global__ __launch_bounds__(256) void fact_prep_kernel(FactPrepArgs a)
     const long rem = gid - (long)j * per_j;
     const int r = (int)(rem / P.CST);
     const int col = (int)(rem - (long)r * P.CST);
-    double cre[4] = {0.0, 0.0, 0.0, 0.0}, cim[4] = {0.0, 0.0, 0.0, 0.0};
-    if (r < P.Nr && col < P.Nc) {
-        const int cell = r * P.Nc + col;
-        const long K2 = 2L * P.K;
-        for (int e = P.cell_start[cell]; e < P.cell_start[cell + 1]; e++) {
-            const int ent = P.entries[e];
-            const int qm = ent >> 1;
-            const bool cj = ent & 1;
-            const double2 Y = a.ylm[cj ? P.K + qm : qm];
-#pragma unroll
-            for (int p = 0; p < 4; p++) {
-                const long base = ((long)p * a.L + j) * K2;
-                const double cr = a.interp[base + qm];
-                const double ci = cj ? -a.interp[base + P.K + qm] : a.interp[base + P.K + qm];
-                cre[p] += cr * Y.x - ci * Y.y;
-                cim[p] += cr * Y.y + ci * Y.x;
-            }
-        }
-        const double hw = a.h_t[j + 1] - a.h_t[j];
-        double hp = hw;
-#pragma unroll
-        for (int p = 1; p < 4; p++) {
-            cre[p] *= hp;
-            cim[p] *= hp;
-            hp *= hw;
-        }
-    }
+    // unfolded: pair col = (x0, x2) = (Re C, Im C); folded: pairs col and
+    // TW + col = (x0, x2) and (x1, x3) of tile column col < TW, zero beyond 2 TW
+    const int TW = 2 * P.KS;
+    if (P.fold && col >= TW && col < 2 * TW) return;  // written by thread col - TW
+    double x[4][4];
+    fact_cell(P, a.interp, a.ylm, a.L, j, a.h_t[j + 1] - a.h_t[j], r, P.fold && col >= TW ? P.tcols : col,
+              x);
+    const bool second = P.fold && col < TW;
     if (a.C64) {
         double *dst = a.C64 + (long)j * a.c_stride;
 #pragma unroll
         for (int p = 0; p < 4; p++) {
             const long o = ((long)(r * 4 + p) * P.CST + col) * 2;
-            dst[o] = cre[p];
-            dst[o + 1] = cim[p];
+            dst[o] = x[p][0];
+            dst[o + 1] = x[p][2];
+            if (second) {
+                dst[o + 2 * TW] = x[p][1];
+                dst[o + 2 * TW + 1] = x[p][3];
+            }
         }
         return;
     }
@@ -158,8 +143,12 @@ __global__ __launch_bounds__(256) void fact_prep_kernel(FactPrepArgs a)
 #pragma unroll
     for (int p = 0; p < 4; p++) {
         const long o = ((long)(r * 4 + p) * P.CST + col) * 2;
-        dst[o] = (float)cre[p];
-        dst[o + 1] = (float)cim[p];
+        dst[o] = (float)x[p][0];
+        dst[o + 1] = (float)x[p][2];
+        if (second) {
+            dst[o + 2 * TW] = (float)x[p][1];
+            dst[o + 2 * TW + 1] = (float)x[p][3];
+        }
     }
 }
 
@@ -200,7 +189,7 @@ __device__ __forceinline__ void fact_tile_body(const FactArgs &a, const FactPlan
     bool ok[NB];
     double v_re[NB], v_im[NB], w_re[NB], w_im[NB], uph[NB], uth[NB];
     float sa[NB];
-    const int n0 = P.n_lo + (g >> 1);
+    const int n0 = P.tn0 + (g >> 1);
 #pragma unroll
     for (int c = 0; c < NB; c++) {
         idx[c] = sg.start + q * TILE + wv * (16 * NB) + c * 16 + li;
@@ -229,12 +218,14 @@ __device__ __forceinline__ void fact_tile_body(const FactArgs &a, const FactPlan
 
     const int e = li & 1;
     const int f = g & 1;
-    const int pidx = e ^ f;
-    const float sgn = (e == 0 && f == 1) ? -1.f : 1.f;
+    // A entry (row 2R + e, column 2 acol + f): [[Re, -Im], [Im, Re]] of the
+    // complex tile, or the folded tile's half f of the row
+    const int pidx = P.fold ? f * 4 * P.KS + e : e ^ f;
+    const float sgn = (!P.fold && e == 0 && f == 1) ? -1.f : 1.f;
     const int rbase = li >> 1;
     const int cst = P.CST;
     const int KS = P.KS;
-    const int Nc = P.Nc;
+    const int Nc = P.tcols;
     for (int s = 0; s < KS; s++) {
         const int acol = 2 * s + (g >> 1);
         float bv[NB];
@@ -390,7 +381,7 @@ __device__ __forceinline__ void fact64_tile_body(const FactArgs &a, const FactPl
     int idx[NB];
     bool ok[NB];
     double v_re[NB], v_im[NB], w_re[NB], w_im[NB], uph[NB], uth[NB], sa[NB];
-    const int n0 = P.n_lo + (g >> 1);
+    const int n0 = P.tn0 + (g >> 1);
 #pragma unroll
     for (int c = 0; c < NB; c++) {
         idx[c] = sg.start + q * TILE + wv * (16 * NB) + c * 16 + li;
@@ -417,11 +408,11 @@ __device__ __forceinline__ void fact64_tile_body(const FactArgs &a, const FactPl
         for (int c = 0; c < NB; c++) acc[rb][c] = v4d{0.0, 0.0, 0.0, 0.0};
     const int f = g & 1;
     const int r_loc = (li & 3) >> 1, p_row = 2 * (li & 1) + (li >> 3), e = (li >> 2) & 1;
-    const int pidx = e ^ f;
-    const double sgn = (e == 0 && f == 1) ? -1.0 : 1.0;
+    const int pidx = P.fold ? f * 4 * P.KS + e : e ^ f;
+    const double sgn = (!P.fold && e == 0 && f == 1) ? -1.0 : 1.0;
     const int cst = P.CST;
     const int KS = P.KS;
-    const int Nc = P.Nc;
+    const int Nc = P.tcols;
     for (int s = 0; s < KS; s++) {
         const int acol = 2 * s + (g >> 1);
         double bv[NB];

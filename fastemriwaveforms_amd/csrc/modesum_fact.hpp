@@ -29,15 +29,16 @@ struct FactPlanHost {
     int Nc = 0;             // columns: n_lo .. n_lo + Nc - 1
     int n_lo = 0;
     int RB = 0;             // 16-row MFMA blocks: ceil(8 * Nr / 16) (4 powers x re/im)
-    int KS = 0;             // 4-deep k-steps: ceil(2 * Nc / 4)
-    int CST = 0;            // LDS row stride in complex entries (== 2 mod 16)
+    int KS = 0;             // 4-deep k-steps: ceil(2 * tcols / 4)
+    int CST = 0;            // LDS row stride in pairs (== 2 mod 16): >= 2 KS, 4 KS folded
     int rows_arith = 0;     // rows are m0, m0+1, ... with one k: phasors by recurrence
     int m0 = 0, k0 = 0;
     int has_k = 0;          // some mode has k != 0 (Phi_theta enters the phase)
-    // Split-FP16 tile columns.  v_{-n} = conj(v_n), so with f16_fold the
-    // coefficients of columns n and -n share tile column n (n = 0 ..
-    // f16_nc - 1) as real pairs; otherwise tile column c is n = f16_n0 + c.
-    int f16_fold = 0, f16_nc = 0, f16_n0 = 0;
+    // Columns of the coefficient tiles.  v_{-n} = conj(v_n), so with fold the
+    // coefficients of columns n and -n share tile column n (n = 0 .. tcols - 1)
+    // as a real 2 x 2 block (fact_cell); otherwise tile column c is n = tn0 + c.
+    // Folding is taken when the n range reaches below zero and it shortens the tiles.
+    int fold = 0, tcols = 0, tn0 = 0;
     std::vector<int> row_m, row_k;        // (Nr,)
     std::vector<int> cell_start;          // (Nr*Nc + 1,)
     std::vector<int> entries;             // q*2 + conj, grouped by cell, q ascending
@@ -49,7 +50,7 @@ struct FactPlanHost {
 struct FactPlanDev {
     int K, Nr, Nc, n_lo, RB, KS, CST;
     int rows_arith, m0, k0, has_k;
-    int f16_fold, f16_nc, f16_n0;
+    int fold, tcols, tn0;
     const int2 *row_mk;      // (Nr,)
     const int *cell_start;   // (Nr*Nc + 1,)
     const int *entries;
@@ -70,10 +71,69 @@ struct FactPlan {
 
 bool build_fact_plan(const int *m, const int *k, const int *n, int K, FactPlanHost &p);
 
-// Coefficient tiles: per interval, (RB*8 rows) x (CST complex) float2.
+// Coefficient tiles of the FP32 / FP64 sums: per interval, (RB*8 rows) x
+// (CST pairs).  Unfolded: C[R, c] at (R * CST + c) * 2 + {re, im}.  Folded:
+// the 2 x 2 block of tile column c, entry (row 2R + e, column 2c + f), at
+// (R * CST + f * 2 KS + c) * 2 + e (the f = 0 and f = 1 halves of a row side
+// by side, each 2 KS pairs wide).
 inline size_t fact_coef_floats_per_interval(const FactPlanHost &p)
 {
     return (size_t)p.RB * 8 * p.CST * 2;
+}
+
+// Adds C_p[r, col] (the entries of one cell, in list order) for the four
+// powers: sum over the cell's modes q of Y c_{p,q} (conjugate entries:
+// Y- conj(c_{p,q})), from the amplitude coefficients interp (4, L, 2K).
+__device__ __forceinline__ void fact_gather(const FactPlanDev &P, const double *interp,
+                                            const double2 *ylm, int L, int j, int cell,
+                                            double (&cre)[4], double (&cim)[4])
+{
+    const long K2 = 2L * P.K;
+    for (int e = P.cell_start[cell]; e < P.cell_start[cell + 1]; e++) {
+        const int ent = P.entries[e];
+        const int qm = ent >> 1;
+        const bool cj = ent & 1;
+        const double2 Y = ylm[cj ? P.K + qm : qm];
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const long base = ((long)p * L + j) * K2;
+            const double cr = interp[base + qm];
+            const double ci = cj ? -interp[base + P.K + qm] : interp[base + P.K + qm];
+            cre[p] += cr * Y.x - ci * Y.y;
+            cim[p] += cr * Y.y + ci * Y.x;
+        }
+    }
+}
+
+// The real 2 x 2 block of complex row r, tile column kc and each power p,
+// scaled by h^p: x[p] = {(2R, 2kc), (2R, 2kc + 1), (2R + 1, 2kc), (2R + 1, 2kc + 1)}
+// of R = 4 r + p, against the B rows (Re v_n, Im v_n) of tile column kc.
+// With C = C_p[r, n] and D = C_p[r, -n] (D = 0 unless the tiles fold and n > 0),
+//   C v_n + D conj(v_n) = (Cr + Dr) Re v - (Ci - Di) Im v
+//                       + i [(Ci + Di) Re v + (Cr - Dr) Im v].
+// Padding rows and columns are zero.
+__device__ __forceinline__ void fact_cell(const FactPlanDev &P, const double *interp,
+                                          const double2 *ylm, int L, int j, double hw, int r,
+                                          int kc, double (&x)[4][4])
+{
+    double cre[4] = {0.0, 0.0, 0.0, 0.0}, cim[4] = {0.0, 0.0, 0.0, 0.0};
+    double dre[4] = {0.0, 0.0, 0.0, 0.0}, dim[4] = {0.0, 0.0, 0.0, 0.0};
+    if (r < P.Nr && kc < P.tcols) {
+        const int n = P.tn0 + kc;
+        const int col = n - P.n_lo;
+        if (col < P.Nc) fact_gather(P, interp, ylm, L, j, r * P.Nc + col, cre, cim);
+        const int colm = -n - P.n_lo;
+        if (P.fold && n > 0 && colm >= 0) fact_gather(P, interp, ylm, L, j, r * P.Nc + colm, dre, dim);
+    }
+    double hp = 1.0;
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        x[p][0] = (cre[p] + dre[p]) * hp;
+        x[p][1] = (dim[p] - cim[p]) * hp;
+        x[p][2] = (cim[p] + dim[p]) * hp;
+        x[p][3] = (cre[p] - dre[p]) * hp;
+        hp *= hw;
+    }
 }
 
 struct FactArgs {
@@ -117,7 +177,7 @@ int launch_fact64_sum(const FactArgs &a, const FactPlanHost &p, int max_blocks, 
 // floats of inverse scales (one per power p), padded to 256 bytes.  The four 16-byte
 // chunks of a row are XOR-swizzled by (row >> 1) & 3, which puts each group of
 // eight lanes of a fragment read on distinct LDS banks without padding.
-inline int fact16_kb(const FactPlanHost &p) { return (2 * p.f16_nc + 31) / 32; }
+inline int fact16_kb(const FactPlanHost &p) { return (2 * p.tcols + 31) / 32; }
 inline size_t fact16_tile_bytes(const FactPlanHost &p)
 {
     const size_t b = 2 * (size_t)fact16_kb(p) * p.RB * 1024 + 4 * sizeof(float);

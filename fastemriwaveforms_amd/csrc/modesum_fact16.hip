@@ -69,59 +69,6 @@ __device__ __forceinline__ int f16_lane_off(int li, int g) { return li * 32 + ((
 // are gathered again rather than held).
 #define F16_PREP_ROWS 4  // complex rows r per prep workgroup, one wave each
 
-// Adds C_p[r, col] (the entries of one cell, in list order) for the four powers.
-__device__ __forceinline__ void f16_gather(const FactPlanDev &P, const double *interp,
-                                           const double2 *ylm, int L, int j, int cell,
-                                           double (&cre)[4], double (&cim)[4])
-{
-    const long K2 = 2L * P.K;
-    for (int e = P.cell_start[cell]; e < P.cell_start[cell + 1]; e++) {
-        const int ent = P.entries[e];
-        const int qm = ent >> 1;
-        const bool cj = ent & 1;
-        const double2 Y = ylm[cj ? P.K + qm : qm];
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-            const long base = ((long)p * L + j) * K2;
-            const double cr = interp[base + qm];
-            const double ci = cj ? -interp[base + P.K + qm] : interp[base + P.K + qm];
-            cre[p] += cr * Y.x - ci * Y.y;
-            cim[p] += cr * Y.y + ci * Y.x;
-        }
-    }
-}
-
-// The four real tile entries of complex row r, tile column kc and each power p,
-// scaled by h^p: x[p] = {row 2R col 2kc, row 2R col 2kc + 1, row 2R + 1 col 2kc,
-// row 2R + 1 col 2kc + 1} of R = 4 r + p, against B rows (Re v_n, Im v_n).
-// With C = C_p[r, n] and D = C_p[r, -n] (D = 0 unless the tiles fold and n > 0),
-//   C v_n + D conj(v_n) = (Cr + Dr) Re v - (Ci - Di) Im v
-//                       + i [(Ci + Di) Re v + (Cr - Dr) Im v].
-// Padding rows and columns are zero.
-__device__ __forceinline__ void f16_cell(const FactPlanDev &P, const double *interp,
-                                         const double2 *ylm, int L, int j, double hw, int r,
-                                         int kc, double (&x)[4][4])
-{
-    double cre[4] = {0.0, 0.0, 0.0, 0.0}, cim[4] = {0.0, 0.0, 0.0, 0.0};
-    double dre[4] = {0.0, 0.0, 0.0, 0.0}, dim[4] = {0.0, 0.0, 0.0, 0.0};
-    if (r < P.Nr && kc < P.f16_nc) {
-        const int n = P.f16_n0 + kc;
-        const int col = n - P.n_lo;
-        if (col < P.Nc) f16_gather(P, interp, ylm, L, j, r * P.Nc + col, cre, cim);
-        const int colm = -n - P.n_lo;
-        if (P.f16_fold && n > 0 && colm >= 0) f16_gather(P, interp, ylm, L, j, r * P.Nc + colm, dre, dim);
-    }
-    double hp = 1.0;
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        x[p][0] = (cre[p] + dre[p]) * hp;
-        x[p][1] = (dim[p] - cim[p]) * hp;
-        x[p][2] = (cim[p] + dim[p]) * hp;
-        x[p][3] = (cre[p] - dre[p]) * hp;
-        hp *= hw;
-    }
-}
-
 __device__ __forceinline__ double f16_cell_max(const double (&x)[4])
 {
     return fmax(fmax(fabs(x[0]), fabs(x[1])), fmax(fabs(x[2]), fabs(x[3])));
@@ -142,7 +89,7 @@ __device__ void fact16_rowmax_block(const FactPlanDev &P, const double *interp, 
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
     for (int col = lane; col < ncolc; col += 64) {
         double x[4][4];
-        f16_cell(P, interp, ylm, L, j, hw, r, col, x);
+        fact_cell(P, interp, ylm, L, j, hw, r, col, x);
 #pragma unroll
         for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], f16_cell_max(x[p]));
     }
@@ -208,7 +155,7 @@ __device__ void fact16_split_block(const FactPlanDev &P, const double *interp, c
     for (int p = 0; p < 4; p++) sigma[p] = s_sigma[p];
     for (int col = lane; col < ncolc; col += 64) {
         double x[4][4];
-        f16_cell(P, interp, ylm, L, j, hw, r, col, x);
+        fact_cell(P, interp, ylm, L, j, hw, r, col, x);
         f16_store_cell(hi, lo, RB, r, col, x, sigma);
     }
 }
@@ -259,7 +206,7 @@ __device__ void fact16_prep_interval_cached(const FactPlanDev &P, const double *
         const int ci = tid + k * F16_PREP_THREADS;
         const int r = ci / ncolc, col = ci - r * ncolc;
         if (ci < ncell) {
-            f16_cell(P, interp, ylm, L, j, hw, r, col, x[k]);
+            fact_cell(P, interp, ylm, L, j, hw, r, col, x[k]);
         } else {
 #pragma unroll
             for (int p = 0; p < 4; p++) x[k][p][0] = x[k][p][1] = x[k][p][2] = x[k][p][3] = 0.0;
@@ -304,7 +251,7 @@ __device__ void fact16_prep_interval(const FactPlanDev &P, const double *interp,
     for (int ci = tid; ci < ncell; ci += F16_PREP_THREADS) {
         const int r = ci / ncolc, col = ci - r * ncolc;
         double x[4][4];
-        f16_cell(P, interp, ylm, L, j, hw, r, col, x);
+        fact_cell(P, interp, ylm, L, j, hw, r, col, x);
 #pragma unroll
         for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], f16_cell_max(x[p]));
     }
@@ -323,7 +270,7 @@ __device__ void fact16_prep_interval(const FactPlanDev &P, const double *interp,
     for (int ci = tid; ci < ncell; ci += F16_PREP_THREADS) {
         const int r = ci / ncolc, col = ci - r * ncolc;
         double x[4][4];
-        f16_cell(P, interp, ylm, L, j, hw, r, col, x);
+        fact_cell(P, interp, ylm, L, j, hw, r, col, x);
         f16_store_cell(hi, lo, RB, r, col, x, sigma);
     }
 }
@@ -533,7 +480,7 @@ __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sP
         }
         // lane group g needs v_n for n = n0 + 16 kb + 4 g + t, t = 0..3 (scaled by 256)
         double sn, cs;
-        sincos_turns_circle(reduce_turns((double)(P.f16_n0 + 4 * g) * ur), sn, cs);
+        sincos_turns_circle(reduce_turns((double)(P.tn0 + 4 * g) * ur), sn, cs);
         const double zr = zcs, zi = -zsn;  // z = exp(-i Phi_r)
         double vr = 256.0 * cs, vi = -256.0 * sn;
         V.vr[c][0] = (f16v_t)vr;

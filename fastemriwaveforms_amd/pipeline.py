@@ -51,10 +51,10 @@ class ModePlan:
         self.supported = bool(info[6] & 1)
         self.supported16 = bool(info[6] & 2)
         self.supported64 = bool(info[6] & 4)
-        # complex columns of the MIXED16 tiles (columns n and -n share one when folded)
-        self.folded16 = bool(info[6] & 8)
-        self.cols16 = (max(-self.n_lo, self.n_lo + self.cols - 1) + 1 if self.folded16
-                       else self.cols)
+        # complex columns of the coefficient tiles (columns n and -n share one when folded)
+        self.folded = bool(info[6] & 8)
+        self.tile_cols = (max(-self.n_lo, self.n_lo + self.cols - 1) + 1 if self.folded
+                          else self.cols)
 
     def workspace_bytes(self, L, N, precision) -> int:
         return int(lib().few_plan_workspace_bytes(self.handle, L, N, PRECISION_CODES[precision]))
@@ -191,7 +191,7 @@ class BatchProblem:
         union = ModePlan(m, k, n)
         self.plan_rows, self.plan_cols = union.rows, union.cols
         self.plan_row_blocks, self.plan_ksteps = union.row_blocks, union.ksteps
-        self.plan_cols16 = union.cols16
+        self.plan_tile_cols = union.tile_cols
         del union
         self.mode_samples = int(np.sum(K.astype(np.int64) * N))
         self.samples = int(np.sum(N))

@@ -125,7 +125,8 @@ int few_prepare_modes_async(const double *teuk, const double *phase_knots, const
  * Returns NULL on error.  info7 = {K, rows, cols, n_lo, row blocks,
  * k-steps, flags}; flags is a bit mask (1: FEW_PRECISION_MIXED supported,
  * 2: FEW_PRECISION_MIXED16 supported, 4: FEW_PRECISION_FACT64 supported,
- * 8: the MIXED16 tiles fold column -n onto n, max|n| + 1 tile columns);
+ * 8: the coefficient tiles of the factorised sums fold column -n onto n,
+ * max|n| + 1 tile columns);
  * unsupported sets (too many rows/columns for the MFMA tiles) can still use
  * FEW_PRECISION_FP64.
  */

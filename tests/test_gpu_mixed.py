@@ -77,7 +77,7 @@ def test_mixed_explicit_pm_modes(gpu, prec):
     h, p = _run(inp, prec)
     assert p.plan.supported16
     assert p.plan.cols == 111
-    assert p.plan.folded16 and p.plan.cols16 == 56  # columns n and -n share a tile column
+    assert p.plan.folded and p.plan.tile_cols == 56  # columns n and -n share a tile column
     idx = spread_indices(inp.num_pts, count=1500)
     ref = oracle_waveform(inp, idx)
     err = oracle.rel_l2(h[idx], ref)
@@ -86,12 +86,13 @@ def test_mixed_explicit_pm_modes(gpu, prec):
 
 
 @pytest.mark.parametrize("case", ["asymmetric_folded", "narrow_unfolded"])
-def test_mixed16_column_folding(gpu, case):
-    """The split-FP16 tiles fold column -n onto n when that needs fewer 32-column
-    steps.  asymmetric_folded: m = 0 modes with n in [-30, 10] beside m != 0 modes
-    with |n| <= 5 (n range [-30, 10]; tile columns 11..30 hold only their -n
-    partner); narrow_unfolded: |n| <= 7 (15 columns, one step either way, no fold).
-    Against the oracle and the FP64 GPU path."""
+def test_column_folding(gpu, case):
+    """The coefficient tiles fold column -n onto n when the n range reaches below
+    zero and folding shortens them.  asymmetric_folded: m = 0 modes with n in
+    [-30, 10] beside m != 0 modes with |n| <= 5 (n range [-30, 10]; tile columns
+    11..30 hold only their -n partner); narrow_unfolded: n in [0, 7] (no negative
+    n, no fold).  mixed16 and mixed against the oracle and the FP64 GPU path,
+    fact64 against the FP64 GPU path."""
     from fastemriwaveforms_amd import synthetic
     inp = synthetic.small_input(seed=37, L=16, K=60, T_years=0.004)
     rng = np.random.default_rng(5)
@@ -102,20 +103,25 @@ def test_mixed16_column_folding(gpu, case):
         inp.n = np.where(zero, rng.integers(-30, 11, K), rng.integers(-5, 6, K)).astype(inp.n.dtype)
         inp.n[0] = -30
     else:
-        inp.n = rng.integers(-7, 8, K).astype(inp.n.dtype)
-    h, p = _run(inp, "mixed16")
-    assert p.plan.supported16
+        inp.m = np.zeros_like(inp.m)
+        inp.n = rng.integers(0, 8, K).astype(inp.n.dtype)
+    h64, p = _run(inp, "fp64")
+    assert p.plan.supported and p.plan.supported16 and p.plan.supported64
     if case == "asymmetric_folded":
-        assert p.plan.n_lo == -30 and p.plan.cols == 41 and p.plan.folded16 and p.plan.cols16 == 31
+        assert p.plan.n_lo == -30 and p.plan.cols == 41 and p.plan.folded and p.plan.tile_cols == 31
     else:
-        assert not p.plan.folded16 and p.plan.cols16 == p.plan.cols
+        assert not p.plan.folded and p.plan.tile_cols == p.plan.cols
     ref = oracle_waveform(inp)
-    err = oracle.rel_l2(h, ref)
-    h64, _ = _run(inp, "fp64")
-    err64 = oracle.rel_l2(h, h64)
-    print(f"{case}: cols {p.plan.cols} -> {p.plan.cols16}, rel-L2 vs oracle {err:.3e}, "
-          f"vs fp64 GPU {err64:.3e}")
-    assert err < MIXED_TOL and err64 < MIXED_TOL
+    for prec in ("mixed16", "mixed"):
+        h, _ = _run(inp, prec)
+        err, err64 = oracle.rel_l2(h, ref), oracle.rel_l2(h, h64)
+        print(f"{case} {prec}: cols {p.plan.cols} -> {p.plan.tile_cols}, rel-L2 vs oracle "
+              f"{err:.3e}, vs fp64 GPU {err64:.3e}")
+        assert err < MIXED_TOL and err64 < MIXED_TOL
+    hf, _ = _run(inp, "fact64")
+    errf = oracle.rel_l2(hf, h64)
+    print(f"{case} fact64: rel-L2 vs fp64 GPU {errf:.3e}")
+    assert errf < 1e-9
 
 
 @pytest.mark.parametrize("prec", PRECS)
