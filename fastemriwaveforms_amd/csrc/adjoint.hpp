@@ -97,7 +97,8 @@ struct Adj16Args {
     int chunk;
     FactPlanDev plan;
     int RBM;                   // 16-row blocks of the moment tile: ceil(2 * 7 * Nr / 16)
-    int NBC;                   // 16-column blocks (waves): ceil(Nc / 16)
+    int NBC;                   // 16-column blocks (waves): ceil(Nc / 16); FP64 with folded
+                               // tiles: ceil(2 tcols / 16)
     float *part;               // (max_chunks, RBM*16, NBC*16) chunk moments (split FP16)
     double *part64;            // (max_chunks, RBM*16, NBC*16) chunk moments (FP64 MFMA)
 };
@@ -121,7 +122,7 @@ size_t adjoint16_lds_bytes(int RBM, int NBC, int Nr);
 #define A64_CHUNK 2048         // samples per chunk of the FP64 factorised adjoint
 size_t adjoint64_lds_bytes(int RBM, int NBC);
 int launch_adjoint64(const Adj16Args &a, int max_chunks, hipStream_t stream);
-int launch_adjoint64_reduce(const Adj16Args &a, long tile, double *M, hipStream_t stream);
+int launch_adjoint64_reduce(const Adj16Args &a, long tile, int ncol, double *M, hipStream_t stream);
 int launch_adjoint16(const Adj16Args &a, int max_chunks, hipStream_t stream);
 // reduce_chunks: add the FP32 chunk moments of the split-FP16 adjoint into f.M
 // first (the FP64 factorised adjoint reduces its own chunks)

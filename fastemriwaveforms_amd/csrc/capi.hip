@@ -454,7 +454,9 @@ static AdjLayout adj_layout(int L, int out_len, int K)
 // Mixed-precision adjoint: shapes and workspace.
 struct Adj16Layout {
     int RBM = 0, NBC = 0, groups = 0;
-    long max_chunks = 0, tile = 0;
+    int NBCM = 0;                  // column blocks of the interval moments M (ceil(Nc / 16))
+    long max_chunks = 0, tile = 0;  // tile: RBM*16 x NBCM*16, one interval's M
+    long tile_part = 0;            // RBM*16 x NBC*16, one chunk's moments
     size_t segs = 0, prefix = 0, part = 0, M = 0, phase = 0, total = 0;
     bool ok = false;
 };
@@ -463,8 +465,10 @@ static Adj16Layout adj16_layout(const FactPlanHost &h, int L, int out_len)
     Adj16Layout a;
     a.RBM = (2 * A16_MOM * h.Nr + 15) / 16;
     a.NBC = (h.Nc + 15) / 16;
+    a.NBCM = a.NBC;
     a.groups = (h.K + 255) / 256;
     a.tile = (long)a.RBM * 16 * a.NBC * 16;
+    a.tile_part = a.tile;
     a.max_chunks = ((long)out_len + A16_CHUNK - 1) / A16_CHUNK + (L > 1 ? L - 1 : 1);
     size_t off = 0;
     a.segs = off; off = align256(off + sizeof(Segment) * (size_t)(L > 1 ? L - 1 : 1));
@@ -478,19 +482,22 @@ static Adj16Layout adj16_layout(const FactPlanHost &h, int L, int out_len)
     return a;
 }
 
-// FP64 factorised adjoint: the split-FP16 adjoint's shapes, FP64 chunk moments.
+// FP64 factorised adjoint: the split-FP16 adjoint's shapes, FP64 chunk moments;
+// with folded tiles the chunk moments have 2 tcols real columns (n >= 0).
 static Adj16Layout adj64_layout(const FactPlanHost &h, int L, int out_len)
 {
     Adj16Layout a;
     a.RBM = (2 * A16_MOM * h.Nr + 15) / 16;
-    a.NBC = (h.Nc + 15) / 16;
+    a.NBCM = (h.Nc + 15) / 16;
+    a.NBC = h.fold ? (2 * h.tcols + 15) / 16 : a.NBCM;
     a.groups = (h.K + 255) / 256;
-    a.tile = (long)a.RBM * 16 * a.NBC * 16;
+    a.tile = (long)a.RBM * 16 * a.NBCM * 16;
+    a.tile_part = (long)a.RBM * 16 * a.NBC * 16;
     a.max_chunks = ((long)out_len + A64_CHUNK - 1) / A64_CHUNK + (L > 1 ? L - 1 : 1);
     size_t off = 0;
     a.segs = off; off = align256(off + sizeof(Segment) * (size_t)(L > 1 ? L - 1 : 1));
     a.prefix = off; off = align256(off + sizeof(int) * (size_t)L);
-    a.part = off; off = align256(off + sizeof(double) * (size_t)a.max_chunks * a.tile);
+    a.part = off; off = align256(off + sizeof(double) * (size_t)a.max_chunks * a.tile_part);
     a.M = off; off = align256(off + sizeof(double) * (size_t)(L > 1 ? L - 1 : 1) * a.tile);
     a.phase = off; off = align256(off + sizeof(double) * (size_t)L * a.groups * 12);
     a.total = off;
@@ -943,7 +950,7 @@ int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_
     f.plan = pl->dev;
     f.M = reinterpret_cast<double *>(ws + al.M);
     f.tile = al.tile;
-    f.ncol = al.NBC * 16;
+    f.ncol = al.NBCM * 16;
     f.h_t = h_t;
     f.L = L;
     f.interp = interp_array;
@@ -955,7 +962,7 @@ int few_waveform_adjoint_plan_async(const void *grad_wave, const double *interp_
     f.g_phase = g_phase;
     f.part_phase = reinterpret_cast<double *>(ws + al.phase);
     if (f64) {
-        rc = launch_adjoint64_reduce(a, al.tile, f.M, st);
+        rc = launch_adjoint64_reduce(a, al.tile, f.ncol, f.M, st);
         if (rc) return rc;
         return launch_adjoint16_finish(a, f, st, false);
     }
