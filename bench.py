@@ -44,7 +44,7 @@ FACT16_FULL1_TRAFFIC = 2.00 * 16244.8 * 1024 + 1.00 * 49313.2 * 1024
 # HBM bytes per launch of batch_fact16_kernel on the 256-waveform batch, from two
 # rocprofv3 PMC runs (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a 1 GiB
 # copy in the same process)
-BATCH_FACT16_TRAFFIC = 2.00 * 1391768.4 * 1024 + 1.00 * 13008271.3 * 1024
+BATCH_FACT16_TRAFFIC = 2.00 * 712088.5 * 1024 + 1.00 * 13008300.7 * 1024
 BATCH_TRAFFIC_SOURCE = "profiles/batch_fact16_256wave_hbm_traffic.md (rocprofv3 PMC)"
 DIRECT_FLOP = 37  # algorithmic FLOP per mode-sample of the direct sum (sincos counted 0)
 
