@@ -40,7 +40,7 @@ PEAK = {"fp64": 78.6, "fp32_mfma": 157.3, "fp16_mfma": 2500.0}  # MI355X TFLOP/s
 # HBM bytes per launch of fact16_kernel on full1, from two rocprofv3 PMC runs
 # (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a 1 GiB copy in the same
 # process): profiles/fact16_full1_hbm_traffic.md
-FACT16_FULL1_TRAFFIC = 2.00 * 16244.8 * 1024 + 1.00 * 49313.2 * 1024
+FACT16_FULL1_TRAFFIC = 2.00 * 11916.8 * 1024 + 1.00 * 49316.8 * 1024
 # HBM bytes per launch of batch_fact16_kernel on the 256-waveform batch, from two
 # rocprofv3 PMC runs (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a 1 GiB
 # copy in the same process)
