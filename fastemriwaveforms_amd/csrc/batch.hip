@@ -34,7 +34,7 @@ static inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Workspace layout of a batch run.
 struct BatchWs {
-    size_t segs, tpref, wt, maxes, region, total;
+    size_t segs, tpref, wt, region, total;
 };
 static BatchWs batch_ws_layout(const BatchPlan &p)
 {
@@ -43,7 +43,6 @@ static BatchWs batch_ws_layout(const BatchPlan &p)
     w.segs = off; off = al256(off + sizeof(Segment) * (size_t)(p.total_intervals > 0 ? p.total_intervals : 1));
     w.tpref = off; off = al256(off + sizeof(int) * (size_t)(p.total_tpref > 0 ? p.total_tpref : 1));
     w.wt = off; off = al256(off + sizeof(int) * (size_t)(p.B + 1));
-    w.maxes = off; off += p.uni.ok ? fact16_max_bytes(p.uni, p.total_intervals) : 0;
     w.region = off;
     const size_t scratch = sizeof(double) * (size_t)p.tot[7];
     const size_t tiles = p.uni.ok ? fact16_tile_bytes(p.uni) * (size_t)p.total_intervals : 0;
@@ -342,8 +341,7 @@ int few_batch_run_async(const void *batch, const double *t, const double *teuk,
     const long tb = (long)fact16_tile_bytes(bp.uni);
     const int sub = ((stage & 2) ? 1 : 0) | ((stage & 4) ? 2 : 0) | ((stage & 8) ? 4 : 0);
     if (!sub) return FEW_OK;
-    return launch_batch_fact16(bp.dev, r, p, tiles, tb, reinterpret_cast<double *>(w + L.maxes),
-                               fact16_kb(bp.uni),
+    return launch_batch_fact16(bp.dev, r, p, tiles, tb, fact16_kb(bp.uni),
                                variant, sub, s);
 }
 

@@ -98,7 +98,7 @@ int launch_batch_splines(const BatchDev &B, const BatchPtrs &p, hipStream_t stre
 int launch_batch_plan(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p, int tile,
                       hipStream_t stream);
 int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
-                        unsigned char *tiles, long tile_bytes, double *maxes, int KB, int variant,
+                        unsigned char *tiles, long tile_bytes, int KB, int variant,
                         int stage, hipStream_t stream);
 int launch_batch_direct_fp64(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
                              int max_blocks, hipStream_t stream);

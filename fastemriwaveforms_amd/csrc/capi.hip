@@ -121,17 +121,17 @@ static PlanLayout plan_layout(int L)
 
 // Workspace of the factorised path: plan layout, then the coefficient tiles.
 static size_t fact_coef_offset(int L) { return plan_layout(L).total; }
-static size_t fact_tiles_bytes(const FactPlanHost &p, int L)
+static size_t fact_tiles_bytes(const FactPlanHost &p, int L, int precision)
 {
     const size_t n = (size_t)(L > 1 ? L - 1 : 1);
-    const size_t f64 = align256(sizeof(double) * fact_coef_floats_per_interval(p) * n);
-    const size_t f16 = align256(fact16_tile_bytes(p) * n);
-    return f64 > f16 ? f64 : f16;  // the FP32 tiles are half the FP64 ones
+    if (precision == FEW_PRECISION_MIXED16) return align256(fact16_tile_bytes(p) * n);
+    const size_t elem = precision == FEW_PRECISION_FACT64 ? sizeof(double) : sizeof(float);
+    return align256(elem * fact_coef_floats_per_interval(p) * n);
 }
-// plan layout | coefficient tiles | row maxima of the mixed16 tile prep
-static size_t fact_ws_bytes(const FactPlanHost &p, int L)
+// plan layout | coefficient tiles of the precision
+static size_t fact_ws_bytes(const FactPlanHost &p, int L, int precision)
 {
-    return fact_coef_offset(L) + fact_tiles_bytes(p, L) + fact16_max_bytes(p, L - 1);
+    return fact_coef_offset(L) + fact_tiles_bytes(p, L, precision);
 }
 static bool fact16_supported(const FactPlanHost &p)
 {
@@ -269,9 +269,9 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
             set_error("get_waveform: plan built for %d modes, called with %d", ph.K, K);
             return FEW_ERR_ARG;
         }
-        if (ws_bytes < fact_ws_bytes(ph, L)) {
+        if (ws_bytes < fact_ws_bytes(ph, L, precision)) {
             set_error("get_waveform: workspace of %zu bytes, need %zu", ws_bytes,
-                      fact_ws_bytes(ph, L));
+                      fact_ws_bytes(ph, L, precision));
             return FEW_ERR_ARG;
         }
         unsigned char *tiles = reinterpret_cast<unsigned char *>(ws + fact_coef_offset(L));
@@ -288,10 +288,8 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
             PlanArgs p{h_t, pt, L, dt, out_len, tile, segs, prefix};
             int rc = launch_plan(p, stream);
             if (rc) return rc;
-            double *maxes = reinterpret_cast<double *>(ws + fact_coef_offset(L) +
-                                                       fact_tiles_bytes(ph, L));
             Fact16PrepArgs pa{interp, ylm, h_t, L, tiles, tb, fact16_kb(ph), ph.RB,
-                              plan->dev, maxes};
+                              plan->dev};
             rc = launch_fact16_prep(pa, stream);
             if (rc) return rc;
         }
@@ -312,9 +310,9 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
             set_error("get_waveform: plan built for %d modes, called with %d", ph.K, K);
             return FEW_ERR_ARG;
         }
-        if (ws_bytes < fact_ws_bytes(ph, L)) {
+        if (ws_bytes < fact_ws_bytes(ph, L, precision)) {
             set_error("get_waveform: workspace of %zu bytes, need %zu", ws_bytes,
-                      fact_ws_bytes(ph, L));
+                      fact_ws_bytes(ph, L, precision));
             return FEW_ERR_ARG;
         }
         float *C = reinterpret_cast<float *>(ws + fact_coef_offset(L));
@@ -353,9 +351,9 @@ static int run_waveform(double2 *wave, const double *interp, const double *pt, c
             set_error("get_waveform: plan built for %d modes, called with %d", ph.K, K);
             return FEW_ERR_ARG;
         }
-        if (ws_bytes < fact_ws_bytes(ph, L)) {
+        if (ws_bytes < fact_ws_bytes(ph, L, precision)) {
             set_error("get_waveform: workspace of %zu bytes, need %zu", ws_bytes,
-                      fact_ws_bytes(ph, L));
+                      fact_ws_bytes(ph, L, precision));
             return FEW_ERR_ARG;
         }
         double *C = reinterpret_cast<double *>(ws + fact_coef_offset(L));
@@ -747,7 +745,7 @@ size_t few_plan_workspace_bytes(const void *plan, int init_len, int out_len, int
     if (plan && (precision == FEW_PRECISION_MIXED || precision == FEW_PRECISION_MIXED16 ||
                  precision == FEW_PRECISION_FACT64)) {
         const FactPlanHost &h = static_cast<const FactPlan *>(plan)->host;
-        if (h.ok) b = fact_ws_bytes(h, L);
+        if (h.ok) b = fact_ws_bytes(h, L, precision);
     }
     return b;
 }

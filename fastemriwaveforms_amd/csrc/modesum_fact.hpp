@@ -193,15 +193,7 @@ struct Fact16PrepArgs {
     long tile_bytes;
     int KB, RB;
     FactPlanDev plan;
-    double *maxes;                // (L-1) x 2RB x 4 row maxima (workspace)
 };
-
-// Workspace of the row maxima the coefficient-tile prep passes between its two launches.
-inline size_t fact16_max_bytes(const FactPlanHost &p, long intervals)
-{
-    const size_t b = sizeof(double) * 8 * (size_t)p.RB * (size_t)(intervals > 0 ? intervals : 1);
-    return (b + 255) & ~(size_t)255;
-}
 
 // Wave priority of the mixed16 sum kernels (modesum_fact16.hip): with prio
 // a wave raises its priority over its MFMA contraction, so the two waves

@@ -61,44 +61,17 @@ __device__ __forceinline__ int f16_lane_off(int li, int g) { return li * 32 + ((
 #define INV_2PI16 0.15915494309189533577
 
 // ------------------------------------------------------------- prep kernel
-// Gathers C_p[r, a] for every cell and power p (FP64, fixed order per cell),
-// scales the rows of each power of an interval by one power of two (exact;
-// the largest entry of the power near 2^14, so no FP16 overflow) and writes
-// the split FP16 planes.  Two launches over (interval, group of four rows):
-// the row maxima, then the interval's scales and the split values (the cells
-// are gathered again rather than held).
-#define F16_PREP_ROWS 4  // complex rows r per prep workgroup, one wave each
+// One 512-thread workgroup per interval: each thread gathers its cells
+// (fact_cell: C_p[r, n], FP64, fixed order per cell, h^p applied), the
+// workgroup reduces the largest entry of each power, each power's rows are
+// scaled by one power of two (exact; the largest entry near 2^14, so no FP16
+// overflow) and the split FP16 planes are written.  The gathers are the
+// kernel's HBM traffic: up to 4 cells per thread are kept in registers
+// between the reduction and the split, larger tiles gather twice.
 
 __device__ __forceinline__ double f16_cell_max(const double (&x)[4])
 {
     return fmax(fmax(fabs(x[0]), fabs(x[1])), fmax(fabs(x[2]), fabs(x[3])));
-}
-
-// Pass 1: workgroup (j, row group) -- one wave per complex row r, lanes over
-// the columns -- writes the row's largest |Re|, |Im| of each power to
-// maxes[r][p] of interval j.
-__device__ void fact16_rowmax_block(const FactPlanDev &P, const double *interp, const double2 *ylm,
-                                    const double *h_t, int L, int j, int rg, int KB, int RB,
-                                    double *maxes)
-{
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int r = rg * F16_PREP_ROWS + wv;
-    if (r >= 2 * RB) return;  // uniform over the wave
-    const int ncolc = 16 * KB;
-    const double hw = h_t[j + 1] - h_t[j];
-    double mx[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int col = lane; col < ncolc; col += 64) {
-        double x[4][4];
-        fact_cell(P, interp, ylm, L, j, hw, r, col, x);
-#pragma unroll
-        for (int p = 0; p < 4; p++) mx[p] = fmax(mx[p], f16_cell_max(x[p]));
-    }
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) mx[p] = fmax(mx[p], __shfl_xor(mx[p], off, 64));
-    }
-    if (lane < 4) maxes[r * 4 + lane] = lane == 0 ? mx[0] : lane == 1 ? mx[1] : lane == 2 ? mx[2] : mx[3];
 }
 
 // The split of one cell's powers into the two planes (rows 2R and 2R + 1 of
@@ -122,49 +95,7 @@ __device__ __forceinline__ void f16_store_cell(_Float16 *hi, _Float16 *lo, int R
     }
 }
 
-// Pass 2: the interval's scale per power from the row maxima, then the split
-// planes of the workgroup's rows.
-__device__ void fact16_split_block(const FactPlanDev &P, const double *interp, const double2 *ylm,
-                                   const double *h_t, int L, int j, int rg, int KB, int RB,
-                                   const double *maxes, unsigned char *tile)
-{
-    __shared__ double s_sigma[4];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const long plane = (long)KB * RB * F16_BLK;
-    _Float16 *hi = reinterpret_cast<_Float16 *>(tile);
-    _Float16 *lo = hi + plane;
-    if (tid < 4) {
-        double m = 0.0;
-        for (int rr = 0; rr < 2 * RB; rr++) m = fmax(m, maxes[rr * 4 + tid]);
-        // sigma = 2^e with m * sigma in [2^13, 2^14): exact scaling
-        int ex = 0;
-        if (m > 0.0) {
-            frexp(m, &ex);  // m = f * 2^ex, f in [0.5, 1)
-            ex = 14 - ex;
-        }
-        s_sigma[tid] = ldexp(1.0, ex);
-        if (rg == 0) reinterpret_cast<float *>(lo + plane)[tid] = (float)ldexp(1.0, -ex);
-    }
-    __syncthreads();
-    const int r = rg * F16_PREP_ROWS + wv;
-    if (r >= 2 * RB) return;
-    const int ncolc = 16 * KB;
-    const double hw = h_t[j + 1] - h_t[j];
-    double sigma[4];
-#pragma unroll
-    for (int p = 0; p < 4; p++) sigma[p] = s_sigma[p];
-    for (int col = lane; col < ncolc; col += 64) {
-        double x[4][4];
-        fact_cell(P, interp, ylm, L, j, hw, r, col, x);
-        f16_store_cell(hi, lo, RB, r, col, x, sigma);
-    }
-}
-
-// Both passes in one workgroup of F16_PREP_THREADS per interval (the block
-// reduces the maxima itself): for launches with many intervals (batches),
-// where one workgroup per interval already fills the device.
 #define F16_PREP_THREADS 512
-#define F16_PREP_WHOLE_MIN 1024  // intervals from which a launch takes this form
 
 // The interval's scales from the per-wave maxima in s_red; writes the inverse
 // scales after the planes (thread 0).
@@ -282,31 +213,11 @@ __global__ __launch_bounds__(F16_PREP_THREADS) void fact16_prep_whole_kernel(Fac
                          a.KB, a.RB);
 }
 
-__global__ __launch_bounds__(64 * F16_PREP_ROWS) void fact16_rowmax_kernel(Fact16PrepArgs a)
-{
-    const int j = blockIdx.x;
-    fact16_rowmax_block(a.plan, a.interp, a.ylm, a.h_t, a.L, j, blockIdx.y, a.KB, a.RB,
-                        a.maxes + (long)j * 8 * a.RB);
-}
-
-__global__ __launch_bounds__(64 * F16_PREP_ROWS) void fact16_prep_kernel(Fact16PrepArgs a)
-{
-    const int j = blockIdx.x;
-    fact16_split_block(a.plan, a.interp, a.ylm, a.h_t, a.L, j, blockIdx.y, a.KB, a.RB,
-                       a.maxes + (long)j * 8 * a.RB, a.tiles + (long)j * a.tile_bytes);
-}
-
 int launch_fact16_prep(const Fact16PrepArgs &a, hipStream_t stream)
 {
     if (a.L < 2) return FEW_OK;
-    if (a.L - 1 >= F16_PREP_WHOLE_MIN) {
-        hipLaunchKernelGGL(fact16_prep_whole_kernel, dim3(a.L - 1), dim3(F16_PREP_THREADS), 0,
-                           stream, a);
-    } else {
-        const dim3 grid(a.L - 1, (2 * a.RB + F16_PREP_ROWS - 1) / F16_PREP_ROWS);
-        hipLaunchKernelGGL(fact16_rowmax_kernel, grid, dim3(64 * F16_PREP_ROWS), 0, stream, a);
-        hipLaunchKernelGGL(fact16_prep_kernel, grid, dim3(64 * F16_PREP_ROWS), 0, stream, a);
-    }
+    hipLaunchKernelGGL(fact16_prep_whole_kernel, dim3(a.L - 1), dim3(F16_PREP_THREADS), 0, stream,
+                       a);
     FEW_HIP(hipGetLastError());
     return FEW_OK;
 }
@@ -1080,31 +991,6 @@ __global__ __launch_bounds__(F16_PREP_THREADS) void batch_fact16_prep_whole_kern
                          j, tiles + (long)gi * tile_bytes, KB, RB);
 }
 
-__global__ __launch_bounds__(64 * F16_PREP_ROWS) void batch_fact16_rowmax_kernel(
-    BatchDev B, BatchPtrs p, double *maxes, int KB, int RB)
-{
-    const int gi = blockIdx.x;
-    const int b = upper_index(B.seg_prefix, B.B, gi);
-    const WaveDesc d = B.desc[b];
-    const int j = gi - B.seg_prefix[b];
-    fact16_rowmax_block(batch_wave_plan(B, d), p.interp + d.interp_off,
-                        reinterpret_cast<const double2 *>(p.ylm + d.ylm_off), p.t + d.t_off, d.L, j,
-                        blockIdx.y, KB, RB, maxes + (long)gi * 8 * RB);
-}
-
-__global__ __launch_bounds__(64 * F16_PREP_ROWS) void batch_fact16_prep_kernel(
-    BatchDev B, BatchPtrs p, const double *maxes, unsigned char *tiles, long tile_bytes, int KB,
-    int RB)
-{
-    const int gi = blockIdx.x;
-    const int b = upper_index(B.seg_prefix, B.B, gi);
-    const WaveDesc d = B.desc[b];
-    const int j = gi - B.seg_prefix[b];
-    fact16_split_block(batch_wave_plan(B, d), p.interp + d.interp_off,
-                       reinterpret_cast<const double2 *>(p.ylm + d.ylm_off), p.t + d.t_off, d.L, j,
-                       blockIdx.y, KB, RB, maxes + (long)gi * 8 * RB, tiles + (long)gi * tile_bytes);
-}
-
 int fact16_tile(int variant) { return 16 * F16_NB * fact16_waves(variant); }
 
 static int device_cus16()
@@ -1242,23 +1128,14 @@ static int launch_batch_fact16s_w(const BatchDev &B, const BatchRunDev &r, const
 }
 
 int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs &p,
-                        unsigned char *tiles, long tile_bytes, double *maxes, int KB, int variant,
+                        unsigned char *tiles, long tile_bytes, int KB, int variant,
                         int stage, hipStream_t stream)
 {
     if (stage & 1) {
         if (B.total_intervals > 0) {
-            if (B.total_intervals >= F16_PREP_WHOLE_MIN) {
-                hipLaunchKernelGGL(batch_fact16_prep_whole_kernel, dim3(B.total_intervals),
-                                   dim3(F16_PREP_THREADS), 0, stream, B, p, tiles, tile_bytes, KB,
-                                   B.uni.RB);
-            } else {
-                const dim3 grid(B.total_intervals,
-                                (2 * B.uni.RB + F16_PREP_ROWS - 1) / F16_PREP_ROWS);
-                hipLaunchKernelGGL(batch_fact16_rowmax_kernel, grid, dim3(64 * F16_PREP_ROWS), 0,
-                                   stream, B, p, maxes, KB, B.uni.RB);
-                hipLaunchKernelGGL(batch_fact16_prep_kernel, grid, dim3(64 * F16_PREP_ROWS), 0,
-                                   stream, B, p, maxes, tiles, tile_bytes, KB, B.uni.RB);
-            }
+            hipLaunchKernelGGL(batch_fact16_prep_whole_kernel, dim3(B.total_intervals),
+                               dim3(F16_PREP_THREADS), 0, stream, B, p, tiles, tile_bytes, KB,
+                               B.uni.RB);
             FEW_HIP(hipGetLastError());
         }
     }

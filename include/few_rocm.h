@@ -424,8 +424,8 @@ int few_release_workspace(void);
  * array of FEW_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends).
  * A call whose few_plan_workspace_bytes() exceeds that size is refused with
  * FEW_ERR_LIMIT (FP64 needs ~36 B per knot; MIXED16 adds the coefficient
- * tiles: 10.8 MB for 120 knots and the 3843-mode set, 31.5 MB for 200 knots
- * and n in [-55, 55]).  The call allocates nothing: no hipMalloc / hipFree
+ * tiles: 5.4 MB for 120 knots and the 3843-mode set, 18 MB for 200 knots
+ * and n in [-55, 55]; FACT64 its FP64 tiles, 11 MB and 32 MB).  The call allocates nothing: no hipMalloc / hipFree
  * and no allocating library routine.  Every call on one GPU shares that
  * GPU's array, which keeps the last call's contents: a stage-2 call sums
  * with what the last stage-1 call on the same GPU prepared, and calls that

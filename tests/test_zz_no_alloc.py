@@ -64,11 +64,11 @@ def test_noalloc_matches_workspace_entry(gpu, precision):
 
 def test_noalloc_refuses_beyond_scratch(gpu):
     """A mixed16 call whose coefficient tiles exceed the static array is refused
-    before anything runs (400 knots of the 3843-mode set: ~36 MB)."""
+    before anything runs (800 knots of the 3843-mode set: ~36 MB)."""
     _, m, n = synthetic.mode_set("m_nonneg_30")
     k = np.zeros_like(m)
     plan = ModePlan(m, k, n)
-    L, N = 400, 1000
+    L, N = 800, 1000
     assert plan.workspace_bytes(L, N, "mixed16") > _lib.FEW_NOALLOC_SCRATCH_BYTES
     buf = torch.zeros(16, dtype=torch.float64, device=gpu)
     p = buf.data_ptr()
