@@ -390,10 +390,24 @@ __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sP
             sincos_turns_circle(reduce_turns(ur), zsn, zcs);
         }
         // lane group g needs v_n for n = n0 + 16 kb + 4 g + t, t = 0..3 (scaled by 256)
-        double sn, cs;
-        sincos_turns_circle(reduce_turns((double)(P.tn0 + 4 * g) * ur), sn, cs);
         const double zr = zcs, zi = -zsn;  // z = exp(-i Phi_r)
-        double vr = 256.0 * cs, vi = -256.0 * sn;
+        double z2r, z2i, z4r, z4i, z8r, z8i, z16r, z16i;  // powers of z by squaring
+        cmul(zr, zi, zr, zi, z2r, z2i);
+        cmul(z2r, z2i, z2r, z2i, z4r, z4i);
+        cmul(z4r, z4i, z4r, z4i, z8r, z8i);
+        cmul(z8r, z8i, z8r, z8i, z16r, z16i);
+        double vr, vi;
+        if (P.tn0 == 0) {
+            // v_{4g} = z^{4g} = (z^4 if g odd) (z^8 if g >= 2): no second sincos
+            const double ar = (g & 1) ? 256.0 * z4r : 256.0, ai = (g & 1) ? 256.0 * z4i : 0.0;
+            const double br = (g & 2) ? z8r : 1.0, bi = (g & 2) ? z8i : 0.0;
+            cmul(ar, ai, br, bi, vr, vi);
+        } else {
+            double sn, cs;
+            sincos_turns_circle(reduce_turns((double)(P.tn0 + 4 * g) * ur), sn, cs);
+            vr = 256.0 * cs;
+            vi = -256.0 * sn;
+        }
         V.vr[c][0] = (f16v_t)vr;
         V.vi[c][0] = (f16v_t)vi;
 #pragma unroll
@@ -405,15 +419,8 @@ __device__ __forceinline__ void f16_setup(const FactPlanDev &P, const double *sP
             V.vr[c][t2] = (f16v_t)vr;
             V.vi[c][t2] = (f16v_t)vi;
         }
-        double wr = zr, wi = zi, tr, ti;
-#pragma unroll
-        for (int k2 = 0; k2 < 4; k2++) {  // z^16 by squaring
-            cmul(wr, wi, wr, wi, tr, ti);
-            wr = tr;
-            wi = ti;
-        }
-        V.zr16[c] = (f16v_t)wr;
-        V.zi16[c] = (f16v_t)wi;
+        V.zr16[c] = (f16v_t)z16r;
+        V.zi16[c] = (f16v_t)z16i;
     }
 }
 
