@@ -46,9 +46,16 @@ def compile_flags() -> list[str]:
     ]
 
 
+# Per-source scheduler choice: the split-FP16 sum is bound by VALU issue, and the
+# iterative ILP strategy's instruction order measured 1.5-2 % faster on the
+# 256-waveform batch than the default (same instructions, same results).
+SOURCE_FLAGS = {"modesum_fact16.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
+
+
 def _compile(src: str, verbose: bool) -> str:
     obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *compile_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *compile_flags(), *SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src),
+           "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
