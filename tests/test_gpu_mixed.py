@@ -1,6 +1,6 @@
 """GPU accuracy of the mixed-precision factorised (MFMA) mode sum.
 
-Bar (BASELINE metric): rel-L2 < 1e-6 against the FP64 oracle; the
+Bar (the headline metric's): rel-L2 < 1e-6 against the FP64 oracle; the
 factorised sum is also compared with the FP64 GPU path on every sample.
 """
 
