@@ -164,19 +164,27 @@ __device__ void fact16_prep_interval_cached(const FactPlanDev &P, const double *
     }
 }
 
+// Cells per thread of an interval's tile (the kernels are specialised on it, so
+// the common small tiles get the registers of their own path: two workgroups
+// per CU for up to two cells per thread); 0: more than 4, gathered twice.
+inline int f16_prep_cpt(int KB, int RB)
+{
+    const int cpt = (2 * RB * 16 * KB + F16_PREP_THREADS - 1) / F16_PREP_THREADS;
+    return cpt <= 4 ? (cpt < 1 ? 1 : cpt) : 0;
+}
+
+template <int CPT>
 __device__ void fact16_prep_interval(const FactPlanDev &P, const double *interp, const double2 *ylm,
                                      const double *h_t, int L, int j, unsigned char *tile, int KB,
                                      int RB)
 {
+    if constexpr (CPT > 0) {
+        return fact16_prep_interval_cached<CPT>(P, interp, ylm, h_t, L, j, tile, KB, RB);
+    }
     __shared__ double s_red[F16_PREP_THREADS / 64][4];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int ncolc = 16 * KB;         // complex columns incl. padding
     const int ncell = 2 * RB * ncolc;  // complex rows r incl. padding x columns
-    const int cpt = (ncell + F16_PREP_THREADS - 1) / F16_PREP_THREADS;  // uniform
-    if (cpt <= 1) return fact16_prep_interval_cached<1>(P, interp, ylm, h_t, L, j, tile, KB, RB);
-    if (cpt == 2) return fact16_prep_interval_cached<2>(P, interp, ylm, h_t, L, j, tile, KB, RB);
-    if (cpt == 3) return fact16_prep_interval_cached<3>(P, interp, ylm, h_t, L, j, tile, KB, RB);
-    if (cpt == 4) return fact16_prep_interval_cached<4>(P, interp, ylm, h_t, L, j, tile, KB, RB);
     const double hw = h_t[j + 1] - h_t[j];
     double mx[4] = {0.0, 0.0, 0.0, 0.0};
     for (int ci = tid; ci < ncell; ci += F16_PREP_THREADS) {
@@ -206,18 +214,28 @@ __device__ void fact16_prep_interval(const FactPlanDev &P, const double *interp,
     }
 }
 
-__global__ __launch_bounds__(F16_PREP_THREADS) void fact16_prep_whole_kernel(Fact16PrepArgs a)
+// Two resident workgroups per CU where at most two cells per thread are held.
+#define F16_PREP_BOUNDS(CPT) __launch_bounds__(F16_PREP_THREADS, (CPT) >= 1 && (CPT) <= 2 ? 2 : 1)
+
+template <int CPT>
+__global__ F16_PREP_BOUNDS(CPT) void fact16_prep_whole_kernel(Fact16PrepArgs a)
 {
     const int j = blockIdx.x;
-    fact16_prep_interval(a.plan, a.interp, a.ylm, a.h_t, a.L, j, a.tiles + (long)j * a.tile_bytes,
-                         a.KB, a.RB);
+    fact16_prep_interval<CPT>(a.plan, a.interp, a.ylm, a.h_t, a.L, j,
+                              a.tiles + (long)j * a.tile_bytes, a.KB, a.RB);
 }
 
 int launch_fact16_prep(const Fact16PrepArgs &a, hipStream_t stream)
 {
     if (a.L < 2) return FEW_OK;
-    hipLaunchKernelGGL(fact16_prep_whole_kernel, dim3(a.L - 1), dim3(F16_PREP_THREADS), 0, stream,
-                       a);
+    const dim3 grid(a.L - 1), block(F16_PREP_THREADS);
+    switch (f16_prep_cpt(a.KB, a.RB)) {
+    case 1: hipLaunchKernelGGL(fact16_prep_whole_kernel<1>, grid, block, 0, stream, a); break;
+    case 2: hipLaunchKernelGGL(fact16_prep_whole_kernel<2>, grid, block, 0, stream, a); break;
+    case 3: hipLaunchKernelGGL(fact16_prep_whole_kernel<3>, grid, block, 0, stream, a); break;
+    case 4: hipLaunchKernelGGL(fact16_prep_whole_kernel<4>, grid, block, 0, stream, a); break;
+    default: hipLaunchKernelGGL(fact16_prep_whole_kernel<0>, grid, block, 0, stream, a); break;
+    }
     FEW_HIP(hipGetLastError());
     return FEW_OK;
 }
@@ -986,14 +1004,15 @@ void batch_fact16s_kernel(BatchDev B, BatchRunDev r, BatchPtrs p, const unsigned
     }
 }
 
-__global__ __launch_bounds__(F16_PREP_THREADS) void batch_fact16_prep_whole_kernel(
+template <int CPT>
+__global__ F16_PREP_BOUNDS(CPT) void batch_fact16_prep_whole_kernel(
     BatchDev B, BatchPtrs p, unsigned char *tiles, long tile_bytes, int KB, int RB)
 {
     const int gi = blockIdx.x;
     const int b = upper_index(B.seg_prefix, B.B, gi);
     const WaveDesc d = B.desc[b];
     const int j = gi - B.seg_prefix[b];
-    fact16_prep_interval(batch_wave_plan(B, d), p.interp + d.interp_off,
+    fact16_prep_interval<CPT>(batch_wave_plan(B, d), p.interp + d.interp_off,
                          reinterpret_cast<const double2 *>(p.ylm + d.ylm_off), p.t + d.t_off, d.L,
                          j, tiles + (long)gi * tile_bytes, KB, RB);
 }
@@ -1140,9 +1159,19 @@ int launch_batch_fact16(const BatchDev &B, const BatchRunDev &r, const BatchPtrs
 {
     if (stage & 1) {
         if (B.total_intervals > 0) {
-            hipLaunchKernelGGL(batch_fact16_prep_whole_kernel, dim3(B.total_intervals),
-                               dim3(F16_PREP_THREADS), 0, stream, B, p, tiles, tile_bytes, KB,
-                               B.uni.RB);
+            const dim3 grid(B.total_intervals), block(F16_PREP_THREADS);
+            const int RB = B.uni.RB;
+            switch (f16_prep_cpt(KB, RB)) {
+#define F16_BATCH_PREP(C)                                                                        \
+    hipLaunchKernelGGL(batch_fact16_prep_whole_kernel<C>, grid, block, 0, stream, B, p, tiles,   \
+                       tile_bytes, KB, RB)
+            case 1: F16_BATCH_PREP(1); break;
+            case 2: F16_BATCH_PREP(2); break;
+            case 3: F16_BATCH_PREP(3); break;
+            case 4: F16_BATCH_PREP(4); break;
+            default: F16_BATCH_PREP(0); break;
+#undef F16_BATCH_PREP
+            }
             FEW_HIP(hipGetLastError());
         }
     }
