@@ -56,11 +56,19 @@ SOURCE_FLAGS = {"modesum_fact16.hip": _ILP, "modesum_fact.hip": _ILP}
 
 def _compile(src: str, verbose: bool) -> str:
     obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc(), *compile_flags(), *SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src),
-           "-o", obj]
+    base = [hipcc(), *compile_flags()]
+    tail = ["-c", os.path.join(CSRC, src), "-o", obj]
+    extra = SOURCE_FLAGS.get(src, [])
+    cmd = [*base, *extra, *tail]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0 and extra:
+        # the scheduler option is a tuning choice: a compiler that rejects it
+        # (or fails with it) builds the source with its default schedule
+        print(f"note: {src}: {' '.join(extra)} failed, building with the default schedule",
+              file=sys.stderr, flush=True)
+        r = subprocess.run([*base, *tail], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
     return obj
